@@ -1,0 +1,168 @@
+/*
+ * vpbs.h — C ABI of libvpbs, the MI355X (gfx950) Ewald-split Biot–Savart
+ * library. It replaces, behind VortexPasta's backend plug-in interface, the
+ * short-range backend (CellListsBackend) and the long-range backend
+ * (NonuniformFFTsBackend) of BiotSavart.compute_on_nodes!.
+ *
+ * All array arguments are plain pointers with explicit sizes; vectors are
+ * structure-of-arrays, component-major: a field of n 3-vectors is 3n doubles
+ * laid out as x[0..n), y[0..n), z[0..n) — the memory layout of the
+ * reference's StructVector{Vec3{T}} (src/BiotSavart/pointdata.jl:21-39).
+ * Inputs and outputs are always FP64; the internal arithmetic is chosen by
+ * vpbs_params.precision.
+ *
+ * Ownership: the caller owns every host array and only lends it for the
+ * duration of a call. The library owns all device buffers, FFT plans and
+ * streams, and reuses them across calls. Every call returns VPBS_OK (0) or a
+ * negative error code; vpbs_last_error() gives the message. A context is
+ * bound to one GPU and calls on one context must not overlap.
+ */
+#ifndef VPBS_H
+#define VPBS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VPBS_ABI_VERSION 1
+
+/* error codes */
+#define VPBS_OK 0
+#define VPBS_ERR_INVALID -1  /* invalid argument or parameter */
+#define VPBS_ERR_HIP -2      /* HIP runtime error (allocation, launch, copy) */
+#define VPBS_ERR_FFT -3      /* rocFFT error */
+#define VPBS_ERR_STATE -4    /* call out of order (e.g. compute before set_points) */
+
+/* precision of the bulk arithmetic */
+#define VPBS_F64 0 /* plain FP64 path */
+#define VPBS_F32 1 /* FP32 bulk, FP64 node differencing / LIA / local self terms */
+
+/* vpbs_compute flags */
+#define VPBS_VELOCITY 1u
+#define VPBS_STREAMFUNCTION 2u
+#define VPBS_LIA_OFF 4u        /* LIA = Val(false) */
+#define VPBS_LIA_ONLY 8u       /* LIA = Val(:only) */
+#define VPBS_NO_SHORTRANGE 16u /* shortrange = false */
+#define VPBS_NO_LONGRANGE 32u  /* longrange = false */
+#define VPBS_NEIGHBOUR_STATS 64u /* record per-node accepted-pair counts and hashes */
+
+/*
+ * Parameters. Mirrors ParamsBiotSavart (src/BiotSavart/params.jl:326-376)
+ * with the Gaussian splitting already resolved (splitting/gaussian.jl:109-200):
+ * alpha, rcut and Ns are the derived values.
+ */
+typedef struct vpbs_params {
+    int32_t abi_version;   /* must be VPBS_ABI_VERSION */
+    int32_t precision;     /* VPBS_F64 or VPBS_F32 */
+    double Gamma;          /* circulation Γ */
+    double a;              /* core size */
+    double Delta;          /* core parameter Δ */
+    double Ls[3];          /* periods */
+    double alpha;          /* Ewald splitting parameter */
+    double rcut;           /* short-range cut-off distance */
+    int32_t Ns[3];         /* Fourier grid (retained modes follow longrange.jl:556-583) */
+    int32_t nquad;         /* quadrature points per segment */
+    int32_t nsubdiv;       /* cell-list subdivisions M (CellListsBackend(M)) */
+    int32_t nufft_m;       /* NUFFT kernel half-support (HalfSupport(m)) */
+    double nufft_sigma;    /* NUFFT oversampling factor σ */
+    int32_t avoid_explicit_erf; /* params.jl:289-298 */
+    int32_t truncate_spherical; /* params.jl:243-246 */
+    int32_t device;        /* HIP device ordinal */
+    int32_t reserved;
+} vpbs_params;
+
+/* Per-stage GPU times of the last vpbs_compute, in milliseconds (HIP events). */
+typedef struct vpbs_timings {
+    double total;        /* whole compute, host wall-clock (includes waits) */
+    double local_term;   /* LIA kernel */
+    double cell_build;   /* source + target counting sorts */
+    double pair;         /* short-range pair kernel */
+    double self;         /* self-interaction removal */
+    double lr_sort;      /* NUFFT tile sort of quadrature points */
+    double spread;       /* type-1 spreading (all 3 components) */
+    double fft_forward;  /* rocFFT r2c, 3 components */
+    double fourier;      /* fused deconvolution / Green / curl / Ewald kernel */
+    double fft_backward; /* rocFFT c2r, all components */
+    double interp;       /* type-2 interpolation */
+    double merge;        /* SR + LR sum */
+    int64_t pair_candidates; /* unused (reserved) */
+    int64_t grid[3];     /* oversampled NUFFT grid */
+    int64_t cells[3];    /* cell-list cells per direction */
+} vpbs_timings;
+
+typedef struct vpbs_ctx vpbs_ctx;
+
+/*
+ * Create a context: validates parameters (rcut against the cell-list limit
+ * M/(2M+1)·L, CellLists.jl:115,143-147; 2·rcut ≤ min(L), params.jl:105-106),
+ * builds FFT plans and deconvolution tables.
+ * Replaces init_cache_short (shortrange/backends/cell_lists.jl:103-115) and
+ * init_cache_long_ewald (longrange/backends/nonuniformffts.jl:153-171).
+ */
+int vpbs_create(vpbs_ctx **ctx, const vpbs_params *params);
+
+/* Release every resource of the context. */
+void vpbs_destroy(vpbs_ctx *ctx);
+
+/* Message of the last failed call on ctx (or of the last failed vpbs_create when ctx is NULL). */
+const char *vpbs_last_error(const vpbs_ctx *ctx);
+
+/*
+ * Upload point data (host pointers). Replaces the H2D copies of
+ * do_shortrange!/do_longrange! (src/BiotSavart/BiotSavart.jl:425-430,483-499)
+ * of the PointData built by add_point_charges! (pointdata_impl.jl:121-188):
+ *   nodes[3*Np]    folded filament nodes
+ *   idx_prev[Np]   0-based index of the previous node on the same filament
+ *   sp, spp[3*Np]  s′ and s″ on nodes
+ *   dlm, dlp[Np]   sub-segment lengths δ⁻, δ⁺
+ *   points[3*Nq], charges[3*Nq] quadrature points (folded) and w·Δt·s′;
+ *   Nq must equal nquad·Np, points of segment i at nquad·i .. nquad·i+nquad-1.
+ */
+int vpbs_set_points(vpbs_ctx *ctx, int64_t Np, const double *nodes, const int32_t *idx_prev, const double *sp,
+                    const double *spp, const double *dlm, const double *dlp, int64_t Nq, const double *points,
+                    const double *charges);
+
+/* Same as vpbs_set_points with device pointers (device-resident pipelines; device-to-device copies). */
+int vpbs_set_points_device(vpbs_ctx *ctx, int64_t Np, const double *nodes, const int32_t *idx_prev,
+                           const double *sp, const double *spp, const double *dlm, const double *dlp, int64_t Nq,
+                           const double *points, const double *charges);
+
+/*
+ * Compute velocity and/or streamfunction on nodes: the device part of
+ * _compute_on_nodes! (BiotSavart.jl:515-612) — LIA term, short-range pair sum,
+ * self-interaction removal and long-range NUFFT path, summed. The host-side
+ * background-vorticity correction of ψ (BiotSavart.jl:255-286) and
+ * lia_segment_fraction integrals stay with the caller.
+ * integration_cutoff: NaN for the default exp(Δ)a/2 / exp(Δ-1)a/2 cut-offs
+ * (local_integrals.jl:115-125). v_out / psi_out: host arrays of 3*Np doubles,
+ * or NULL to leave the result on the device (see vpbs_get_outputs).
+ */
+int vpbs_compute(vpbs_ctx *ctx, uint32_t flags, double integration_cutoff, double *v_out, double *psi_out);
+
+/* Copy the outputs of the last vpbs_compute to host arrays (either may be NULL). */
+int vpbs_get_outputs(vpbs_ctx *ctx, double *v_out, double *psi_out);
+
+/* Device pointers of the last outputs (3*Np doubles each; NULL if not computed). Valid until the next call. */
+int vpbs_output_device_ptrs(vpbs_ctx *ctx, const double **v_dev, const double **psi_dev);
+
+/*
+ * Per-node short-range neighbour statistics of the last compute run with
+ * VPBS_NEIGHBOUR_STATS: number of accepted (node, quadrature point) pairs
+ * (r² ≤ rcut²) and the wrapping sum of splitmix64(j) over accepted j.
+ * Bit-exact comparison target for the cell-list neighbour sets.
+ */
+int vpbs_neighbour_stats(vpbs_ctx *ctx, int64_t *counts, uint64_t *hashes);
+
+/* Per-stage timings of the last vpbs_compute. */
+int vpbs_get_timings(vpbs_ctx *ctx, vpbs_timings *t);
+
+/* Oversampled grid and cell counts chosen by the context (for reporting). */
+int vpbs_get_sizes(vpbs_ctx *ctx, int64_t *grid3, int64_t *cells3);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VPBS_H */

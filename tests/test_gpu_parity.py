@@ -1,0 +1,203 @@
+"""GPU parity: libvpbs (through its C ABI) against the FP64 CPU oracle.
+
+Bar (from the reference's own CPU-vs-device parity test, test/ring_collision.jl:153-156):
+plain FP64 path ≤ 1e-12 relative L2 against the oracle on identical point data,
+and bit-identical short-range neighbour sets (per-node accepted-pair counts and
+order-independent hashes). The FP32-bulk path is checked against the FP64 path
+with the tolerances stated below.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2 * math.pi
+Q = ("velocity", "streamfunction")
+TOL64 = 1e-12
+
+
+def _params(vp, T=np.float64, Ns=64, beta=3.5, Ls=TWO_PI, **kws):
+    spl = vp.GaussianSplitting.create(Ls=Ls, β=beta, Ns=Ns)
+    return vp.ParamsBiotSavart.build(T, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl, **kws)
+
+
+def _gpu(vp, params, fs, fields=Q, **kws):
+    cache = vp.init_cache(params)
+    out = {k: [np.zeros((len(f), 3)) for f in fs] for k in fields}
+    vp.compute_on_nodes_(out, cache, fs, **kws)
+    lin = {k: np.concatenate(v, axis=0).T for k, v in out.items()}
+    return cache, lin
+
+
+def _oracle_kw(kws):
+    m = {"LIA": kws.get("LIA", True), "shortrange": kws.get("shortrange", True),
+         "longrange_on": kws.get("longrange", True)}
+    if "integration_cutoff" in kws:
+        m["integration_cutoff"] = kws["integration_cutoff"]
+    return m
+
+
+def _check(vp, O, params, fs, fields=Q, tol=TOL64, **kws):
+    cache, got = _gpu(vp, params, fs, fields, **kws)
+    want = O.compute(cache.pointdata, params, fields, **_oracle_kw(kws))
+    for k in fields:
+        want_k = want[k].copy()
+        if k == "streamfunction" and kws.get("shortrange", True) and kws.get("LIA", True) != "only":
+            pass  # background correction applied by both (zero for closed filaments)
+        err = rel_l2(got[k], want_k)
+        print(f"  {k}: rel L2 vs oracle = {err:.3e}", flush=True)
+        assert err <= tol, (k, err)
+    cache.close()
+    return got
+
+
+@pytest.fixture(scope="module")
+def cfg1(vp):
+    return vp.tangles.config_tangle(1)
+
+
+@pytest.fixture(scope="module")
+def cfg2(vp):
+    return vp.tangles.config_tangle(2)
+
+
+def test_cfg1_ring_fp64(vp, O, cfg1):
+    """One ring, 64 nodes, 64³ grid, β = 3.5 (configuration 1)."""
+    _check(vp, O, _params(vp), cfg1)
+
+
+@pytest.mark.parametrize("variant", [
+    dict(LIA="only"), dict(LIA=False), dict(longrange=False), dict(shortrange=False),
+    dict(integration_cutoff=1e-3),
+])
+def test_cfg1_variants(vp, O, cfg1, variant):
+    _check(vp, O, _params(vp), cfg1, **variant)
+
+
+def test_cfg1_no_avoid_explicit_erf(vp, O, cfg1):
+    _check(vp, O, _params(vp, avoid_explicit_erf=False), cfg1)
+
+
+def test_cfg1_truncate_spherical(vp, O, cfg1):
+    _check(vp, O, _params(vp, longrange_truncate_spherical=True), cfg1)
+
+
+def test_cfg2_random_rings_fp64(vp, O, cfg2):
+    """64 random rings (≈16k nodes), 128³ grid (configuration 2): values and neighbour sets."""
+    params = _params(vp, Ns=128)
+    cache, got = _gpu(vp, params, cfg2, neighbour_stats=True)
+    pd = cache.pointdata
+    print(f"  Np = {pd.Np}", flush=True)
+    assert 14000 < pd.Np < 19000
+    want = O.compute(pd, params, Q, neighbour_stats=True)
+    for k in Q:
+        err = rel_l2(got[k], want[k])
+        print(f"  {k}: rel L2 vs oracle = {err:.3e}", flush=True)
+        assert err <= TOL64
+    counts, hashes = cache.ctx.neighbour_stats()
+    oc, oh = want["stats"]
+    assert np.array_equal(counts, oc) and np.array_equal(hashes, oh)
+    bc, bh = O.neighbour_pairs_bruteforce(pd, params)
+    assert np.array_equal(counts, bc) and np.array_equal(hashes, bh)
+    print(f"  accepted pairs: {counts.sum()} ({counts.mean():.1f} per node)", flush=True)
+    cache.close()
+
+
+def test_trefoil_noncubic_box(vp, O):
+    """Non-cubic periods and odd grid sizes (test/trefoil.jl:393-396)."""
+    R = math.pi / 3
+    S = vp.define_curve(vp.TrefoilKnot(), scale=R, translate=R)
+    fs = [vp.ClosedFilament(S(np.arange(30) / 30), np.zeros(3))]
+    Ls = (1.5 * math.pi, 1.5 * math.pi, 2 * math.pi)
+    spl = vp.GaussianSplitting.create(Ls=Ls, α=3.0, rcut=1.0, Ns=(45, 45, 61))
+    params = vp.ParamsBiotSavart.build(np.float64, Γ=7.4, a=1e-5, splitting=spl)
+    _check(vp, O, params, fs)
+
+
+def test_unclosed_lines_background_vorticity(vp, O):
+    """Periodic lines with a net end-to-end offset: ψ gets the background correction (BiotSavart.jl:255-286)."""
+    fs = vp.tangles.kelvin_lines(2, TWO_PI / 48, seed=5)
+    fs = [fs[0]]  # one line along x: non-zero mean vorticity
+    params = _params(vp, Ns=48)
+    cache, got = _gpu(vp, params, fs)
+    want = O.compute(cache.pointdata, params, Q)
+    for k in Q:
+        assert rel_l2(got[k], want[k]) <= TOL64
+    assert np.abs(O.background_vorticity_psi(cache.pointdata, params)).max() > 0
+    cache.close()
+
+
+def test_fp32_bulk_path(vp, O, cfg2):
+    """FP32 bulk arithmetic against the FP64 oracle.
+
+    Tolerances: 2e-6 relative L2 for velocity and streamfunction (the NUFFT itself
+    is accurate to ~4e-7 at m = 4, σ = 1.5; FP32 rounding of the FFT and of the
+    compensated sums stays below that).
+    """
+    params = _params(vp, T=np.float32, Ns=128)
+    cache, got = _gpu(vp, params, cfg2, neighbour_stats=True)
+    p64 = _params(vp, Ns=128)
+    want = O.compute(cache.pointdata, p64, Q, neighbour_stats=True)
+    for k in Q:
+        err = rel_l2(got[k], want[k])
+        print(f"  fp32 {k}: rel L2 vs FP64 oracle = {err:.3e}", flush=True)
+        assert err < 2e-6, (k, err)
+    # the FP32 path tests r² ≤ rcut² in FP64: same pair set as the oracle's
+    # local-segment-excluding pass (avoid_explicit_erf = false semantics)
+    p64b = _params(vp, Ns=128, avoid_explicit_erf=False)
+    ref = O.compute(cache.pointdata, p64b, ("velocity",), longrange_on=False, neighbour_stats=True)["stats"]
+    counts, hashes = cache.ctx.neighbour_stats()
+    assert np.array_equal(counts, ref[0]) and np.array_equal(hashes, ref[1])
+    cache.close()
+
+
+def test_deterministic_repeat(vp, cfg2):
+    """Two evaluations of the same input are bit-identical (deterministic sorts, no float atomics)."""
+    params = _params(vp, Ns=128)
+    cache = vp.init_cache(params)
+    a = vp.velocity_on_nodes(cache, cfg2)
+    b = vp.velocity_on_nodes(cache, cfg2)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    cache.close()
+
+
+def test_resize_between_calls(vp, O, cfg1, cfg2):
+    """One context, point sets of different sizes in a row."""
+    params = _params(vp, Ns=128)
+    cache = vp.init_cache(params)
+    for fs in (cfg2, cfg1, cfg2[:5]):
+        out = {k: [np.zeros((len(f), 3)) for f in fs] for k in Q}
+        vp.compute_on_nodes_(out, cache, fs)
+        want = O.compute(cache.pointdata, params, Q)
+        for k in Q:
+            assert rel_l2(np.concatenate(out[k]).T, want[k]) <= TOL64
+    cache.close()
+
+
+def test_error_paths(vp, cfg1):
+    params = _params(vp)
+    ctx = vp._lib.Context(params)
+    with pytest.raises(vp._lib.VpbsError, match="before vpbs_set_points"):
+        ctx.compute(vp._lib.VELOCITY)
+    with pytest.raises(vp._lib.VpbsError, match="VPBS_VELOCITY"):
+        ctx.set_points(vp.add_point_charges(cfg1, params))
+        ctx.compute(0)
+    ctx.close()
+    bad = vp._lib.make_params(params)
+    bad.rcut = 3.0  # above M/(2M+1)·L
+    import ctypes
+
+    h = ctypes.c_void_p()
+    L = vp._lib.load()
+    assert L.vpbs_create(ctypes.byref(h), ctypes.byref(bad)) == -1
+    assert b"larger than that allowed" in L.vpbs_last_error(None)
+    bad = vp._lib.make_params(params)
+    bad.abi_version = 99
+    assert L.vpbs_create(ctypes.byref(h), ctypes.byref(bad)) == -1
+    with pytest.raises(ValueError):
+        cache = vp.init_cache(params)
+        vp.compute_on_nodes_({"velocity": [np.zeros((3, 3))]}, cache, cfg1)

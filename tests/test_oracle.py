@@ -1,0 +1,246 @@
+"""Pins the FP64 CPU oracle against closed forms, brute force and invariances.
+
+Each check follows a reference test (cited) and is chosen so that a plausible
+mistake (dropped term, wrong sign, wrong normalisation) fails it; the
+planted-mistake tests at the end show that explicitly by breaking the oracle
+on purpose and checking that the invariance test notices.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+TWO_PI = 2 * math.pi
+
+
+def ring_setup(vp, N, quad=4, R=math.pi / 3, Γ=2.4, a=1e-6, Δ=0.25):
+    f = vp.filaments.init(vp.define_curve(vp.Ring(), scale=R, translate=math.pi), N)
+    spl = vp.GaussianSplitting.create(Ls=TWO_PI, β=3.5, Ns=64)
+    p = vp.ParamsBiotSavart.build(np.float64, Γ=Γ, a=a, Δ=Δ, splitting=spl, quadrature=vp.GaussLegendre(quad))
+    return [f], p, vp.add_point_charges([f], p)
+
+
+def ring_errors(vp, O, N):
+    R, Γ, a, Δ = math.pi / 3, 2.4, 1e-6, 0.25
+    fs, p, pd = ring_setup(vp, N)
+    out = O.compute_open_domain(pd, p, ("velocity", "streamfunction"))
+    v, psi = out["velocity"], out["streamfunction"]
+    U = np.linalg.norm(v.mean(axis=1))
+    U_ex = Γ / (4 * math.pi * R) * (math.log(8 * R / a) - Δ)
+    t = pd.sp / np.linalg.norm(pd.sp, axis=0)
+    ps = (psi * t).sum(axis=0).mean()
+    ps_ex = Γ / (2 * math.pi) * (math.log(8 * R / a) - (1 + Δ))
+    E = O.energy_from_streamfunction(pd, psi, p, open_domain=True)
+    E_ex = (Γ ** 2 * R / 2) * (math.log(8 * R / a) - (Δ + 1))
+    perp = np.linalg.norm(psi - (psi * t).sum(0) * t, axis=0).max() / abs(ps)
+    return abs(U - U_ex) / U_ex, abs(ps - ps_ex) / ps_ex, abs(E - E_ex) / E_ex, perp, v
+
+
+def test_ring_closed_forms(vp, O):
+    """Vortex ring velocity / streamfunction / energy (test/ring.jl:35-40,64-121)."""
+    eU32, eP32, eE32, perp, v = ring_errors(vp, O, 32)
+    assert eU32 < 1e-4  # ring.jl:86 (quintic, GL(4), N = 32)
+    assert perp < 1e-10  # ψ parallel to the tangent (ring.jl:101-107)
+    # all nodes move with the same velocity along the ring axis
+    assert np.abs(v[0]).max() < 1e-12 and np.abs(v[1]).max() < 1e-12
+    assert np.std(v[2]) / np.mean(v[2]) < 1e-12
+    # Without lia_segment_fraction the local-segment error of ψ and E is O(δ²):
+    eU64, eP64, eE64, _, _ = ring_errors(vp, O, 64)
+    assert eP64 < 2e-4 and eE64 < 2e-4
+    assert 3.5 < eP32 / eP64 < 4.5 and 3.5 < eE32 / eE64 < 4.5 and 3.5 < eU32 / eU64 < 4.5
+
+
+def test_lia_decomposition(vp, O):
+    """LIA + non-LIA = total (test/ring.jl:134-140), periodic Ewald path."""
+    fs = vp.tangles.random_rings(6, 0.5, 1.0, TWO_PI / 64, seed=3)
+    spl = vp.GaussianSplitting.create(Ls=TWO_PI, β=3.5, Ns=48)
+    p = vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, splitting=spl)
+    pd = vp.add_point_charges(fs, p)
+    q = ("velocity", "streamfunction")
+    tot = O.compute(pd, p, q)
+    only = O.compute(pd, p, q, LIA="only")
+    off = O.compute(pd, p, q, LIA=False)
+    for k in q:
+        assert rel_l2(only[k] + off[k], tot[k]) < 1e-14
+
+
+def _numpy_bruteforce_stats(O, pd, p):
+    """Independent numpy enumeration of accepted pairs (r² ≤ rcut², min-image)."""
+    L = np.array(p.Ls)[:, None]
+    rc2 = p.rcut * p.rcut
+    counts = np.zeros(pd.Np, dtype=np.int64)
+    hashes = np.zeros(pd.Np, dtype=np.uint64)
+    H = np.array([O.lib().or_pair_hash(j) for j in range(pd.Nq)], dtype=np.uint64)
+    for i in range(pd.Np):
+        r = pd.nodes[:, i : i + 1] - pd.points
+        r = np.where(r >= L / 2, r - L, r)
+        r = np.where(r < -L / 2, r + L, r)
+        r2 = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]
+        acc = r2 <= rc2
+        counts[i] = acc.sum()
+        with np.errstate(over="ignore"):
+            hashes[i] = np.sum(H[acc], dtype=np.uint64)
+    return counts, hashes
+
+
+def test_neighbour_sets_celllists_equal_bruteforce(vp, O):
+    """Cell lists find exactly the brute-force pair set (test/cell_lists.jl:43-60,218-239)."""
+    fs = vp.tangles.random_rings(24, 0.3, 0.8, TWO_PI / 96, seed=11)
+    spl = vp.GaussianSplitting.create(Ls=TWO_PI, β=3.5, Ns=48)
+    p = vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, splitting=spl)
+    pd = vp.add_point_charges(fs, p)
+    cl = O.compute(pd, p, ("velocity",), longrange_on=False, neighbour_stats=True)["stats"]
+    bf = O.neighbour_pairs_bruteforce(pd, p)
+    npbf = _numpy_bruteforce_stats(O, pd, p)
+    assert np.array_equal(cl[0], bf[0]) and np.array_equal(cl[1], bf[1])
+    assert np.array_equal(bf[0], npbf[0]) and np.array_equal(bf[1], npbf[1])
+    assert cl[0].sum() > 10 * pd.Np  # the test is not vacuous
+    # sums agree to roundoff (cell_lists.jl: rtol 1e-13)
+    a = O.compute(pd, p, ("velocity",), longrange_on=False, use_celllist=True)["velocity"]
+    b = O.compute(pd, p, ("velocity",), longrange_on=False, use_celllist=False)["velocity"]
+    assert rel_l2(a, b) < 1e-13
+
+
+def trefoil_params(vp, rcut=None, α=None, rtol=1e-6, **kws):
+    Ls = (1.5 * math.pi, 1.5 * math.pi, 2 * math.pi)  # test/trefoil.jl:395
+    if rcut is not None:
+        spl = vp.GaussianSplitting.create(Ls=Ls, rcut=rcut, rtol=rtol)
+    else:
+        spl = vp.GaussianSplitting.create(Ls=Ls, α=α, rtol=rtol)
+    return vp.ParamsBiotSavart.build(np.float64, Γ=7.4, a=1e-5, splitting=spl, **kws)
+
+
+def trefoil_fs(vp, N=30):
+    R = math.pi / 3
+    S = vp.define_curve(vp.TrefoilKnot(), scale=R, translate=R)
+    return [vp.ClosedFilament(S(np.arange(N) / N), np.zeros(3))]
+
+
+def test_exact_sum_against_numpy(vp, O):
+    """Type-1 exact sum and retained-mode set against a direct numpy evaluation."""
+    fs = trefoil_fs(vp, 12)
+    p = trefoil_params(vp, α=3.0)
+    pd = vp.add_point_charges(fs, p)
+    p2 = vp.ParamsBiotSavart.build(np.float64, Γ=7.4, a=1e-5, splitting=vp.GaussianSplitting.create(
+        Ls=p.Ls, α=p.α, rcut=p.rcut, Ns=(10, 9, 12)))
+    Ns = np.array(p2.Ns, dtype=np.int32)
+    Ls = np.array(p2.Ls)
+    Nkx = Ns[0] // 2 + 1
+    uh = np.zeros((3, Ns[2], Ns[1], Nkx, 2))
+    pref = p2.Γ / np.prod(Ls)
+    O.lib().or_exact_type1(pd.Nq, O._p(O._c(pd.points)), O._p(O._c(pd.charges)), O._p(Ls), O._p(Ns, O._ip), pref,
+                           O._p(uh))
+    uh = uh[..., 0] + 1j * uh[..., 1]
+    kxm, kym, kzm = O.mode_limits(p2.Ns)
+    assert (kxm, kym, kzm) == (4, 4, 5)  # N even: N/2-1 ; N odd: (N-1)/2 (longrange.jl:556-583)
+    for kx, ky, kz in [(0, 0, 0), (1, -2, 3), (4, 4, -5), (2, -4, 5)]:
+        k = 2 * np.pi * np.array([kx, ky, kz]) / Ls
+        want = pref * (pd.charges * np.exp(-1j * (k @ pd.points))).sum(axis=1)
+        got = uh[:, kz % Ns[2], ky % Ns[1], kx]
+        assert np.abs(got - want).max() < 1e-12 * np.abs(want).max() + 1e-15
+    # dropped modes are zero: kx = N/2 (r2c Nyquist), ky = -N/2 absent for odd N, kz = -6 (N even)
+    assert np.all(uh[:, :, :, 5] == 0) and np.all(uh[:, 6, :, :] == 0)
+
+
+def test_nufft_vs_exact_sum(vp, O):
+    """NUFFT long range agrees with the exact Fourier sum (test/trefoil.jl:135-245,402-406: 1e-5)."""
+    fs = trefoil_fs(vp)
+    p = trefoil_params(vp, α=3.0)
+    pd = vp.add_point_charges(fs, p)
+    q = ("velocity", "streamfunction")
+    ex = O.longrange(pd, p, q, method="exact")
+    nu = O.longrange(pd, p, q, method="nufft")
+    for k in q:
+        err = np.abs(nu[k] - ex[k]).max() / np.abs(ex[k]).max()
+        assert err < 1e-5, (k, err)
+
+
+def _splitting_maxdiffs(vp, O, fs, rcuts, **kws):
+    fields = []
+    for rc in rcuts:
+        p = trefoil_params(vp, rcut=rc, **kws)
+        pd = vp.add_point_charges(fs, p)
+        fields.append(O.compute(pd, p, ("velocity", "streamfunction")))
+    last = fields[-1]
+    dv = max(np.max(np.linalg.norm(f["velocity"] - last["velocity"], axis=0) / np.linalg.norm(last["velocity"], axis=0))
+             for f in fields)
+    dp = max(np.max(np.linalg.norm(f["streamfunction"] - last["streamfunction"], axis=0)
+                    / np.linalg.norm(last["streamfunction"], axis=0)) for f in fields)
+    return dv, dp
+
+
+@pytest.mark.parametrize("avoid_explicit_erf", [True, False])
+def test_independence_on_splitting_parameter(vp, O, avoid_explicit_erf):
+    """Total v and ψ do not depend on α / rcut (test/trefoil.jl:297-329: vel < 10·rtol, ψ < rtol/5)."""
+    rtol = 1e-6
+    dv, dp = _splitting_maxdiffs(vp, O, trefoil_fs(vp), [0.3, 0.5, 0.7, 0.9], rtol=rtol,
+                                 avoid_explicit_erf=avoid_explicit_erf)
+    assert dv < 10 * rtol, dv
+    assert dp < rtol / 5, dp
+
+
+# ----------------------------------------------------------------------------
+# Planted mistakes: each breaks the oracle in a plausible way and the checks
+# above must notice.
+
+
+def test_planted_mistake_missing_ewald_factor(vp, O, monkeypatch):
+    orig = O._fourier_operator
+
+    def no_ewald(kx, ky, kz, Ls, alpha, quantity, trunc):
+        scal, kv = orig(kx, ky, kz, Ls, 1e9, quantity, trunc)  # exp(-k²/4α²) → 1
+        return scal, kv
+
+    monkeypatch.setattr(O, "_fourier_operator", no_ewald)
+    dv, dp = _splitting_maxdiffs(vp, O, trefoil_fs(vp), [0.3, 0.9])
+    assert dv > 1e-4 and dp > 1e-4
+
+
+def test_planted_mistake_curl_sign(vp, O, monkeypatch):
+    orig = O._apply_operator
+    monkeypatch.setattr(O, "_apply_operator", lambda w, s, k, q: -orig(w, s, k, q) if q == "velocity" else orig(w, s, k, q))
+    dv, _ = _splitting_maxdiffs(vp, O, trefoil_fs(vp), [0.3, 0.9])
+    assert dv > 1e-3
+
+
+def periodic_ring_velocity_error(vp, O, N=64, R=0.5):
+    """|U - U_closed_form| / U for a small ring in the periodic box (Ewald path).
+
+    The periodic images shift U by O((R/L)³); for R = 0.5 in a 2π box that is
+    below 2e-3, far below the effect of any missing local term.
+    """
+    Γ, a, Δ = 2.4, 1e-6, 0.25
+    fs, p, pd = ring_setup(vp, N, quad=4, R=R)
+    v = O.compute(pd, p, ("velocity",))["velocity"]
+    U_ex = Γ / (4 * math.pi * R) * (math.log(8 * R / a) - Δ)
+    return abs(v[2].mean() - U_ex) / U_ex
+
+
+def test_periodic_ring_velocity(vp, O):
+    assert periodic_ring_velocity_error(vp, O) < 2e-3
+
+
+def test_planted_mistake_no_self_interaction(vp, O, monkeypatch):
+    L = O.lib()
+
+    class NoSelf:
+        def __getattr__(self, name):
+            if name == "or_self_interaction":
+                return lambda *a: None
+            return getattr(L, name)
+
+    monkeypatch.setattr(O, "lib", lambda: NoSelf())
+    # invisible to the splitting-independence test (the double-counted local
+    # integral does not depend on α) ...
+    dv, _ = _splitting_maxdiffs(vp, O, trefoil_fs(vp), [0.3, 0.9])
+    assert dv < 1e-4
+    # ... but caught by the closed form
+    assert periodic_ring_velocity_error(vp, O) > 0.1
+
+
+def test_planted_mistake_lia_cutoff(vp, O, monkeypatch):
+    """Using exp(Δ-1)a/2 for the velocity cut-off too (a plausible slip) breaks the ring velocity."""
+    monkeypatch.setattr(O, "local_cutoffs", lambda params, ic=None: (math.exp(params.Δ - 1) * params.a / 2,) * 2)
+    assert periodic_ring_velocity_error(vp, O) > 1e-2

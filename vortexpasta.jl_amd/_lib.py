@@ -1,0 +1,237 @@
+"""ctypes binding of libvpbs (the C ABI declared in include/vpbs.h).
+
+The library is built in-tree (``__graft_entry__.build()``) as
+``vortexpasta.jl_amd/libvpbs.so``. There is no fallback: if the library is
+missing or fails to load, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import pathlib
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libvpbs.so"
+
+ABI_VERSION = 1
+F64, F32 = 0, 1
+VELOCITY = 1
+STREAMFUNCTION = 2
+LIA_OFF = 4
+LIA_ONLY = 8
+NO_SHORTRANGE = 16
+NO_LONGRANGE = 32
+NEIGHBOUR_STATS = 64
+
+EXPORTED = (
+    "vpbs_create",
+    "vpbs_destroy",
+    "vpbs_last_error",
+    "vpbs_set_points",
+    "vpbs_set_points_device",
+    "vpbs_compute",
+    "vpbs_get_outputs",
+    "vpbs_output_device_ptrs",
+    "vpbs_neighbour_stats",
+    "vpbs_get_timings",
+    "vpbs_get_sizes",
+)
+
+
+class VpbsParams(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("Gamma", ctypes.c_double),
+        ("a", ctypes.c_double),
+        ("Delta", ctypes.c_double),
+        ("Ls", ctypes.c_double * 3),
+        ("alpha", ctypes.c_double),
+        ("rcut", ctypes.c_double),
+        ("Ns", ctypes.c_int32 * 3),
+        ("nquad", ctypes.c_int32),
+        ("nsubdiv", ctypes.c_int32),
+        ("nufft_m", ctypes.c_int32),
+        ("nufft_sigma", ctypes.c_double),
+        ("avoid_explicit_erf", ctypes.c_int32),
+        ("truncate_spherical", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class VpbsTimings(ctypes.Structure):
+    _fields_ = [
+        ("total", ctypes.c_double),
+        ("local_term", ctypes.c_double),
+        ("cell_build", ctypes.c_double),
+        ("pair", ctypes.c_double),
+        ("self", ctypes.c_double),
+        ("lr_sort", ctypes.c_double),
+        ("spread", ctypes.c_double),
+        ("fft_forward", ctypes.c_double),
+        ("fourier", ctypes.c_double),
+        ("fft_backward", ctypes.c_double),
+        ("interp", ctypes.c_double),
+        ("merge", ctypes.c_double),
+        ("pair_candidates", ctypes.c_int64),
+        ("grid", ctypes.c_int64 * 3),
+        ("cells", ctypes.c_int64 * 3),
+    ]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["grid"] = tuple(self.grid)
+        d["cells"] = tuple(self.cells)
+        return d
+
+
+class VpbsError(RuntimeError):
+    pass
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def load() -> ctypes.CDLL:
+    """Load libvpbs.so and declare the signatures. Raises if the library is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise VpbsError(f"libvpbs not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+    L = ctypes.CDLL(str(LIB_PATH))
+    vp = ctypes.c_void_p
+    i64, u32, d = ctypes.c_int64, ctypes.c_uint32, ctypes.c_double
+    L.vpbs_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(VpbsParams)]
+    L.vpbs_create.restype = ctypes.c_int
+    L.vpbs_destroy.argtypes = [vp]
+    L.vpbs_destroy.restype = None
+    L.vpbs_last_error.argtypes = [vp]
+    L.vpbs_last_error.restype = ctypes.c_char_p
+    pts_sig = [vp, i64, vp, vp, vp, vp, vp, vp, i64, vp, vp]
+    L.vpbs_set_points.argtypes = pts_sig
+    L.vpbs_set_points.restype = ctypes.c_int
+    L.vpbs_set_points_device.argtypes = pts_sig
+    L.vpbs_set_points_device.restype = ctypes.c_int
+    L.vpbs_compute.argtypes = [vp, u32, d, vp, vp]
+    L.vpbs_compute.restype = ctypes.c_int
+    L.vpbs_get_outputs.argtypes = [vp, vp, vp]
+    L.vpbs_get_outputs.restype = ctypes.c_int
+    L.vpbs_output_device_ptrs.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    L.vpbs_output_device_ptrs.restype = ctypes.c_int
+    L.vpbs_neighbour_stats.argtypes = [vp, vp, vp]
+    L.vpbs_neighbour_stats.restype = ctypes.c_int
+    L.vpbs_get_timings.argtypes = [vp, ctypes.POINTER(VpbsTimings)]
+    L.vpbs_get_timings.restype = ctypes.c_int
+    L.vpbs_get_sizes.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+    L.vpbs_get_sizes.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def make_params(params, device: int = 0) -> VpbsParams:
+    """Translate a ParamsBiotSavart into the C parameter struct."""
+    p = VpbsParams()
+    p.abi_version = ABI_VERSION
+    p.precision = F32 if params.precision == "f32" else F64
+    p.Gamma = params.Γ
+    p.a = params.a
+    p.Delta = params.Δ
+    for d in range(3):
+        p.Ls[d] = params.Ls[d]
+        p.Ns[d] = params.Ns[d]
+    p.alpha = params.α
+    p.rcut = params.rcut
+    p.nquad = len(params.quadrature)
+    p.nsubdiv = params.backend_short.nsubdiv
+    p.nufft_m = params.backend_long.m
+    p.nufft_sigma = params.backend_long.σ
+    p.avoid_explicit_erf = int(params.avoid_explicit_erf)
+    p.truncate_spherical = int(params.longrange_truncate_spherical)
+    p.device = device
+    return p
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Context:
+    """Owns one vpbs_ctx (one GPU). Not thread-safe: calls must not overlap."""
+
+    def __init__(self, params, device: int = 0):
+        self._L = load()
+        self._h = ctypes.c_void_p()
+        self._p = make_params(params, device)
+        rc = self._L.vpbs_create(ctypes.byref(self._h), ctypes.byref(self._p))
+        if rc != 0:
+            raise VpbsError(f"vpbs_create failed ({rc}): {self._L.vpbs_last_error(None).decode()}")
+        self.Np = 0
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._L.vpbs_last_error(self._h).decode()
+            raise VpbsError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if self._h:
+            self._L.vpbs_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_points(self, pd):
+        arrs = dict(
+            nodes=np.ascontiguousarray(pd.nodes, dtype=np.float64),
+            idx=np.ascontiguousarray(pd.node_idx_prev, dtype=np.int32),
+            sp=np.ascontiguousarray(pd.sp, dtype=np.float64),
+            spp=np.ascontiguousarray(pd.spp, dtype=np.float64),
+            dlm=np.ascontiguousarray(pd.dlm, dtype=np.float64),
+            dlp=np.ascontiguousarray(pd.dlp, dtype=np.float64),
+            pts=np.ascontiguousarray(pd.points, dtype=np.float64),
+            chg=np.ascontiguousarray(pd.charges, dtype=np.float64),
+        )
+        Np = arrs["nodes"].shape[1]
+        Nq = arrs["pts"].shape[1]
+        rc = self._L.vpbs_set_points(
+            self._h, Np, _ptr(arrs["nodes"]), _ptr(arrs["idx"]), _ptr(arrs["sp"]), _ptr(arrs["spp"]),
+            _ptr(arrs["dlm"]), _ptr(arrs["dlp"]), Nq, _ptr(arrs["pts"]), _ptr(arrs["chg"]),
+        )
+        self._check(rc, "vpbs_set_points")
+        self.Np = Np
+
+    def compute(self, flags: int, integration_cutoff=None, fetch=True):
+        """Run vpbs_compute. Returns (v, psi) as (3, Np) arrays (None where not requested / not fetched)."""
+        cut = math.nan if integration_cutoff is None else float(integration_cutoff)
+        v = np.empty((3, self.Np)) if (fetch and flags & VELOCITY) else None
+        psi = np.empty((3, self.Np)) if (fetch and flags & STREAMFUNCTION) else None
+        rc = self._L.vpbs_compute(self._h, flags, cut, _ptr(v), _ptr(psi))
+        self._check(rc, "vpbs_compute")
+        return v, psi
+
+    def neighbour_stats(self):
+        counts = np.empty(self.Np, dtype=np.int64)
+        hashes = np.empty(self.Np, dtype=np.uint64)
+        self._check(self._L.vpbs_neighbour_stats(self._h, _ptr(counts), _ptr(hashes)), "vpbs_neighbour_stats")
+        return counts, hashes
+
+    def timings(self) -> dict:
+        t = VpbsTimings()
+        self._check(self._L.vpbs_get_timings(self._h, ctypes.byref(t)), "vpbs_get_timings")
+        return t.as_dict()
+
+    def sizes(self):
+        g = (ctypes.c_int64 * 3)()
+        c = (ctypes.c_int64 * 3)()
+        self._check(self._L.vpbs_get_sizes(self._h, g, c), "vpbs_get_sizes")
+        return tuple(g), tuple(c)

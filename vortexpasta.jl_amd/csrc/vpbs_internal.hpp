@@ -1,0 +1,154 @@
+// Internal declarations of libvpbs (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/vpbs.h"
+
+namespace vpbs {
+
+// Geometry of the periodic box.
+struct Box {
+    double L[3];
+    double Lh[3];
+};
+
+// Deterministic counting sort scratch (grow-only device buffers).
+struct SortBuffers {
+    int32_t *rank = nullptr;   // [n] rank within bin from the histogram pass
+    int64_t cap_n = 0;
+    int32_t *scan_tmp = nullptr;  // block sums for the multi-level scan
+    int64_t cap_tmp = 0;
+};
+
+// Per-stage event pairs.
+enum Stage {
+    ST_LOCAL = 0,
+    ST_CELL,
+    ST_PAIR,
+    ST_SELF,
+    ST_LRSORT,
+    ST_SPREAD,
+    ST_FFTF,
+    ST_FOURIER,
+    ST_FFTB,
+    ST_INTERP,
+    ST_MERGE,
+    ST_COUNT
+};
+
+}  // namespace vpbs
+
+struct vpbs_ctx {
+    vpbs_params p{};
+    std::string err;
+    int device = 0;
+    hipStream_t s_sr = nullptr, s_lr = nullptr;
+    hipEvent_t ev_beg[vpbs::ST_COUNT]{}, ev_end[vpbs::ST_COUNT]{};
+    bool stage_ran[vpbs::ST_COUNT]{};
+    hipEvent_t ev_lr_done = nullptr;
+    vpbs_timings timings{};
+    vpbs::Box box{};
+
+    // point data (device, FP64)
+    int64_t Np = 0, Nq = 0, cap_np = 0, cap_nq = 0;
+    bool have_points = false;
+    double *nodes = nullptr, *sp = nullptr, *spp = nullptr, *dlm = nullptr, *dlp = nullptr;
+    int32_t *idx_prev = nullptr;
+    double *points = nullptr, *charges = nullptr;
+
+    // outputs (device, FP64, 3*Np each)
+    double *v_sr = nullptr, *psi_sr = nullptr, *v_lr = nullptr, *psi_lr = nullptr, *v_out = nullptr,
+           *psi_out = nullptr;
+    bool last_vel = false, last_psi = false;
+
+    // cell lists
+    int32_t Nc[3] = {0, 0, 0};
+    int64_t ncells = 0;
+    int32_t *src_key = nullptr, *src_start = nullptr, *src_perm = nullptr;
+    int32_t *tgt_key = nullptr, *tgt_start = nullptr, *tgt_perm = nullptr;
+    double *src_pos = nullptr;   // sorted sources, 3*Nq (component-major)
+    void *src_chg = nullptr;     // sorted charges, 3*Nq of the compute type
+    int32_t *src_idx = nullptr;  // original index of each sorted source
+    int64_t *stat_counts = nullptr;
+    uint64_t *stat_hashes = nullptr;
+    bool have_stats = false;
+    vpbs::SortBuffers sort_sr, sort_tg, sort_lr;
+
+    // long range
+    int32_t G[3] = {0, 0, 0};   // oversampled grid
+    int64_t Gxp = 0;            // padded x length of the in-place r2c layout (2*(G0/2+1))
+    int64_t comp_stride = 0;    // elements (of the grid type) per component
+    int32_t ntile[3] = {0, 0, 0};
+    int64_t ntiles = 0;
+    void *grid = nullptr;       // 3 components, velocity (or vorticity) spectrum in place
+    void *grid2 = nullptr;      // 3 components, streamfunction
+    int32_t *lr_key = nullptr, *lr_start = nullptr, *lr_perm = nullptr;
+    double *lr_pos = nullptr;   // tile-sorted quadrature points, 3*Nq
+    void *lr_chg = nullptr;     // tile-sorted charges, 3*Nq (compute type)
+    double *deconv = nullptr;   // 1/ψ̂ tables for retained kx, ky, kz (concatenated)
+    int32_t kmax_mode[3] = {0, 0, 0};
+    double es_beta = 0;
+    rocfft_plan plan_fwd = nullptr, plan_bwd = nullptr;
+    rocfft_execution_info info_fwd = nullptr, info_bwd = nullptr;
+    void *fft_work = nullptr;
+    size_t fft_work_size = 0;
+    bool plans_ready = false;
+};
+
+namespace vpbs {
+
+int fail(vpbs_ctx *c, int code, const char *fmt, ...);
+
+#define VPBS_HIP(ctx, expr)                                                                        \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return ::vpbs::fail((ctx), VPBS_ERR_HIP, "%s failed: %s (%s:%d)", #expr,               \
+                                hipGetErrorString(e_), __FILE__, __LINE__);                        \
+    } while (0)
+
+#define VPBS_TRY(expr)          \
+    do {                        \
+        int r_ = (expr);        \
+        if (r_ != VPBS_OK)      \
+            return r_;          \
+    } while (0)
+
+template <typename T>
+int grow(vpbs_ctx *c, T **ptr, int64_t *cap, int64_t n) {
+    if (n <= *cap && *ptr) return VPBS_OK;
+    if (*ptr) {
+        VPBS_HIP(c, hipFree(*ptr));
+        *ptr = nullptr;
+    }
+    int64_t m = n > 0 ? n : 1;
+    VPBS_HIP(c, hipMalloc((void **)ptr, sizeof(T) * (size_t)m));
+    *cap = m;
+    return VPBS_OK;
+}
+
+// sort.hip
+int counting_sort(vpbs_ctx *c, SortBuffers &sb, const int32_t *keys, int64_t n, int64_t nbins, int32_t *start,
+                  int32_t *perm, hipStream_t s);
+
+// shortrange.hip
+int shortrange_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff);
+
+// longrange.hip
+int longrange_setup(vpbs_ctx *c);
+int longrange_compute(vpbs_ctx *c, uint32_t flags);
+void longrange_release(vpbs_ctx *c);
+int oversampled_size(int N, double sigma);
+double es_beta(int m, double sigma);
+
+// helpers
+void stage_begin(vpbs_ctx *c, Stage st, hipStream_t s);
+void stage_end(vpbs_ctx *c, Stage st, hipStream_t s);
+
+}  // namespace vpbs
