@@ -1,0 +1,254 @@
+#!/usr/bin/env python3
+"""Benchmark: Ewald-split Biot–Savart velocity on ~1M filament nodes (one MI355X).
+
+Workload (default): 3000 random vortex rings, radii U[0.3, 1.0], node spacing
+2π/512 (≈1.0M nodes, ≈3.0M quadrature points), periodic box [0, 2π)³, Fourier
+grid 512³ (σ = 1.5 → 768³ oversampled NUFFT grid), β = 3.5 (α = 36.43,
+rcut = 0.0961), quintic splines, GL(3), Γ = 1, a = 1e-6, Δ = 1/4.
+
+A step is one velocity evaluation (velocity_on_nodes!) of the whole tangle with
+the point data already resident in HBM: cell-list build, short-range pair sum,
+LIA and self terms, NUFFT spreading, FFTs, Fourier-space kernel, interpolation
+and the final sum. ``value`` is node-velocities/s of the FP32-bulk path; the
+plain FP64 path is timed too, and the FP32 result is compared with the FP64
+one (the FP64 path matches the CPU oracle to ≤1e-12 in the parity tests).
+
+Prints ONE JSON line. ``--gpus`` above 1 is not supported (one GPU per box).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+import __graft_entry__ as ge
+
+METRIC = "Biot–Savart node-velocities/s at 1M nodes; rel L2 error vs FP64 oracle"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP64_VALU_PEAK_TFS = 78.6
+FP32_VALU_PEAK_TFS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(vp, cfg: int):
+    fs = vp.tangles.config_tangle(cfg)
+    Ns = {1: 64, 2: 128, 3: 512, 4: 256}[cfg]
+    spl = vp.GaussianSplitting.create(Ls=2 * math.pi, β=3.5, Ns=Ns)
+    p32 = vp.ParamsBiotSavart.build(np.float32, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl)
+    p64 = vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl)
+    batch = vp.FilamentBatch(fs)
+    pd = vp.add_point_charges(fs, p64, batch=batch)
+    return fs, spl, p32, p64, pd
+
+
+def time_path(vp, params, pd, steps: int, warmup: int):
+    ctx = vp._lib.Context(params)
+    ctx.set_points(pd)  # H2D, outside the timed region
+    flags = vp._lib.VELOCITY
+    for _ in range(warmup):
+        ctx.compute(flags, fetch=False)
+    acc = {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.compute(flags, fetch=False)  # returns after both streams are synchronised
+        t = ctx.timings()
+        for k, v in t.items():
+            if isinstance(v, float):
+                acc[k] = acc.get(k, 0.0) + v
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / steps
+    stages = {k: v / steps for k, v in acc.items() if k != "total"}
+    v, _ = ctx.compute(flags, fetch=True)
+    grid, cells = ctx.sizes()
+    ctx.close()
+    return ms, stages, v, grid, cells
+
+
+def stage_bytes(stage: str, s: int, G, Ns, Np: int, Nq: int) -> float:
+    """Algorithmic HBM bytes per launch of each long-range stage (velocity only).
+
+    G: oversampled grid, Ns: Fourier grid, s: bytes per real of the compute type.
+    spread 3Gs + 6Nq·s ; FFT forward 3Gs + 6Hs ; Fourier kernel 6Ks + 6Hs ;
+    FFT backward 6Hs + 3Gs ; interpolation 3Gs + 6Np·s, with G = ∏G_d,
+    H = (G_0/2+1)G_1G_2 and K = (N_0/2+1)N_1N_2.
+    """
+    Gc = float(np.prod(G))
+    H = (G[0] // 2 + 1) * float(G[1]) * G[2]
+    K = (Ns[0] // 2 + 1) * float(Ns[1]) * Ns[2]
+    return {
+        "spread": 3 * Gc * s + 6 * Nq * s,
+        "fft_forward": 3 * Gc * s + 6 * H * s,
+        "fourier": 6 * K * s + 6 * H * s,
+        "fft_backward": 6 * H * s + 3 * Gc * s,
+        "interp": 3 * Gc * s + 6 * Np * s,
+    }[stage]
+
+
+def cpu_baseline(O, pd, params, target_s: float = 12.0):
+    """Time the FP64 CPU oracle on a bounded sample and scale to node-velocities/s.
+
+    Sample: short-range pair sum for a contiguous block of nodes (whole rings),
+    spreading/interpolation for a contiguous block of points/nodes, one 3D r2c
+    FFT of one velocity component on the full oversampled grid (scaled ×6 for
+    the 3 forward + 3 backward transforms), LIA + self terms on all nodes.
+    """
+    import scipy.fft
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    L = O.lib()
+    Np, Nq = pd.Np, pd.Nq
+    nodes = O._c(pd.nodes)
+    pts = O._c(pd.points)
+    chg = O._c(pd.charges)
+    idx = O._c(pd.node_idx_prev, np.int32)
+    Ls = np.array(params.Ls)
+    v = np.zeros((3, Np))
+    flags = O.FLAG_VEL | (O.FLAG_AVOID_ERF if params.avoid_explicit_erf else 0)
+
+    def sr(i1):
+        t = time.perf_counter()
+        L.or_shortrange(Np, O._p(nodes), O._p(idx, O._ip), Nq, O._p(pts), O._p(chg), pd.nquad, O._p(Ls), params.α,
+                        params.rcut, params.Γ, params.backend_short.nsubdiv, 1, flags, O._p(v), None, None, None, 0,
+                        i1)
+        return time.perf_counter() - t
+
+    t_probe = sr(min(Np, 2000))  # includes the cell-list build
+    n_sr = int(min(Np, max(2000, 2000 * (0.3 * target_s) / max(t_probe, 1e-6))))
+    t_sr = sr(n_sr)
+    t_sr_full = t_sr * Np / n_sr
+    t = time.perf_counter()
+    cv, cpsi = O.local_cutoffs(params)
+    L.or_local_term(Np, O._p(O._c(pd.sp)), O._p(O._c(pd.spp)), O._p(O._c(pd.dlm)), O._p(O._c(pd.dlp)), params.Γ, cv,
+                    cpsi, flags, O._p(v), None)
+    L.or_self_interaction(Np, O._p(nodes), O._p(idx, O._ip), Nq, O._p(pts), O._p(chg), pd.nquad, O._p(Ls), params.α,
+                          params.Γ, flags, O._p(v), None)
+    t_local = time.perf_counter() - t
+    # long range
+    m, sigma = params.backend_long.m, params.backend_long.σ
+    beta = O.es_beta(m, sigma)
+    G = np.array([O.oversampled_size(int(n), sigma, m) for n in params.Ns], dtype=np.int32)
+    grid = np.empty((3, G[2], G[1], G[0]))
+    grid.fill(1.0)  # fault the pages in once (a CPU code reuses its grid across evaluations)
+    t = time.perf_counter()
+    grid.fill(0.0)  # per-evaluation zeroing of the spreading grid
+    t_zero = time.perf_counter() - t
+    nq_s = min(Nq, 300000)
+    t = time.perf_counter()
+    L.or_spread(nq_s, O._p(O._c(pts[:, :nq_s])), O._p(O._c(chg[:, :nq_s])), O._p(Ls), O._p(G, O._ip), m, beta,
+                O._p(grid))
+    t_spread = (time.perf_counter() - t) * Nq / nq_s + t_zero
+    np_s = min(Np, 100000)
+    out = np.zeros((3, np_s))
+    t = time.perf_counter()
+    L.or_interp(np_s, O._p(O._c(nodes[:, :np_s])), O._p(Ls), O._p(G, O._ip), m, beta, O._p(grid), O._p(out))
+    t_interp = (time.perf_counter() - t) * Np / np_s
+    t = time.perf_counter()
+    spec = scipy.fft.rfftn(grid[0], workers=cores)
+    t_fft = (time.perf_counter() - t) * 6
+    del spec, grid
+    total = t_sr_full + t_local + t_spread + t_fft + t_interp
+    sample = (f"FP64 oracle, {cores} threads: pair sum on {n_sr} of {Np} nodes, spread {nq_s} of {Nq} points "
+              f"(+ grid zero-fill), "
+              f"interp {np_s} nodes, one {G[0]}^3 r2c FFT (x6), LIA+self on all nodes; scaled to one evaluation "
+              f"of {total:.1f} s")
+    return {"value": Np / total, "unit": "node-velocities/s", "cores": cores, "kind": "port", "sample": sample,
+            "seconds_per_eval": total,
+            "breakdown_s": {"pairs": t_sr_full, "local_self": t_local, "spread": t_spread, "fft": t_fft,
+                            "interp": t_interp}}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3, 4])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 path (and the error figure)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU per box "
+              "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    vp = ge.load_package()
+    t = time.perf_counter()
+    fs, spl, p32, p64, pd = build_workload(vp, args.config)
+    log(f"workload: {len(fs)} filaments, Np = {pd.Np}, Nq = {pd.Nq}, Ns = {spl.Ns}, α = {spl.α:.4f}, "
+        f"rcut = {spl.rcut:.5f} (host prep {time.perf_counter() - t:.1f} s)")
+
+    ms32, st32, v32, grid, cells = time_path(vp, p32, pd, args.steps, args.warmup)
+    log(f"FP32 path: {ms32:.3f} ms/step; stages {json.dumps({k: round(v, 3) for k, v in st32.items()})}")
+    res64 = None
+    if not args.no_fp64:
+        ms64, st64, v64, _, _ = time_path(vp, p64, pd, args.steps, args.warmup)
+        log(f"FP64 path: {ms64:.3f} ms/step; stages {json.dumps({k: round(v, 3) for k, v in st64.items()})}")
+        err = float(np.linalg.norm(v32 - v64) / np.linalg.norm(v64))
+        res64 = {"value": pd.Np / (ms64 * 1e-3), "ms_per_step": ms64, "dtype": "f64",
+                 "stages_ms": {k: round(v, 4) for k, v in st64.items()}}
+
+    # roofline of the dominant long-range kernel of the FP32 path (our own kernels only)
+    lr_kernels = ["spread", "fourier", "interp"]
+    dom = max(lr_kernels, key=lambda k: st32.get(k, 0.0))
+    nbytes = stage_bytes(dom, 4, grid, spl.Ns, pd.Np, pd.Nq)
+    achieved = nbytes / (st32[dom] * 1e-3) / 1e9
+    per_stage = {k: round(stage_bytes(k, 4, grid, spl.Ns, pd.Np, pd.Nq) / (st32[k] * 1e-3) / 1e9, 1)
+                 for k in ("spread", "fft_forward", "fourier", "fft_backward", "interp") if st32.get(k, 0) > 0}
+
+    result = {
+        "metric": METRIC,
+        "value": pd.Np / (ms32 * 1e-3),
+        "unit": "node-velocities/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"config {args.config}: {len(fs)} random vortex rings, {pd.Np} nodes, {pd.Nq} quadrature "
+                        f"points, Fourier grid {spl.Ns[0]}^3 (NUFFT grid {grid[0]}^3), beta 3.5, FP32-bulk path",
+            "nodes": pd.Np, "quadrature_points": pd.Nq, "fourier_grid": list(spl.Ns),
+            "nufft_grid": list(grid), "cells": list(cells), "alpha": spl.α, "rcut": spl.rcut,
+        },
+        "roofline": {
+            "kernel": f"{dom}_kernel<float>",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes": nbytes,
+            "avg_ms": round(st32[dom], 4),
+        },
+        "stages_ms": {k: round(v, 4) for k, v in st32.items()},
+        "stage_algorithmic_GBps": per_stage,
+    }
+    if res64 is not None:
+        result["fp64_path"] = res64
+        result["rel_l2_error"] = {"velocity_fp32_vs_fp64": err,
+                                  "note": "FP64 path matches the CPU oracle to <=1e-12 (tests/test_gpu_parity.py)"}
+    if not args.no_cpu_baseline:
+        O = ge.load_oracle()
+        log("timing the CPU oracle sample ...")
+        cb = cpu_baseline(O, pd, p64)
+        log(f"CPU baseline: {cb['value']:.3e} node-velocities/s ({cb['sample']})")
+        result["cpu_baseline"] = cb
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

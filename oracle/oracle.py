@@ -59,7 +59,8 @@ def lib():
             raise RuntimeError(f"oracle library not built: {path} (run __graft_entry__.build())")
         L = ctypes.CDLL(str(path))
         i64, i32, d, u = ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_uint
-        L.or_shortrange.argtypes = [i64, _dp, _ip, i64, _dp, _dp, i32, _dp, d, d, d, i32, i32, u, _dp, _dp, _lp, _up]
+        L.or_shortrange.argtypes = [i64, _dp, _ip, i64, _dp, _dp, i32, _dp, d, d, d, i32, i32, u, _dp, _dp, _lp, _up,
+                                    i64, i64]
         L.or_direct_open.argtypes = [i64, _dp, _ip, i64, _dp, _dp, i32, d, u, _dp, _dp]
         L.or_self_interaction.argtypes = [i64, _dp, _ip, i64, _dp, _dp, i32, _dp, d, d, u, _dp, _dp]
         L.or_local_term.argtypes = [i64, _dp, _dp, _dp, _dp, d, d, d, u, _dp, _dp]
@@ -266,7 +267,7 @@ def compute(pd, params, quantities=("velocity",), LIA=True, shortrange=True, lon
             hashes = np.zeros(Np, dtype=np.uint64) if neighbour_stats else None
             L.or_shortrange(Np, _p(nodes), _p(idx, _ip), Nq, _p(pts), _p(chg), pd.nquad, _p(Ls), params.α,
                             params.rcut, params.Γ, params.backend_short.nsubdiv, int(use_celllist), flags,
-                            _p(v), _p(psi), _p(counts, _lp), _p(hashes, _up))
+                            _p(v), _p(psi), _p(counts, _lp), _p(hashes, _up), 0, -1)
             L.or_self_interaction(Np, _p(nodes), _p(idx, _ip), Nq, _p(pts), _p(chg), pd.nquad, _p(Ls), params.α,
                                   params.Γ, flags, _p(v), _p(psi))
             if neighbour_stats:
@@ -340,7 +341,8 @@ def neighbour_pairs_bruteforce(pd, params):
     flags = FLAG_VEL | (FLAG_AVOID_ERF if params.avoid_explicit_erf else 0)
     v = np.zeros((3, Np))
     L.or_shortrange(Np, _p(nodes), _p(idx, _ip), Nq, _p(pts), _p(chg), pd.nquad, _p(Ls), params.α, params.rcut,
-                    params.Γ, params.backend_short.nsubdiv, 0, flags, _p(v), None, _p(counts, _lp), _p(hashes, _up))
+                    params.Γ, params.backend_short.nsubdiv, 0, flags, _p(v), None, _p(counts, _lp), _p(hashes, _up),
+                    0, -1)
     return counts, hashes
 
 

@@ -32,6 +32,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define FLAG_VEL 1u
 #define FLAG_PSI 2u
@@ -131,12 +134,13 @@ static inline int pair_term(const double x[3], const double *pts, const double *
  * (component-major, 3×Np), multiplied by Γ/4π. With use_celllist = 0 every
  * source is visited (the NaiveShortRangeBackend, backends/naive.jl:217-229).
  * counts/hashes (optional) receive, per node, the number of accepted pairs and
- * the wrapping sum of mix64(j) over accepted j.
+ * the wrapping sum of mix64(j) over accepted j. Only nodes in [i_begin, i_end)
+ * are processed (i_end < 0: all), which lets a CPU timing run take a sample.
  */
 void or_shortrange(int64_t Np, const double *nodes, const int32_t *idx_prev, int64_t Nq, const double *pts,
                    const double *chg, int nquad, const double *Ls, double alpha, double rcut, double Gamma,
                    int nsubdiv, int use_celllist, unsigned flags, double *v, double *psi, int64_t *counts,
-                   uint64_t *hashes) {
+                   uint64_t *hashes, int64_t i_begin, int64_t i_end) {
     const double Lhs[3] = {Ls[0] / 2, Ls[1] / 2, Ls[2] / 2};
     const double rcut2 = rcut * rcut;
     const double pref = Gamma / (4 * M_PI);
@@ -148,8 +152,9 @@ void or_shortrange(int64_t Np, const double *nodes, const int32_t *idx_prev, int
         or_cell_dims(rcut, Ls, M, cl.Nc);
         celllist_build(&cl, pts, Nq, Ls);
     }
+    if (i_end < 0 || i_end > Np) i_end = Np;
 #pragma omp parallel for schedule(dynamic, 64)
-    for (int64_t i = 0; i < Np; i++) {
+    for (int64_t i = i_begin; i < i_end; i++) {
         double x[3] = {nodes[i], nodes[Np + i], nodes[2 * Np + i]};
         double vi[3] = {0, 0, 0}, pi[3] = {0, 0, 0};
         int64_t cnt = 0;
@@ -409,36 +414,53 @@ static inline void es_weights(double u, int M, double beta, int64_t *i0, double 
  * NUFFT spreading on the oversampled grid G (shape Gz×Gy×Gx, x fastest, 3
  * components one after the other): grid[n] += q_j · Π_d φ(n_d - u_{j,d})
  * with u = x·G/L, periodically wrapped. ``grid`` must be zeroed by the caller.
- * Sequential over points (deterministic); parallel over components.
+ * Threads own disjoint sets of z-planes and visit the points in index order,
+ * so every grid value is summed in the same (deterministic) order.
  */
 void or_spread(int64_t Nq, const double *pts, const double *chg, const double *Ls, const int32_t *G, int M,
                double beta, double *grid) {
     const int64_t gsz = (int64_t)G[0] * G[1] * G[2];
+    const int W = 2 * M;
+    double *wts = malloc(sizeof(double) * (size_t)(Nq > 0 ? Nq : 1) * 3 * W);
+    int32_t *idx = malloc(sizeof(int32_t) * (size_t)(Nq > 0 ? Nq : 1) * 3 * W);
 #pragma omp parallel for schedule(static)
-    for (int comp = 0; comp < 3; comp++) {
-        double wx[32], wy[32], wz[32];
+    for (int64_t j = 0; j < Nq; j++) {
+        for (int d = 0; d < 3; d++) {
+            int64_t i0;
+            es_weights(pts[d * Nq + j] * ((double)G[d] / Ls[d]), M, beta, &i0, wts + (j * 3 + d) * W);
+            for (int k = 0; k < W; k++) idx[(j * 3 + d) * W + k] = (int32_t)(((i0 + k) % G[d] + G[d]) % G[d]);
+        }
+    }
+#pragma omp parallel
+    {
+#ifdef _OPENMP
+        int nt = omp_get_num_threads(), tid = omp_get_thread_num();
+#else
+        int nt = 1, tid = 0;
+#endif
         for (int64_t j = 0; j < Nq; j++) {
-            int64_t ix0, iy0, iz0;
-            es_weights(pts[j] * ((double)G[0] / Ls[0]), M, beta, &ix0, wx);
-            es_weights(pts[Nq + j] * ((double)G[1] / Ls[1]), M, beta, &iy0, wy);
-            es_weights(pts[2 * Nq + j] * ((double)G[2] / Ls[2]), M, beta, &iz0, wz);
-            double q = chg[comp * Nq + j];
-            double *g = grid + comp * gsz;
-            for (int c = 0; c < 2 * M; c++) {
-                int64_t iz = ((iz0 + c) % G[2] + G[2]) % G[2];
-                double qz = q * wz[c];
-                for (int b = 0; b < 2 * M; b++) {
-                    int64_t iy = ((iy0 + b) % G[1] + G[1]) % G[1];
-                    double qzy = qz * wy[b];
-                    double *row = g + (iz * G[1] + iy) * G[0];
-                    for (int a = 0; a < 2 * M; a++) {
-                        int64_t ix = ((ix0 + a) % G[0] + G[0]) % G[0];
-                        row[ix] += qzy * wx[a];
+            const int32_t *iz = idx + (j * 3 + 2) * W;
+            const int32_t *iy = idx + (j * 3 + 1) * W;
+            const int32_t *ix = idx + (j * 3 + 0) * W;
+            const double *wz = wts + (j * 3 + 2) * W;
+            const double *wy = wts + (j * 3 + 1) * W;
+            const double *wx = wts + (j * 3 + 0) * W;
+            for (int c = 0; c < W; c++) {
+                if (iz[c] % nt != tid) continue; /* plane owned by another thread */
+                for (int comp = 0; comp < 3; comp++) {
+                    double qz = chg[comp * Nq + j] * wz[c];
+                    double *g = grid + comp * gsz + (int64_t)iz[c] * G[1] * G[0];
+                    for (int b = 0; b < W; b++) {
+                        double qzy = qz * wy[b];
+                        double *row = g + (int64_t)iy[b] * G[0];
+                        for (int a = 0; a < W; a++) row[ix[a]] += qzy * wx[a];
                     }
                 }
             }
         }
     }
+    free(wts);
+    free(idx);
 }
 
 /* NUFFT interpolation: out_i = Σ_n Π_d φ(u_{i,d} - n_d) grid[n] (3 components). */
