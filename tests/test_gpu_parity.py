@@ -201,3 +201,18 @@ def test_error_paths(vp, cfg1):
     with pytest.raises(ValueError):
         cache = vp.init_cache(params)
         vp.compute_on_nodes_({"velocity": [np.zeros((3, 3))]}, cache, cfg1)
+
+
+@pytest.mark.parametrize("sigma,m", [(1.25, 5), (2.0, 3), (1.5, 6)])
+def test_nufft_kernel_widths(vp, O, cfg1, sigma, m):
+    """Other oversampling factors / half-supports (generic spreading and interpolation paths)."""
+    params = _params(vp, backend_long=vp.HIPLongRangeBackend(σ=sigma, m=m))
+    _check(vp, O, params, cfg1)
+
+
+def test_tiny_grid_few_tiles(vp, O):
+    """Ns = 10 → a 16³ NUFFT grid with fewer than 3 spreading tiles per direction."""
+    fs = vp.tangles.random_rings(3, 0.8, 1.2, TWO_PI / 24, seed=2)
+    spl = vp.GaussianSplitting.create(Ls=TWO_PI, α=1.0, rcut=2.0, Ns=10)
+    params = vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, splitting=spl)
+    _check(vp, O, params, fs)

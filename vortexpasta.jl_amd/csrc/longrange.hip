@@ -25,6 +25,7 @@
 // for the FP32-bulk path).
 #include "vpbs_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -34,7 +35,6 @@ namespace {
 
 constexpr int TX = 16, TY = 8, TZ = 8;  // spread tile
 constexpr int SPREAD_THREADS = 256;
-constexpr int PB = 128;     // staged points per batch
 constexpr int MAXM = 8;     // largest supported half-support
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -95,9 +95,10 @@ __global__ __launch_bounds__(SPREAD_THREADS) void spread_kernel(LRGeom g, int64_
                                                                const double *__restrict__ spos,
                                                                const CT *__restrict__ schg, CT beta,
                                                                CT *__restrict__ grid) {
+    constexpr int PB = sizeof(CT) == 4 ? 256 : 128;  // staged points per batch
     __shared__ int nb_start[27];
+    __shared__ int nb_len[27];
     __shared__ int nb_pre[28];
-    __shared__ int nnb;
     __shared__ CT s_wx[PB][TX + 1];
     __shared__ CT s_wy[PB][TY + 1];
     __shared__ CT s_wz[PB][TZ + 1];
@@ -116,34 +117,38 @@ __global__ __launch_bounds__(SPREAD_THREADS) void spread_kernel(LRGeom g, int64_
     const int M = g.M;
     const int W = 2 * M;
 
-    if (tid == 0) {
-        // unique neighbour tiles per direction (fewer than 3 when the grid has < 3 tiles)
-        int nt[3][3], cntd[3];
+    // Neighbour tiles t + (-1..1)³: lane k < 27 loads the point range of one of
+    // them; a direction with fewer than 3 tiles would visit a tile twice, so the
+    // repeated offsets get an empty range.
+    if (tid < 27) {
+        const int o[3] = {tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1};
         const int tt[3] = {tix, tiy, tiz};
+        int tn[3];
+        bool dup = false;
+#pragma unroll
         for (int d = 0; d < 3; d++) {
-            cntd[d] = 0;
-            for (int o = -1; o <= 1; o++) {
-                int v = wrapi(tt[d] + o, g.ntile[d]);
-                bool dup = false;
-                for (int q = 0; q < cntd[d]; q++) dup |= (nt[d][q] == v);
-                if (!dup) nt[d][cntd[d]++] = v;
-            }
+            tn[d] = wrapi(tt[d] + o[d], g.ntile[d]);
+            for (int op = -1; op < o[d]; op++) dup |= (wrapi(tt[d] + op, g.ntile[d]) == tn[d]);
         }
-        int n = 0;
-        nb_pre[0] = 0;
-        for (int c = 0; c < cntd[2]; c++)
-            for (int b = 0; b < cntd[1]; b++)
-                for (int a = 0; a < cntd[0]; a++) {
-                    int64_t tl = ((int64_t)nt[2][c] * g.ntile[1] + nt[1][b]) * g.ntile[0] + nt[0][a];
-                    int s0 = tstart[tl], s1 = tstart[tl + 1];
-                    nb_start[n] = s0;
-                    nb_pre[n + 1] = nb_pre[n] + (s1 - s0);
-                    n++;
-                }
-        nnb = n;
+        const int64_t tl = ((int64_t)tn[2] * g.ntile[1] + tn[1]) * g.ntile[0] + tn[0];
+        const int s0 = tstart[tl];
+        nb_start[tid] = s0;
+        nb_len[tid] = dup ? 0 : tstart[tl + 1] - s0;
     }
     __syncthreads();
-    const int ncand = nb_pre[nnb];
+    if (tid < 64) {
+        int len = tid < 27 ? nb_len[tid] : 0;
+        int incl = len;
+#pragma unroll
+        for (int off = 1; off < 32; off <<= 1) {
+            int y = __shfl_up(incl, off, 64);
+            if (tid >= off) incl += y;
+        }
+        if (tid < 27) nb_pre[tid + 1] = incl;
+        if (tid == 0) nb_pre[0] = 0;
+    }
+    __syncthreads();
+    const int ncand = nb_pre[27];
 
     // cells of this thread: x = tid % 16, y = (tid / 16) % 8, z = tid / 128 + 2m
     const int cxl = tid & 15, cyl = (tid >> 4) & 7, czl0 = tid >> 7;
@@ -161,10 +166,13 @@ __global__ __launch_bounds__(SPREAD_THREADS) void spread_kernel(LRGeom g, int64_
         int i0[3];
         int64_t sidx = 0;
         if (tid < PB && base + tid < ncand) {
-            int c = base + tid;
-            int nbi = 0;
-            while (nb_pre[nbi + 1] <= c) nbi++;
-            sidx = nb_start[nbi] + (c - nb_pre[nbi]);
+            const int c = base + tid;
+            int lo = 0, hi = 26;  // largest nbi with nb_pre[nbi] <= c
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (nb_pre[mid] <= c) lo = mid; else hi = mid - 1;
+            }
+            sidx = nb_start[lo] + (c - nb_pre[lo]);
             keep = true;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -372,6 +380,59 @@ __global__ __launch_bounds__(256) void interp_kernel(LRGeom g, int64_t Np, const
     for (int d = 0; d < 3; d++) out[d * Np + i] = (double)acc[d].s;
 }
 
+// Type-2 interpolation for half-support m = 4 (8³ stencil): one wavefront per
+// node. Lane (a, b) = (lane % 8, lane / 8) owns grid row (y = i0y + b) at
+// x = i0x + a and walks the 8 z-planes, so each load instruction touches 8 rows
+// of 8 consecutive values; the 64 partial sums are combined by a butterfly
+// reduction (pairwise summation, error O(log₂ 512)·ε per node).
+template <typename CT>
+__global__ __launch_bounds__(256) void interp_kernel_w8(LRGeom g, int64_t Np, const double *__restrict__ nodes,
+                                                        CT beta, const CT *__restrict__ grid,
+                                                        double *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int a = lane & 7, b = lane >> 3;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < Np; i += nwaves) {
+        const double ux = nodes[i] * g.scale[0];
+        const double uy = nodes[Np + i] * g.scale[1];
+        const double uz = nodes[2 * Np + i] * g.scale[2];
+        const int i0x = (int)floor(ux) - 3, i0y = (int)floor(uy) - 3, i0z = (int)floor(uz) - 3;
+        const CT wx = es_kernel<CT>((CT)(((double)(i0x + a) - ux) / 4.0), beta);
+        const CT wy = es_kernel<CT>((CT)(((double)(i0y + b) - uy) / 4.0), beta);
+        // Each lane evaluates all 8 z-weights itself. Broadcasting them from
+        // lanes 0..7 with a wave shuffle was measured to lose precision in the
+        // FP64 build (~1e-7 relative against the oracle); the redundant
+        // evaluation is exact and cheap next to the 24 grid loads per lane.
+        CT wz[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) wz[c] = es_kernel<CT>((CT)(((double)(i0z + c) - uz) / 4.0), beta);
+        const int64_t gx = wrapi(i0x + a, g.G[0]);
+        const int64_t gy = wrapi(i0y + b, g.G[1]);
+        const CT wxy = wx * wy;
+        CT acc0 = 0, acc1 = 0, acc2 = 0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int64_t gz = wrapi(i0z + c, g.G[2]);
+            const int64_t off = (gz * g.G[1] + gy) * g.Gxp + gx;
+            const CT w = wxy * wz[c];
+            acc0 += w * grid[off];
+            acc1 += w * grid[g.cstride + off];
+            acc2 += w * grid[2 * g.cstride + off];
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            acc0 += __shfl_xor(acc0, o, 64);
+            acc1 += __shfl_xor(acc1, o, 64);
+            acc2 += __shfl_xor(acc2, o, 64);
+        }
+        if (lane == 0) {
+            out[i] = (double)acc0;
+            out[Np + i] = (double)acc1;
+            out[2 * Np + i] = (double)acc2;
+        }
+    }
+}
+
 __global__ void add3(int64_t n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ o) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) o[i] = a[i] + b[i];
@@ -494,8 +555,14 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
 
     stage_begin(c, ST_INTERP, s);
     const int32_t *order = nullptr;  // filament order is already spatially local
-    if (vel) interp_kernel<CT><<<nblocks(Np, 256), 256, 0, s>>>(g, Np, order, c->nodes, beta, grid, c->v_lr);
-    if (psi) interp_kernel<CT><<<nblocks(Np, 256), 256, 0, s>>>(g, Np, order, c->nodes, beta, grid2, c->psi_lr);
+    if (g.M == 4) {
+        const unsigned nb = (unsigned)std::min<int64_t>((Np + 3) / 4, 1 << 20);  // 4 nodes (waves) per block
+        if (vel) interp_kernel_w8<CT><<<nb, 256, 0, s>>>(g, Np, c->nodes, beta, grid, c->v_lr);
+        if (psi) interp_kernel_w8<CT><<<nb, 256, 0, s>>>(g, Np, c->nodes, beta, grid2, c->psi_lr);
+    } else {
+        if (vel) interp_kernel<CT><<<nblocks(Np, 256), 256, 0, s>>>(g, Np, order, c->nodes, beta, grid, c->v_lr);
+        if (psi) interp_kernel<CT><<<nblocks(Np, 256), 256, 0, s>>>(g, Np, order, c->nodes, beta, grid2, c->psi_lr);
+    }
     stage_end(c, ST_INTERP, s);
     VPBS_HIP(c, hipGetLastError());
     return VPBS_OK;
