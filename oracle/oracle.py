@@ -69,6 +69,7 @@ def lib():
         L.or_spread.argtypes = [i64, _dp, _dp, _dp, _ip, i32, d, _dp]
         L.or_interp.argtypes = [i64, _dp, _dp, _ip, i32, d, _dp, _dp]
         L.or_cell_dims.argtypes = [d, _dp, i32, _ip]
+        L.or_local_integrals.argtypes = [i64, _dp, i32, _dp, _dp, _dp, i32, d, u, _dp, _dp]
         L.or_pair_hash.argtypes = [i64]
         L.or_pair_hash.restype = ctypes.c_uint64
         _LIB = L
@@ -278,8 +279,10 @@ def compute(pd, params, quantities=("velocity",), LIA=True, shortrange=True, lon
             v += lr["velocity"]
         if "streamfunction" in lr:
             psi += lr["streamfunction"]
-    if shortrange and LIA != "only" and "streamfunction" in quantities:
-        psi += background_vorticity_psi(pd, params)[:, None]
+    if shortrange and LIA != "only":
+        local_integrals(pd, params, flags, v, psi, nodes, periodic=True)
+        if "streamfunction" in quantities:
+            psi += background_vorticity_psi(pd, params)[:, None]
     out = {}
     if "velocity" in quantities:
         out["velocity"] = v
@@ -288,6 +291,43 @@ def compute(pd, params, quantities=("velocity",), LIA=True, shortrange=True, lon
     if neighbour_stats:
         out["stats"] = stats
     return out
+
+
+def _subsegment_points(pd, params, fold: bool):
+    """Quadrature points/charges on the non-LIA parts of the two segments of each node.
+
+    Segment i-1 over ζ ∈ [0, 1-γ] and segment i over ζ ∈ [γ, 1]
+    (local_integrals.jl:103-104,224-241), node-major layout.
+    """
+    g = params.lia_segment_fraction
+    b = pd.batch
+    zs, ws = params.quadrature_near_singularity.rule()
+    nq = len(zs)
+    N = b.Ns[b.fid]
+    pts, chg = [], []
+    for seg, (lo, hi) in (((b.local - 1) % N, (0.0, 1.0 - g)), (b.local, (g, 1.0))):
+        dt = (b.knot(b.fid, seg + 1) - b.knot(b.fid, seg)) * (hi - lo)
+        for z, w in zip(zs, ws):
+            y = np.full(b.Np, lo + (hi - lo) * z)
+            pts.append(b.evaluate(b.fid, seg, y, 0))
+            chg.append((w * dt)[:, None] * b.evaluate(b.fid, seg, y, 1))
+    P = np.stack(pts, axis=1).reshape(-1, 3).T  # (3, 2nq*Np), node-major
+    C = np.stack(chg, axis=1).reshape(-1, 3).T
+    if fold:
+        for d in range(3):
+            L = params.Ls[d]
+            P[d] = np.mod(P[d], L)
+            P[d][P[d] >= L] -= L
+    return 2 * nq, np.ascontiguousarray(P), np.ascontiguousarray(C)
+
+
+def local_integrals(pd, params, flags, v, psi, nodes, periodic=True):
+    if params.lia_segment_fraction is None:
+        return
+    nsub, P, C = _subsegment_points(pd, params, fold=periodic)
+    Ls = np.array(params.Ls, dtype=np.float64)
+    lib().or_local_integrals(nodes.shape[1], _p(nodes), nsub, _p(P), _p(C), _p(Ls), int(periodic), params.Γ, flags,
+                             _p(v), _p(psi))
 
 
 def background_vorticity_psi(pd, params) -> np.ndarray:
@@ -321,6 +361,7 @@ def compute_open_domain(pd, params, quantities=("velocity",), integration_cutoff
     L.or_local_term(Np, _p(_c(pd.sp)), _p(_c(pd.spp)), _p(_c(pd.dlm)), _p(_c(pd.dlp)), params.Γ, cv, cpsi, flags,
                     _p(v), _p(psi))
     L.or_direct_open(Np, _p(nodes), _p(idx, _ip), Nq, _p(pts), _p(chg), nq, params.Γ, flags, _p(v), _p(psi))
+    local_integrals(pd, params, flags, v, psi, nodes, periodic=False)
     out = {}
     if flags & FLAG_VEL:
         out["velocity"] = v

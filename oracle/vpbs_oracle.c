@@ -494,3 +494,40 @@ void or_interp(int64_t Np, const double *nodes, const double *Ls, const int32_t 
         }
     }
 }
+
+/*
+ * add_local_integrals! (local_integrals.jl:250-296): for each node i, add the
+ * full Biot–Savart integrand over nsub sub-segment quadrature points that the
+ * caller built (points spts[3][nsub*Np], charges schg, node-major: node i owns
+ * entries nsub*i .. nsub*i+nsub-1). periodic != 0: min-image separations.
+ */
+void or_local_integrals(int64_t Np, const double *nodes, int nsub, const double *spts, const double *schg,
+                        const double *Ls, int periodic, double Gamma, unsigned flags, double *v, double *psi) {
+    const double pref = Gamma / (4 * M_PI);
+    const int64_t n = (int64_t)nsub * Np;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < Np; i++) {
+        double x[3] = {nodes[i], nodes[Np + i], nodes[2 * Np + i]};
+        double vi[3] = {0, 0, 0}, pi[3] = {0, 0, 0};
+        for (int k = 0; k < nsub; k++) {
+            int64_t j = (int64_t)nsub * i + k;
+            double r[3];
+            for (int d = 0; d < 3; d++) {
+                r[d] = x[d] - spts[d * n + j];
+                if (periodic) r[d] = fold_sep(r[d], Ls[d], Ls[d] / 2);
+            }
+            double r2 = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+            double rinv = 1.0 / sqrt(r2);
+            double q[3] = {schg[j], schg[n + j], schg[2 * n + j]};
+            double f = rinv * rinv * rinv;
+            vi[0] += f * (q[1] * r[2] - q[2] * r[1]);
+            vi[1] += f * (q[2] * r[0] - q[0] * r[2]);
+            vi[2] += f * (q[0] * r[1] - q[1] * r[0]);
+            for (int d = 0; d < 3; d++) pi[d] += rinv * q[d];
+        }
+        if (flags & FLAG_VEL)
+            for (int d = 0; d < 3; d++) v[d * Np + i] += pref * vi[d];
+        if (flags & FLAG_PSI)
+            for (int d = 0; d < 3; d++) psi[d * Np + i] += pref * pi[d];
+    }
+}

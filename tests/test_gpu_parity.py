@@ -216,3 +216,9 @@ def test_tiny_grid_few_tiles(vp, O):
     spl = vp.GaussianSplitting.create(Ls=TWO_PI, α=1.0, rcut=2.0, Ns=10)
     params = vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, splitting=spl)
     _check(vp, O, params, fs)
+
+
+def test_lia_segment_fraction(vp, O, cfg2):
+    """lia_segment_fraction = 0.2 with GL(8) near the singularity (host-side local integrals on both sides)."""
+    params = _params(vp, Ns=128, lia_segment_fraction=0.2, quadrature_near_singularity=vp.GaussLegendre(8))
+    _check(vp, O, params, cfg2)

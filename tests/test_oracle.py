@@ -244,3 +244,35 @@ def test_planted_mistake_lia_cutoff(vp, O, monkeypatch):
     """Using exp(Δ-1)a/2 for the velocity cut-off too (a plausible slip) breaks the ring velocity."""
     monkeypatch.setattr(O, "local_cutoffs", lambda params, ic=None: (math.exp(params.Δ - 1) * params.a / 2,) * 2)
     assert periodic_ring_velocity_error(vp, O) > 1e-2
+
+
+def _ring_open(vp, O, N, quad, gamma, qns=None, Γ=2.4, a=1e-6, Δ=0.25, R=math.pi / 3):
+    f = vp.filaments.init(vp.define_curve(vp.Ring(), scale=R, translate=math.pi), N)
+    spl = vp.GaussianSplitting.create(Ls=TWO_PI, β=3.5, Ns=64)
+    p = vp.ParamsBiotSavart.build(np.float64, Γ=Γ, a=a, Δ=Δ, splitting=spl, quadrature=vp.GaussLegendre(quad),
+                                  quadrature_near_singularity=qns, lia_segment_fraction=gamma)
+    pd = vp.add_point_charges([f], p)
+    return p, pd, O.compute_open_domain(pd, p, ("velocity", "streamfunction"))
+
+
+def test_ring_closed_forms_lia_segment_fraction(vp, O):
+    """test/ring.jl:42-121 with lia_segment_fraction = 0.4, GL(4), N = 32: U, ψ, E within 1e-4."""
+    R, Γ, a, Δ = math.pi / 3, 2.4, 1e-6, 0.25
+    p, pd, out = _ring_open(vp, O, 32, 4, 0.4)
+    v, psi = out["velocity"], out["streamfunction"]
+    U = np.linalg.norm(v.mean(axis=1))
+    assert U == pytest.approx(Γ / (4 * math.pi * R) * (math.log(8 * R / a) - Δ), rel=1e-4)
+    t = pd.sp / np.linalg.norm(pd.sp, axis=0)
+    assert (psi * t).sum(0).mean() == pytest.approx(Γ / (2 * math.pi) * (math.log(8 * R / a) - (1 + Δ)), rel=1e-4)
+    E = O.energy_from_streamfunction(pd, psi, p, open_domain=True)
+    assert E == pytest.approx((Γ ** 2 * R / 2) * (math.log(8 * R / a) - (Δ + 1)), rel=1e-4)
+
+
+def test_lia_segment_fraction_convergence(vp, O):
+    """test/ring.jl:219-260: γ = 0.2, GL(8) near the singularity, N = 64: σ(v_z)/v_z < 1e-12, error < 2e-6."""
+    R, Γ, a, Δ = math.pi / 3, 1.0, 1e-8, 0.25
+    p, pd, out = _ring_open(vp, O, 64, 3, 0.2, qns=vp.GaussLegendre(8), Γ=Γ, a=a)
+    vz = out["velocity"][2]
+    assert np.std(vz) / np.mean(vz) < 1e-12
+    U = Γ / (4 * math.pi * R) * (math.log(8 * R / a) - Δ)
+    assert abs(np.mean(vz) - U) / U < 2e-6

@@ -16,8 +16,9 @@ Mirrors ``src/BiotSavart/BiotSavart.jl``:
 The work split is the reference's: point data (quadrature points, charges,
 derivatives on nodes) is built on the host from the filament splines
 (``add_point_charges!``, ``pointdata_impl.jl``); the short-range and long-range
-parts run on the GPU through libvpbs; the background-vorticity correction of ψ
-for unclosed filaments stays on the host (``BiotSavart.jl:255-286``).
+parts run on the GPU through libvpbs; the ``lia_segment_fraction`` local
+integrals and the background-vorticity correction of ψ for unclosed filaments
+stay on the host (``BiotSavart.jl:255-286,570-580``).
 Errors from the library raise ``VpbsError``; invalid arguments raise
 ``ValueError`` like the reference's ``ArgumentError``/``DimensionMismatch``.
 """
@@ -30,6 +31,7 @@ import numpy as np
 
 from . import _lib
 from .filaments import ClosedFilament, FilamentBatch
+from .local_integrals import add_local_integrals
 from .params import ParamsBiotSavart
 from .pointdata import PointData, add_point_charges
 
@@ -120,8 +122,11 @@ def compute_on_nodes_(
         flags |= _lib.NEIGHBOUR_STATS
     v, psi = cache.ctx.compute(flags, integration_cutoff)
     cache.timings = cache.ctx.timings()
-    if psi is not None and shortrange and LIA != "only":
-        psi = psi + background_vorticity_correction(pd, params)[:, None]
+    if shortrange and LIA != "only":
+        # host-side terms, as in the reference (BiotSavart.jl:570-580)
+        add_local_integrals(v, psi, pd, params)
+        if psi is not None:
+            psi += background_vorticity_correction(pd, params)[:, None]
     for name, arr in (("velocity", v), ("streamfunction", psi)):
         if name in fields:
             lin = arr.T
