@@ -39,12 +39,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(vp, cfg: int):
+def build_workload(vp, cfg: int, sigma: float = 1.5, m: int = 4):
     fs = vp.tangles.config_tangle(cfg)
     Ns = {1: 64, 2: 128, 3: 512, 4: 256}[cfg]
     spl = vp.GaussianSplitting.create(Ls=2 * math.pi, β=3.5, Ns=Ns)
-    p32 = vp.ParamsBiotSavart.build(np.float32, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl)
-    p64 = vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl)
+    bl = vp.HIPLongRangeBackend(σ=sigma, m=m)
+    p32 = vp.ParamsBiotSavart.build(np.float32, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl, backend_long=bl)
+    p64 = vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl, backend_long=bl)
     batch = vp.FilamentBatch(fs)
     pd = vp.add_point_charges(fs, p64, batch=batch)
     return fs, spl, p32, p64, pd
@@ -172,6 +173,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3, 4])
+    ap.add_argument("--sigma", type=float, default=1.5, help="NUFFT oversampling factor")
+    ap.add_argument("--m", type=int, default=4, help="NUFFT kernel half-support")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 path (and the error figure)")
     args = ap.parse_args()
@@ -182,7 +185,7 @@ def main():
 
     vp = ge.load_package()
     t = time.perf_counter()
-    fs, spl, p32, p64, pd = build_workload(vp, args.config)
+    fs, spl, p32, p64, pd = build_workload(vp, args.config, args.sigma, args.m)
     log(f"workload: {len(fs)} filaments, Np = {pd.Np}, Nq = {pd.Nq}, Ns = {spl.Ns}, α = {spl.α:.4f}, "
         f"rcut = {spl.rcut:.5f} (host prep {time.perf_counter() - t:.1f} s)")
 
@@ -219,9 +222,11 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"config {args.config}: {len(fs)} random vortex rings, {pd.Np} nodes, {pd.Nq} quadrature "
-                        f"points, Fourier grid {spl.Ns[0]}^3 (NUFFT grid {grid[0]}^3), beta 3.5, FP32-bulk path",
+                        f"points, Fourier grid {spl.Ns[0]}^3 (NUFFT grid {grid[0]}^3, sigma {args.sigma}, m {args.m}), beta 3.5, "
+                        f"FP32-bulk path",
             "nodes": pd.Np, "quadrature_points": pd.Nq, "fourier_grid": list(spl.Ns),
-            "nufft_grid": list(grid), "cells": list(cells), "alpha": spl.α, "rcut": spl.rcut,
+            "nufft_grid": list(grid), "nufft_sigma": args.sigma, "nufft_m": args.m, "cells": list(cells),
+            "alpha": spl.α, "rcut": spl.rcut,
         },
         "roofline": {
             "kernel": f"{dom}_kernel<float>",

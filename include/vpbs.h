@@ -90,6 +90,7 @@ typedef struct vpbs_timings {
     int64_t pair_candidates; /* unused (reserved) */
     int64_t grid[3];     /* oversampled NUFFT grid */
     int64_t cells[3];    /* cell-list cells per direction */
+    double point_data;   /* kernels of the last vpbs_set_filaments (0 after vpbs_set_points*) */
 } vpbs_timings;
 
 typedef struct vpbs_ctx vpbs_ctx;
@@ -128,6 +129,33 @@ int vpbs_set_points(vpbs_ctx *ctx, int64_t Np, const double *nodes, const int32_
 int vpbs_set_points_device(vpbs_ctx *ctx, int64_t Np, const double *nodes, const int32_t *idx_prev,
                            const double *sp, const double *spp, const double *dlm, const double *dlp, int64_t Nq,
                            const double *points, const double *charges);
+
+/*
+ * Build the point data on the device from the filaments' spline description:
+ * the device counterpart of add_point_charges! (pointdata_impl.jl:121-219),
+ * replacing the host loop and the H2D copy of its ~240 B/node output by an
+ * upload of ~56 B/node. The result is what vpbs_set_points would upload, and
+ * is checked the same way (folded into [0, L), idx_prev in range).
+ *   Nf                number of filaments; counts[Nf] nodes of each (≥ 5), Np = Σ counts;
+ *                     nodes of filament f are f's block of the linear node index
+ *   X[3*Np]           node positions, not necessarily folded
+ *   knots[Np]         spline parameter t_i of each node, t = 0 at each filament's
+ *                     first node (chordal knots, spline.jl); periods[Nf] knot period T_f
+ *   offsets[3*Nf]     end-to-end offset of each filament (0 when closed; periodic lines)
+ *   coefs[3*Np]       B-spline coefficients of the periodised curve X(t) - (t/T)·offset
+ *                     (QuinticSplineMethod, Filaments/spline/quintic.jl; index j ↔ B_{j-5,6})
+ *   quad_nodes, quad_weights[nquad]  quadrature rule on [0, 1] (GaussLegendre);
+ *                     both NULL for NoQuadrature (straight segments, nquad = 1)
+ *   lia_segment_fraction  NaN for `nothing`, else γ ∈ (0, 1]: δ± then cover the
+ *                     fraction γ of each segment (local_integrals.jl:100-101)
+ */
+int vpbs_set_filaments(vpbs_ctx *ctx, int64_t Nf, const int64_t *counts, const double *X, const double *knots,
+                       const double *periods, const double *offsets, const double *coefs, const double *quad_nodes,
+                       const double *quad_weights, double lia_segment_fraction);
+
+/* Copy the device point data (from vpbs_set_points* or vpbs_set_filaments) to host arrays; any may be NULL. */
+int vpbs_get_points(vpbs_ctx *ctx, double *nodes, int32_t *idx_prev, double *sp, double *spp, double *dlm,
+                    double *dlp, double *points, double *charges);
 
 /*
  * Compute velocity and/or streamfunction on nodes: the device part of

@@ -94,10 +94,10 @@ def es_beta(m: int, sigma: float) -> float:
 
 
 def oversampled_size(N: int, sigma: float, m: int = 4) -> int:
-    """Smallest multiple of 16 of the form 2^a·3^b·5^c that is ≥ ⌈σN⌉ (and ≥ 4m)."""
+    """Smallest multiple of 32 of the form 2^a·3^b·5^c that is ≥ ⌈σN⌉ (and ≥ 4m)."""
     n = int(math.ceil(sigma * N - 1e-9))
     while True:
-        if n % 16 == 0:
+        if n % 32 == 0:
             r = n
             for p in (2, 3, 5):
                 while r % p == 0:
@@ -106,7 +106,7 @@ def oversampled_size(N: int, sigma: float, m: int = 4) -> int:
                 break
         n += 1
     if n < 4 * m:
-        n = ((4 * m + 15) // 16) * 16
+        n = ((4 * m + 31) // 32) * 32
     return n
 
 

@@ -32,6 +32,8 @@ EXPORTED = (
     "vpbs_last_error",
     "vpbs_set_points",
     "vpbs_set_points_device",
+    "vpbs_set_filaments",
+    "vpbs_get_points",
     "vpbs_compute",
     "vpbs_get_outputs",
     "vpbs_output_device_ptrs",
@@ -80,6 +82,7 @@ class VpbsTimings(ctypes.Structure):
         ("pair_candidates", ctypes.c_int64),
         ("grid", ctypes.c_int64 * 3),
         ("cells", ctypes.c_int64 * 3),
+        ("point_data", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -119,6 +122,10 @@ def load() -> ctypes.CDLL:
     L.vpbs_set_points.restype = ctypes.c_int
     L.vpbs_set_points_device.argtypes = pts_sig
     L.vpbs_set_points_device.restype = ctypes.c_int
+    L.vpbs_set_filaments.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, d]
+    L.vpbs_set_filaments.restype = ctypes.c_int
+    L.vpbs_get_points.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.vpbs_get_points.restype = ctypes.c_int
     L.vpbs_compute.argtypes = [vp, u32, d, vp, vp]
     L.vpbs_compute.restype = ctypes.c_int
     L.vpbs_get_outputs.argtypes = [vp, vp, vp]
@@ -209,6 +216,47 @@ class Context:
         )
         self._check(rc, "vpbs_set_points")
         self.Np = Np
+
+    def set_filaments(self, batch, params):
+        """Build the point data on the device from a FilamentBatch (vpbs_set_filaments).
+
+        Takes the batch's nodes, knots, periods, end-to-end offsets and spline
+        coefficients; the quadrature rule and ``lia_segment_fraction`` come from
+        ``params`` (NoQuadrature → straight segments).
+        """
+        from .params import NoQuadrature
+
+        counts = np.ascontiguousarray(batch.Ns, dtype=np.int64)
+        X = np.ascontiguousarray(batch.X.T, dtype=np.float64)
+        ts = np.ascontiguousarray(batch.ts, dtype=np.float64)
+        T = np.ascontiguousarray(batch.T, dtype=np.float64)
+        xoff = np.ascontiguousarray(batch.Xoff.T, dtype=np.float64)
+        c = np.ascontiguousarray(batch.c.T, dtype=np.float64)
+        quad = params.quadrature
+        if isinstance(quad, NoQuadrature):
+            zs = ws = None
+        else:
+            zs, ws = (np.ascontiguousarray(a, dtype=np.float64) for a in quad.rule())
+        g = params.lia_segment_fraction
+        rc = self._L.vpbs_set_filaments(
+            self._h, len(counts), _ptr(counts), _ptr(X), _ptr(ts), _ptr(T), _ptr(xoff), _ptr(c),
+            _ptr(zs), _ptr(ws), math.nan if g is None else float(g),
+        )
+        self._check(rc, "vpbs_set_filaments")
+        self.Np = int(counts.sum())
+
+    def get_points(self):
+        """Device point data as a dict of host arrays (vpbs_get_points)."""
+        Np = self.Np
+        nq = self._p.nquad
+        out = dict(
+            nodes=np.empty((3, Np)), node_idx_prev=np.empty(Np, dtype=np.int32), sp=np.empty((3, Np)),
+            spp=np.empty((3, Np)), dlm=np.empty(Np), dlp=np.empty(Np), points=np.empty((3, nq * Np)),
+            charges=np.empty((3, nq * Np)),
+        )
+        rc = self._L.vpbs_get_points(self._h, *(_ptr(a) for a in out.values()))
+        self._check(rc, "vpbs_get_points")
+        return out
 
     def compute(self, flags: int, integration_cutoff=None, fetch=True):
         """Run vpbs_compute. Returns (v, psi) as (3, Np) arrays (None where not requested / not fetched)."""

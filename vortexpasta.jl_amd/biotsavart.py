@@ -13,12 +13,15 @@ Mirrors ``src/BiotSavart/BiotSavart.jl``:
 * ``velocity_on_nodes_(vs, cache, fs)`` / ``velocity_on_nodes(cache, fs)`` —
   ``BiotSavart.jl:94-161``.
 
-The work split is the reference's: point data (quadrature points, charges,
-derivatives on nodes) is built on the host from the filament splines
-(``add_point_charges!``, ``pointdata_impl.jl``); the short-range and long-range
-parts run on the GPU through libvpbs; the ``lia_segment_fraction`` local
-integrals and the background-vorticity correction of ψ for unclosed filaments
-stay on the host (``BiotSavart.jl:255-286,570-580``).
+Work split: the spline coefficients of the filaments are solved on the host
+(as the reference's ``Filaments`` module does); the point data (quadrature
+points, charges, derivatives on nodes, ``add_point_charges!``,
+``pointdata_impl.jl``) is built from them on the GPU by default
+(``pointdata="device"``, ``vpbs_set_filaments``) or on the host and uploaded
+(``pointdata="host"``, ``vpbs_set_points``, the reference's split); the
+short-range and long-range parts run on the GPU through libvpbs; the
+``lia_segment_fraction`` local integrals and the background-vorticity correction
+of ψ for unclosed filaments stay on the host (``BiotSavart.jl:255-286,570-580``).
 Errors from the library raise ``VpbsError``; invalid arguments raise
 ``ValueError`` like the reference's ``ArgumentError``/``DimensionMismatch``.
 """
@@ -48,11 +51,19 @@ _FIELDS = ("velocity", "streamfunction")
 
 
 class BiotSavartCache:
-    """Parameters + GPU context + last point data (``cache.jl:238-250``)."""
+    """Parameters + GPU context + last point data (``cache.jl:238-250``).
 
-    def __init__(self, params: ParamsBiotSavart, device: int = 0):
+    ``pointdata``: ``"device"`` builds the point data on the GPU from the
+    filament splines, ``"host"`` builds it with :func:`add_point_charges` and
+    uploads it (then kept in ``self.pointdata``).
+    """
+
+    def __init__(self, params: ParamsBiotSavart, device: int = 0, pointdata: str = "device"):
+        if pointdata not in ("device", "host"):
+            raise ValueError("pointdata must be 'device' or 'host'")
         self.params = params
         self.ctx = _lib.Context(params, device=device)
+        self.pointdata_mode = pointdata
         self.pointdata: Optional[PointData] = None
         self.timings: Dict[str, float] = {}
 
@@ -60,14 +71,14 @@ class BiotSavartCache:
         self.ctx.close()
 
 
-def init_cache(params: ParamsBiotSavart, device: int = 0) -> BiotSavartCache:
-    return BiotSavartCache(params, device=device)
+def init_cache(params: ParamsBiotSavart, device: int = 0, pointdata: str = "device") -> BiotSavartCache:
+    return BiotSavartCache(params, device=device, pointdata=pointdata)
 
 
-def background_vorticity_correction(pd: PointData, params: ParamsBiotSavart) -> np.ndarray:
+def background_vorticity_correction(batch: FilamentBatch, params: ParamsBiotSavart) -> np.ndarray:
     """Uniform ψ shift −(Γ/V)Σ offsets/(4α²) when the filaments' mean vorticity is non-zero."""
     Ls = np.asarray(params.Ls, dtype=np.float64)
-    ptot = pd.batch.Xoff.sum(axis=0)
+    ptot = batch.Xoff.sum(axis=0)
     if np.allclose(ptot + Ls, Ls, rtol=np.sqrt(np.finfo(float).eps), atol=0):
         return np.zeros(3)
     wmean = ptot * (params.Γ / np.prod(Ls))
@@ -114,9 +125,14 @@ def compute_on_nodes_(
         if len(us) != len(fs) or any(np.shape(u) != (len(f), 3) for u, f in zip(us, fs)):
             raise ValueError(f"wrong dimensions of output vector '{name}'")
     params = cache.params
-    pd = add_point_charges(fs, params)
-    cache.pointdata = pd
-    cache.ctx.set_points(pd)
+    batch = FilamentBatch(fs)
+    if cache.pointdata_mode == "device":
+        cache.pointdata = None
+        cache.ctx.set_filaments(batch, params)
+    else:
+        pd = add_point_charges(fs, params, batch)
+        cache.pointdata = pd
+        cache.ctx.set_points(pd)
     flags = _flags(fields, LIA, shortrange, longrange)
     if neighbour_stats:
         flags |= _lib.NEIGHBOUR_STATS
@@ -124,13 +140,13 @@ def compute_on_nodes_(
     cache.timings = cache.ctx.timings()
     if shortrange and LIA != "only":
         # host-side terms, as in the reference (BiotSavart.jl:570-580)
-        add_local_integrals(v, psi, pd, params)
+        add_local_integrals(v, psi, batch, params)
         if psi is not None:
-            psi += background_vorticity_correction(pd, params)[:, None]
+            psi += background_vorticity_correction(batch, params)[:, None]
     for name, arr in (("velocity", v), ("streamfunction", psi)):
         if name in fields:
             lin = arr.T
-            for u, a, n in zip(fields[name], pd.batch.offs, pd.batch.Ns):
+            for u, a, n in zip(fields[name], batch.offs, batch.Ns):
                 u[...] = lin[a : a + n]
     return fields
 

@@ -1,0 +1,370 @@
+// Device-side point data: the work of add_point_charges!
+// (src/BiotSavart/pointdata_impl.jl:121-188) on the GPU, from the filaments'
+// periodic quintic splines (src/Filaments/spline/spline.jl:116-319).
+//
+// Inputs (uploaded by vpbs_set_filaments): node positions X, knots t_i
+// (t = 0 at each filament's first node), knot periods T, end-to-end offsets
+// Xoff and the B-spline coefficients c of the periodised curve
+// Y(t) = X(t) - (t/T)·Xoff (spline.jl:163-193). The derivative coefficients
+//   d_j = 5(c_j - c_{j-1})/(t_{j+5} - t_j),  e_j = 4(d_j - d_{j-1})/(t_{j+4} - t_j)
+// (spline_derivative!, src/Filaments/spline/spline.jl:298-319) give S′ and S″
+// as order-5 and order-4 splines. One thread per node then
+// writes the folded node, idx_prev, s′, s″, δ± (Gauss–Legendre integral of |S′|
+// over the LIA fraction of the two adjacent segments, pointdata_impl.jl:133-143)
+// and the nquad quadrature points (folded) and charges w·Δt·S′(ζ) of its right
+// segment (pointdata_impl.jl:173-183), in the layout of vpbs_set_points.
+#include "vpbs_internal.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace vpbs {
+
+// No FMA contraction in this file: the point data comes out bit-identical to a
+// plain IEEE evaluation of the same formulas (numpy's). This matters beyond
+// rounding: a quadrature point lying exactly on a NUFFT grid plane (a ring in
+// the plane z = L/2, say) selects its 2m-point ES stencil by floor(x·G/L), and
+// a 1-ulp change there moves a weight φ(±1) = e^{-β} (≈ 1e-7 at m = 4) from one
+// end of the stencil to the other.
+#pragma clang fp contract(off)
+
+namespace {
+
+inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+struct Fil {
+    const int32_t *fid;    // [Np] filament of each node
+    const int64_t *off;    // [Nf] first node
+    const int32_t *N;      // [Nf] nodes per filament
+    const double *T;       // [Nf] knot period
+    const double *Xoff;    // [3*Nf] end-to-end offset
+    const double *ts;      // [Np] knots
+    int64_t Np, Nf;
+};
+
+__device__ inline double knot(const Fil &f, int64_t off, int N, double T, int j) {
+    int q = j / N, r = j - q * N;
+    if (r < 0) {
+        r += N;
+        q -= 1;
+    }
+    return f.ts[off + r] + q * T;
+}
+
+// Non-zero order-K B-splines at t ∈ [t_i, t_{i+1}): B[r] = B_{i-K+1+r,K}(t).
+template <int K>
+__device__ inline void basis(const Fil &f, int64_t off, int N, double T, int i, double t, double *B) {
+    double left[K], right[K];
+    B[0] = 1.0;
+#pragma unroll
+    for (int j = 1; j < K; j++) {
+        left[j] = t - knot(f, off, N, T, i + 1 - j);
+        right[j] = knot(f, off, N, T, i + j) - t;
+        double saved = 0.0;
+#pragma unroll
+        for (int r = 0; r < j; r++) {
+            const double temp = B[r] / (right[r + 1] + left[j - r]);
+            B[r] = saved + right[r + 1] * temp;
+            saved = left[j - r] * temp;
+        }
+        B[j] = saved;
+    }
+}
+
+// S^(deriv)(t) on segment `seg` at local coordinate ζ, coefficient array `co`
+// (c for deriv 0, d for 1, e for 2), with the periodisation undone.
+template <int DERIV>
+__device__ inline void eval_spline(const Fil &f, int fi, int seg, double zeta, const double *co, double *out) {
+    constexpr int K = 6 - DERIV;
+    const int64_t off = f.off[fi];
+    const int N = f.N[fi];
+    const double T = f.T[fi];
+    const double ti = knot(f, off, N, T, seg), ti1 = knot(f, off, N, T, seg + 1);
+    const double t = (1 - zeta) * ti + zeta * ti1;
+    double B[K];
+    basis<K>(f, off, N, T, seg, t, B);
+    double s[3] = {0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        int j = seg - K + 1 + r;
+        j %= N;
+        j += (j < 0) ? N : 0;
+        const int64_t idx = off + j;
+#pragma unroll
+        for (int d = 0; d < 3; d++) s[d] += B[r] * co[d * f.Np + idx];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double xo = f.Xoff[d * f.Nf + fi];
+        if (DERIV == 0) s[d] += (t / T) * xo;
+        if (DERIV == 1) s[d] += xo / T;
+        out[d] = s[d];
+    }
+}
+
+// fold_coordinates_periodic (min_distance.jl:5-20): the x < 0 loop first, so
+// that a tiny negative value ends at 0, never at L. Values more than 64
+// periods away are first reduced by the exact fmod (the loops alone would not
+// end for |x| ≫ L); non-finite values become NaN, which the folded-range check
+// after the build reports as an error.
+__device__ inline double fold(double x, double L) {
+    if (!isfinite(x)) return __builtin_nan("");
+    if (fabs(x) > 64.0 * L) x = fmod(x, L);
+    while (x < 0) x += L;
+    while (x >= L) x -= L;
+    return x;
+}
+
+__global__ void derivative_coefs(Fil f, const double *__restrict__ c, double *__restrict__ dco) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= f.Np) return;
+    const int fi = f.fid[i];
+    const int64_t off = f.off[fi];
+    const int N = f.N[fi];
+    const double T = f.T[fi];
+    const int l = (int)(i - off);
+    const int64_t ip = off + (l == 0 ? N - 1 : l - 1);
+    const double den = knot(f, off, N, T, l + 5) - knot(f, off, N, T, l);
+#pragma unroll
+    for (int d = 0; d < 3; d++) dco[d * f.Np + i] = 5.0 * (c[d * f.Np + i] - c[d * f.Np + ip]) / den;
+}
+
+__global__ void second_derivative_coefs(Fil f, const double *__restrict__ dco, double *__restrict__ eco) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= f.Np) return;
+    const int fi = f.fid[i];
+    const int64_t off = f.off[fi];
+    const int N = f.N[fi];
+    const double T = f.T[fi];
+    const int l = (int)(i - off);
+    const int64_t ip = off + (l == 0 ? N - 1 : l - 1);
+    const double den = knot(f, off, N, T, l + 4) - knot(f, off, N, T, l);
+#pragma unroll
+    for (int d = 0; d < 3; d++) eco[d * f.Np + i] = 4.0 * (dco[d * f.Np + i] - dco[d * f.Np + ip]) / den;
+}
+
+// Filament of each node: the last filament whose first node is ≤ i.
+__global__ void node_filament(const int64_t *__restrict__ off, int64_t Nf, int64_t Np, int32_t *__restrict__ fid) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np) return;
+    int64_t lo = 0, hi = Nf - 1;  // off[0] = 0 ≤ i
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) / 2;
+        if (off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    fid[i] = (int32_t)lo;
+}
+
+struct Quad {
+    int n;                      // quadrature points per segment (1 for straight segments)
+    bool straight;              // NoQuadrature: midpoint of straight segments (pointdata_impl.jl:190-219)
+    double gamma;               // lia_segment_fraction (1 if unset) for straight segments
+    double z[16], w[16];        // rule on [0, 1]
+    double lim_m[2], lim_p[2];  // integration limits of δ⁻ (segment i-1) and δ⁺ (segment i)
+};
+
+__global__ void point_data_kernel(Fil f, Quad qd, Box box, const double *__restrict__ X, const double *__restrict__ c,
+                                  const double *__restrict__ dco, const double *__restrict__ eco,
+                                  double *__restrict__ nodes, int32_t *__restrict__ idx_prev, double *__restrict__ sp,
+                                  double *__restrict__ spp, double *__restrict__ dlm, double *__restrict__ dlp,
+                                  double *__restrict__ pts, double *__restrict__ chg) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t Np = f.Np;
+    if (i >= Np) return;
+    const int fi = f.fid[i];
+    const int64_t off = f.off[fi];
+    const int N = f.N[fi];
+    const double T = f.T[fi];
+    const int l = (int)(i - off);
+    const int64_t ip = l == 0 ? off + N - 1 : i - 1;
+    for (int d = 0; d < 3; d++) nodes[d * Np + i] = fold(X[d * Np + i], box.L[d]);
+    idx_prev[i] = (int32_t)ip;
+    double v[3];
+    eval_spline<1>(f, fi, l, 0.0, dco, v);
+    for (int d = 0; d < 3; d++) sp[d * Np + i] = v[d];
+    eval_spline<2>(f, fi, l, 0.0, eco, v);
+    for (int d = 0; d < 3; d++) spp[d * Np + i] = v[d];
+    if (qd.straight) {
+        // f[i-1], f[i+1] of the periodically extended filament
+        const int64_t in = l == N - 1 ? off : i + 1;
+        double xi[3], xp[3], xn[3];
+        for (int d = 0; d < 3; d++) {
+            const double xo = f.Xoff[d * f.Nf + fi];
+            xi[d] = X[d * Np + i];
+            xp[d] = X[d * Np + ip] - (l == 0 ? xo : 0.0);
+            xn[d] = X[d * Np + in] + (l == N - 1 ? xo : 0.0);
+        }
+        double am = 0, ap = 0;
+        for (int d = 0; d < 3; d++) {
+            am += (xi[d] - xp[d]) * (xi[d] - xp[d]);
+            ap += (xn[d] - xi[d]) * (xn[d] - xi[d]);
+        }
+        dlm[i] = qd.gamma * sqrt(am);
+        dlp[i] = qd.gamma * sqrt(ap);
+        for (int d = 0; d < 3; d++) {
+            pts[d * Np + i] = fold((xi[d] + xn[d]) / 2, box.L[d]);
+            chg[d * Np + i] = xn[d] - xi[d];
+        }
+        return;
+    }
+    // δ⁻, δ⁺ (integrate.jl:40-61 with Quadratures.integrate over (a, b))
+    for (int side = 0; side < 2; side++) {
+        const int seg = side == 0 ? (l == 0 ? N - 1 : l - 1) : l;
+        const double a = side == 0 ? qd.lim_m[0] : qd.lim_p[0];
+        const double b = side == 0 ? qd.lim_m[1] : qd.lim_p[1];
+        const double delta = b - a;
+        const double dt = knot(f, off, N, T, seg + 1) - knot(f, off, N, T, seg);
+        double acc = 0.0;
+        for (int q = 0; q < qd.n; q++) {
+            eval_spline<1>(f, fi, seg, a + delta * qd.z[q], dco, v);
+            acc = acc + qd.w[q] * sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        }
+        (side == 0 ? dlm : dlp)[i] = dt * (delta * acc);
+    }
+    // quadrature points and charges of segment l
+    const double dt = knot(f, off, N, T, l + 1) - knot(f, off, N, T, l);
+    const int64_t Nq = (int64_t)qd.n * Np;
+    for (int q = 0; q < qd.n; q++) {
+        const int64_t j = (int64_t)qd.n * i + q;
+        eval_spline<0>(f, fi, l, qd.z[q], c, v);
+        for (int d = 0; d < 3; d++) pts[d * Nq + j] = fold(v[d], box.L[d]);
+        eval_spline<1>(f, fi, l, qd.z[q], dco, v);
+        const double wq = qd.w[q] * dt;
+        for (int d = 0; d < 3; d++) chg[d * Nq + j] = wq * v[d];
+    }
+}
+
+}  // namespace
+
+// Upload the filament description, then derive the point data on the device.
+// Argument meaning as in vpbs_set_filaments (include/vpbs.h); the host checks
+// done here make every index the kernels form stay inside its array.
+int build_point_data(vpbs_ctx *c, int64_t Nf, const int64_t *counts, const double *X, const double *knots,
+                     const double *periods, const double *offsets, const double *coefs, const double *quad_nodes,
+                     const double *quad_weights, double lia_segment_fraction) {
+    if (Nf < 1) return fail(c, VPBS_ERR_INVALID, "need at least one filament (got Nf = %lld)", (long long)Nf);
+    if (!counts || !X || !knots || !periods || !offsets || !coefs)
+        return fail(c, VPBS_ERR_INVALID, "null input array");
+    if ((quad_nodes == nullptr) != (quad_weights == nullptr))
+        return fail(c, VPBS_ERR_INVALID, "quad_nodes and quad_weights must both be given or both be NULL");
+    const bool straight = quad_nodes == nullptr;
+    const int nq = c->p.nquad;
+    if (straight && nq != 1)
+        return fail(c, VPBS_ERR_INVALID, "straight segments (no quadrature rule) need nquad = 1 (got %d)", nq);
+    if (!std::isnan(lia_segment_fraction) && !(lia_segment_fraction > 0.0 && lia_segment_fraction <= 1.0))
+        return fail(c, VPBS_ERR_INVALID, "lia_segment_fraction must be in (0, 1] (got %g)", lia_segment_fraction);
+    std::vector<int64_t> off((size_t)Nf);
+    std::vector<int32_t> nn((size_t)Nf);
+    int64_t Np = 0;
+    for (int64_t f = 0; f < Nf; f++) {
+        if (counts[f] < 5 || counts[f] > INT32_MAX)
+            return fail(c, VPBS_ERR_INVALID, "filament %lld has %lld nodes; quintic splines need at least 5",
+                        (long long)f, (long long)counts[f]);
+        off[(size_t)f] = Np;
+        nn[(size_t)f] = (int32_t)counts[f];
+        Np += counts[f];
+        if (Np > INT32_MAX) return fail(c, VPBS_ERR_INVALID, "too many nodes for 32-bit indices");
+        if (!(periods[f] > 0.0) || !std::isfinite(periods[f]))
+            return fail(c, VPBS_ERR_INVALID, "period of filament %lld is %g (must be finite and > 0)",
+                        (long long)f, periods[f]);
+    }
+    const int64_t Nq = (int64_t)nq * Np;
+    if (Nq > INT32_MAX / 2) return fail(c, VPBS_ERR_INVALID, "too many quadrature points for 32-bit indices");
+    VPBS_HIP(c, hipSetDevice(c->device));
+    c->have_points = false;
+    VPBS_TRY(ensure_capacity(c, Np, Nq));
+    if (Np > c->cap_fil_np) {
+        double **bufs[] = {&c->fil_x, &c->fil_c, &c->fil_d, &c->fil_e};
+        for (double **b : bufs) {
+            if (*b) VPBS_HIP(c, hipFree(*b));
+            *b = nullptr;
+            VPBS_HIP(c, hipMalloc((void **)b, sizeof(double) * 3 * (size_t)Np));
+        }
+        if (c->fil_ts) VPBS_HIP(c, hipFree(c->fil_ts));
+        if (c->fil_fid) VPBS_HIP(c, hipFree(c->fil_fid));
+        c->fil_ts = nullptr;
+        c->fil_fid = nullptr;
+        VPBS_HIP(c, hipMalloc((void **)&c->fil_ts, sizeof(double) * (size_t)Np));
+        VPBS_HIP(c, hipMalloc((void **)&c->fil_fid, sizeof(int32_t) * (size_t)Np));
+        c->cap_fil_np = Np;
+    }
+    if (Nf > c->cap_fil_nf) {
+        if (c->fil_off) VPBS_HIP(c, hipFree(c->fil_off));
+        if (c->fil_n) VPBS_HIP(c, hipFree(c->fil_n));
+        if (c->fil_T) VPBS_HIP(c, hipFree(c->fil_T));
+        if (c->fil_xoff) VPBS_HIP(c, hipFree(c->fil_xoff));
+        c->fil_off = nullptr;
+        c->fil_n = nullptr;
+        c->fil_T = c->fil_xoff = nullptr;
+        VPBS_HIP(c, hipMalloc((void **)&c->fil_off, sizeof(int64_t) * (size_t)Nf));
+        VPBS_HIP(c, hipMalloc((void **)&c->fil_n, sizeof(int32_t) * (size_t)Nf));
+        VPBS_HIP(c, hipMalloc((void **)&c->fil_T, sizeof(double) * (size_t)Nf));
+        VPBS_HIP(c, hipMalloc((void **)&c->fil_xoff, sizeof(double) * 3 * (size_t)Nf));
+        c->cap_fil_nf = Nf;
+    }
+    hipStream_t s = c->s_sr;
+    const size_t d3 = sizeof(double) * 3 * (size_t)Np;
+    VPBS_HIP(c, hipMemcpyAsync(c->fil_x, X, d3, hipMemcpyHostToDevice, s));
+    VPBS_HIP(c, hipMemcpyAsync(c->fil_c, coefs, d3, hipMemcpyHostToDevice, s));
+    VPBS_HIP(c, hipMemcpyAsync(c->fil_ts, knots, sizeof(double) * (size_t)Np, hipMemcpyHostToDevice, s));
+    VPBS_HIP(c, hipMemcpyAsync(c->fil_off, off.data(), sizeof(int64_t) * (size_t)Nf, hipMemcpyHostToDevice, s));
+    VPBS_HIP(c, hipMemcpyAsync(c->fil_n, nn.data(), sizeof(int32_t) * (size_t)Nf, hipMemcpyHostToDevice, s));
+    VPBS_HIP(c, hipMemcpyAsync(c->fil_T, periods, sizeof(double) * (size_t)Nf, hipMemcpyHostToDevice, s));
+    VPBS_HIP(c, hipMemcpyAsync(c->fil_xoff, offsets, sizeof(double) * 3 * (size_t)Nf, hipMemcpyHostToDevice, s));
+
+    Fil f;
+    f.fid = c->fil_fid;
+    f.off = c->fil_off;
+    f.N = c->fil_n;
+    f.T = c->fil_T;
+    f.Xoff = c->fil_xoff;
+    f.ts = c->fil_ts;
+    f.Np = Np;
+    f.Nf = Nf;
+    Quad qd{};
+    qd.n = nq;
+    qd.straight = straight;
+    qd.gamma = std::isnan(lia_segment_fraction) ? 1.0 : lia_segment_fraction;
+    for (int q = 0; q < nq && !straight; q++) {
+        qd.z[q] = quad_nodes[q];
+        qd.w[q] = quad_weights[q];
+    }
+    if (std::isnan(lia_segment_fraction)) {
+        qd.lim_m[0] = 0.0;
+        qd.lim_m[1] = 1.0;
+        qd.lim_p[0] = 0.0;
+        qd.lim_p[1] = 1.0;
+    } else {  // lia_integration_limits (local_integrals.jl:100-101)
+        const double g = lia_segment_fraction;
+        qd.lim_m[0] = 1.0 - g;
+        qd.lim_m[1] = 1.0;
+        qd.lim_p[0] = 0.0;
+        qd.lim_p[1] = g;
+    }
+    stage_begin(c, ST_POINTS, s);
+    const unsigned nb = nblocks(Np, 256);
+    node_filament<<<nb, 256, 0, s>>>(c->fil_off, Nf, Np, c->fil_fid);
+    derivative_coefs<<<nb, 256, 0, s>>>(f, c->fil_c, c->fil_d);
+    second_derivative_coefs<<<nb, 256, 0, s>>>(f, c->fil_d, c->fil_e);
+    point_data_kernel<<<nblocks(Np, 128), 128, 0, s>>>(f, qd, c->box, c->fil_x, c->fil_c, c->fil_d, c->fil_e,
+                                                       c->nodes, c->idx_prev, c->sp, c->spp, c->dlm, c->dlp,
+                                                       c->points, c->charges);
+    VPBS_HIP(c, hipGetLastError());
+    stage_end(c, ST_POINTS, s);
+    VPBS_HIP(c, hipStreamSynchronize(s));
+    c->Np = Np;
+    c->Nq = Nq;
+    VPBS_TRY(device_check(c));
+    float ms = 0.f;
+    VPBS_HIP(c, hipEventElapsedTime(&ms, c->ev_beg[ST_POINTS], c->ev_end[ST_POINTS]));
+    c->timings.point_data = ms;
+    c->have_points = true;
+    c->have_stats = false;
+    c->last_vel = c->last_psi = false;
+    return VPBS_OK;
+}
+
+}  // namespace vpbs

@@ -109,9 +109,10 @@ static void release(vpbs_ctx *c) {
                     c->points,    c->charges,   c->v_sr,     c->psi_sr,    c->v_lr,       c->psi_lr,
                     c->v_out,     c->psi_out,   c->src_key,  c->src_start, c->src_perm,   c->tgt_key,
                     c->tgt_start, c->tgt_perm,  c->src_pos,  c->src_chg,   c->src_idx,    c->stat_counts,
-                    c->stat_hashes, c->lr_key,  c->lr_start, c->lr_perm,   c->lr_pos,     c->lr_chg,
+                    c->stat_hashes, c->lr_key,  c->lr_start, c->lr_active, c->lr_perm,   c->lr_i0,      c->lr_w,    c->lr_chg,
                     c->sort_sr.rank, c->sort_sr.scan_tmp, c->sort_tg.rank, c->sort_tg.scan_tmp,
-                    c->sort_lr.rank, c->sort_lr.scan_tmp};
+                    c->sort_lr.rank, c->sort_lr.scan_tmp, c->fil_x, c->fil_ts, c->fil_c, c->fil_d, c->fil_e,
+                    c->fil_fid, c->fil_n, c->fil_off, c->fil_T, c->fil_xoff};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int st = 0; st < ST_COUNT; st++) {
@@ -134,7 +135,7 @@ static int alloc(vpbs_ctx *c, T **p, int64_t n) {
     return VPBS_OK;
 }
 
-static int ensure_capacity(vpbs_ctx *c, int64_t Np, int64_t Nq) {
+int ensure_capacity(vpbs_ctx *c, int64_t Np, int64_t Nq) {
     const size_t cs = c->p.precision == VPBS_F32 ? sizeof(float) : sizeof(double);
     if (Np > c->cap_np) {
         VPBS_TRY(alloc(c, &c->nodes, 3 * Np));
@@ -164,12 +165,14 @@ static int ensure_capacity(vpbs_ctx *c, int64_t Np, int64_t Nq) {
         VPBS_TRY(alloc(c, &c->src_idx, Nq));
         VPBS_TRY(alloc(c, &c->lr_key, Nq));
         VPBS_TRY(alloc(c, &c->lr_perm, Nq));
-        VPBS_TRY(alloc(c, &c->lr_pos, 3 * Nq));
+        VPBS_TRY(alloc(c, &c->lr_i0, 3 * Nq));
         if (c->src_chg) VPBS_HIP(c, hipFree(c->src_chg));
         if (c->lr_chg) VPBS_HIP(c, hipFree(c->lr_chg));
-        c->src_chg = c->lr_chg = nullptr;
+        if (c->lr_w) VPBS_HIP(c, hipFree(c->lr_w));
+        c->src_chg = c->lr_chg = c->lr_w = nullptr;
         VPBS_HIP(c, hipMalloc(&c->src_chg, cs * 3 * (size_t)Nq));
         VPBS_HIP(c, hipMalloc(&c->lr_chg, cs * 3 * (size_t)Nq));
+        VPBS_HIP(c, hipMalloc(&c->lr_w, cs * 3 * 2 * (size_t)c->p.nufft_m * (size_t)Nq));
         c->cap_nq = Nq;
     }
     return VPBS_OK;
@@ -206,7 +209,7 @@ static int host_check(vpbs_ctx *c, int64_t n, const double *pos, const int32_t *
     return VPBS_OK;
 }
 
-static int device_check(vpbs_ctx *c) {
+int device_check(vpbs_ctx *c) {
     unsigned int *bad = nullptr;
     VPBS_HIP(c, hipMalloc((void **)&bad, 2 * sizeof(unsigned int)));
     VPBS_HIP(c, hipMemset(bad, 0, 2 * sizeof(unsigned int)));
@@ -249,6 +252,7 @@ static int set_points_impl(vpbs_ctx *c, int64_t Np, const double *nodes, const i
     VPBS_HIP(c, hipMemcpy(c->charges, charges, sizeof(double) * 3 * (size_t)Nq, kind));
     c->Np = Np;
     c->Nq = Nq;
+    c->timings.point_data = 0.0;
     if (kind == hipMemcpyDeviceToDevice) VPBS_TRY(device_check(c));
     c->have_points = true;
     c->have_stats = false;
@@ -319,7 +323,8 @@ int vpbs_create(vpbs_ctx **out, const vpbs_params *p) {
     }
     rc = longrange_setup(c);
     if (rc != VPBS_OK) return bail(rc);
-    if (hipMalloc((void **)&c->lr_start, sizeof(int32_t) * (size_t)(c->ntiles + 1)) != hipSuccess) {
+    if (hipMalloc((void **)&c->lr_start, sizeof(int32_t) * (size_t)(c->ntiles + 1)) != hipSuccess ||
+        hipMalloc((void **)&c->lr_active, (size_t)c->ntiles) != hipSuccess) {
         fail(c, VPBS_ERR_HIP, "tile allocation failed");
         return bail(VPBS_ERR_HIP);
     }
@@ -346,6 +351,33 @@ int vpbs_set_points_device(vpbs_ctx *c, int64_t Np, const double *nodes, const i
                            hipMemcpyDeviceToDevice);
 }
 
+int vpbs_set_filaments(vpbs_ctx *c, int64_t Nf, const int64_t *counts, const double *X, const double *knots,
+                       const double *periods, const double *offsets, const double *coefs, const double *quad_nodes,
+                       const double *quad_weights, double lia_segment_fraction) {
+    if (!c) return VPBS_ERR_INVALID;
+    return build_point_data(c, Nf, counts, X, knots, periods, offsets, coefs, quad_nodes, quad_weights,
+                            lia_segment_fraction);
+}
+
+int vpbs_get_points(vpbs_ctx *c, double *nodes, int32_t *idx_prev, double *sp, double *spp, double *dlm,
+                    double *dlp, double *points, double *charges) {
+    if (!c) return VPBS_ERR_INVALID;
+    if (!c->have_points) return fail(c, VPBS_ERR_STATE, "no point data: call vpbs_set_points or vpbs_set_filaments");
+    VPBS_HIP(c, hipSetDevice(c->device));
+    const size_t d3 = sizeof(double) * 3 * (size_t)c->Np, d1 = sizeof(double) * (size_t)c->Np;
+    const size_t q3 = sizeof(double) * 3 * (size_t)c->Nq;
+    const auto D2H = hipMemcpyDeviceToHost;
+    if (nodes) VPBS_HIP(c, hipMemcpy(nodes, c->nodes, d3, D2H));
+    if (idx_prev) VPBS_HIP(c, hipMemcpy(idx_prev, c->idx_prev, sizeof(int32_t) * (size_t)c->Np, D2H));
+    if (sp) VPBS_HIP(c, hipMemcpy(sp, c->sp, d3, D2H));
+    if (spp) VPBS_HIP(c, hipMemcpy(spp, c->spp, d3, D2H));
+    if (dlm) VPBS_HIP(c, hipMemcpy(dlm, c->dlm, d1, D2H));
+    if (dlp) VPBS_HIP(c, hipMemcpy(dlp, c->dlp, d1, D2H));
+    if (points) VPBS_HIP(c, hipMemcpy(points, c->points, q3, D2H));
+    if (charges) VPBS_HIP(c, hipMemcpy(charges, c->charges, q3, D2H));
+    return VPBS_OK;
+}
+
 int vpbs_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff, double *v_out, double *psi_out) {
     if (!c) return VPBS_ERR_INVALID;
     if (!c->have_points) return fail(c, VPBS_ERR_STATE, "vpbs_compute called before vpbs_set_points");
@@ -357,7 +389,7 @@ int vpbs_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff, double 
         return fail(c, VPBS_ERR_INVALID, "integration_cutoff must be positive (or NaN for the default)");
     VPBS_HIP(c, hipSetDevice(c->device));
     auto t0 = std::chrono::steady_clock::now();
-    for (int st = 0; st < ST_COUNT; st++) c->stage_ran[st] = false;
+    for (int st = 0; st <= ST_MERGE; st++) c->stage_ran[st] = false;
     const bool with_sr = !(flags & VPBS_NO_SHORTRANGE);
     const bool with_lr = !(flags & VPBS_NO_LONGRANGE) && !(flags & VPBS_LIA_ONLY);
     const int64_t n3 = 3 * c->Np;
@@ -395,9 +427,9 @@ int vpbs_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff, double 
 
     // timings
     vpbs_timings &T = c->timings;
-    double *slots[ST_COUNT] = {&T.local_term, &T.cell_build, &T.pair,  &T.self,   &T.lr_sort, &T.spread,
-                               &T.fft_forward, &T.fourier,   &T.fft_backward, &T.interp, &T.merge};
-    for (int st = 0; st < ST_COUNT; st++) {
+    double *slots[ST_MERGE + 1] = {&T.local_term, &T.cell_build, &T.pair,  &T.self,   &T.lr_sort, &T.spread,
+                                   &T.fft_forward, &T.fourier,   &T.fft_backward, &T.interp, &T.merge};
+    for (int st = 0; st <= ST_MERGE; st++) {
         float ms = 0.f;
         *slots[st] = 0.0;
         if (c->stage_ran[st] && hipEventElapsedTime(&ms, c->ev_beg[st], c->ev_end[st]) == hipSuccess)

@@ -39,6 +39,7 @@ enum Stage {
     ST_FFTB,
     ST_INTERP,
     ST_MERGE,
+    ST_POINTS,  // vpbs_set_filaments (not part of vpbs_compute)
     ST_COUNT
 };
 
@@ -62,6 +63,13 @@ struct vpbs_ctx {
     int32_t *idx_prev = nullptr;
     double *points = nullptr, *charges = nullptr;
 
+    // filament splines (device, vpbs_set_filaments)
+    int64_t cap_fil_np = 0, cap_fil_nf = 0;
+    double *fil_x = nullptr, *fil_ts = nullptr, *fil_c = nullptr, *fil_d = nullptr, *fil_e = nullptr;
+    int32_t *fil_fid = nullptr, *fil_n = nullptr;
+    int64_t *fil_off = nullptr;
+    double *fil_T = nullptr, *fil_xoff = nullptr;
+
     // outputs (device, FP64, 3*Np each)
     double *v_sr = nullptr, *psi_sr = nullptr, *v_lr = nullptr, *psi_lr = nullptr, *v_out = nullptr,
            *psi_out = nullptr;
@@ -82,14 +90,18 @@ struct vpbs_ctx {
 
     // long range
     int32_t G[3] = {0, 0, 0};   // oversampled grid
-    int64_t Gxp = 0;            // padded x length of the in-place r2c layout (2*(G0/2+1))
-    int64_t comp_stride = 0;    // elements (of the grid type) per component
+    int64_t Gxp = 0;            // row pitch of the real grid (= G0: rows are 128-B aligned)
+    int64_t comp_stride = 0;    // real elements per component (G0·G1·G2)
+    int64_t spec_stride = 0;    // complex elements per component of the half spectrum ((G0/2+1)·G1·G2)
     int32_t ntile[3] = {0, 0, 0};
     int64_t ntiles = 0;
-    void *grid = nullptr;       // 3 components, velocity (or vorticity) spectrum in place
-    void *grid2 = nullptr;      // 3 components, streamfunction
+    void *grid = nullptr;       // real grid, 3 components (spreading output, c2r output)
+    void *spec = nullptr;       // half spectrum, 3 components (vorticity, then velocity, in place)
+    void *spec2 = nullptr;      // half spectrum of the streamfunction (allocated on demand)
     int32_t *lr_key = nullptr, *lr_start = nullptr, *lr_perm = nullptr;
-    double *lr_pos = nullptr;   // tile-sorted quadrature points, 3*Nq
+    uint8_t *lr_active = nullptr;  // per tile: 1 if any neighbour tile holds a point
+    int32_t *lr_i0 = nullptr;   // tile-sorted points: first stencil index per direction, 3*Nq
+    void *lr_w = nullptr;       // tile-sorted points: 2m kernel weights per direction, 3*2m*Nq (compute type)
     void *lr_chg = nullptr;     // tile-sorted charges, 3*Nq (compute type)
     double *deconv = nullptr;   // 1/ψ̂ tables for retained kx, ky, kz (concatenated)
     int32_t kmax_mode[3] = {0, 0, 0};
@@ -146,6 +158,15 @@ int longrange_compute(vpbs_ctx *c, uint32_t flags);
 void longrange_release(vpbs_ctx *c);
 int oversampled_size(int N, double sigma);
 double es_beta(int m, double sigma);
+
+// pointdata.hip
+int build_point_data(vpbs_ctx *c, int64_t Nf, const int64_t *counts, const double *X, const double *knots,
+                     const double *periods, const double *offsets, const double *coefs, const double *quad_nodes,
+                     const double *quad_weights, double lia_segment_fraction);
+
+// vpbs_api.hip
+int ensure_capacity(vpbs_ctx *c, int64_t Np, int64_t Nq);
+int device_check(vpbs_ctx *c);
 
 // helpers
 void stage_begin(vpbs_ctx *c, Stage st, hipStream_t s);

@@ -17,15 +17,18 @@ import numpy as np
 from .filaments import fold_coordinates_periodic
 
 
-def add_local_integrals(v, psi, pd, params) -> None:
-    """Add the non-LIA part of the local segments to v and/or psi ((3, Np) arrays, in place)."""
+def add_local_integrals(v, psi, batch, params) -> None:
+    """Add the non-LIA part of the local segments to v and/or psi ((3, Np) arrays, in place).
+
+    ``batch`` is the FilamentBatch the point data was built from.
+    """
     g = params.lia_segment_fraction
     if g is None:
         return
-    b = pd.batch
+    b = batch
     zs, ws = params.quadrature_near_singularity.rule()
     Ls = np.asarray(params.Ls, dtype=np.float64)[:, None]
-    x = pd.nodes  # folded nodes (3, Np)
+    x = np.stack([fold_coordinates_periodic(b.X[:, d], Ls[d, 0]) for d in range(3)])  # folded nodes (3, Np)
     N = b.Ns[b.fid]
     pref = params.Γ / (4 * np.pi)
     acc_v = np.zeros_like(x)
