@@ -78,18 +78,21 @@ def stage_bytes(stage: str, s: int, G, Ns, Np: int, Nq: int) -> float:
     """Algorithmic HBM bytes per launch of each long-range stage (velocity only).
 
     G: oversampled grid, Ns: Fourier grid, s: bytes per real of the compute type.
-    spread 3Gs + 6Nq·s ; FFT forward 3Gs + 6Hs ; Fourier kernel 6Ks + 6Hs ;
-    FFT backward 6Hs + 3Gs ; interpolation 3Gs + 6Np·s, with G = ∏G_d,
-    H = (G_0/2+1)G_1G_2 and K = (N_0/2+1)N_1N_2.
+    spread 3Gs + 6Nq·s (grid written once, points read once); FFT forward
+    3Gs + 6Zs (real grid in, z-pencils of the retained (kx, ky) out); Fourier
+    kernel 12Zs (z-pencils in and out, 3 complex components); FFT backward
+    6Zs + 3Gs; interpolation 3Gs + 6Np·s, with G = ∏G_d and Z = nkx·nky·G_2 the
+    complex values per component of the z-pencils (nkx = N_0/2, nky = 2⌊(N_1-1)/2⌋+1).
     """
     Gc = float(np.prod(G))
-    H = (G[0] // 2 + 1) * float(G[1]) * G[2]
-    K = (Ns[0] // 2 + 1) * float(Ns[1]) * Ns[2]
+    nkx = Ns[0] // 2
+    nky = 2 * ((Ns[1] - 1) // 2) + 1
+    Z = float(nkx) * nky * G[2]
     return {
         "spread": 3 * Gc * s + 6 * Nq * s,
-        "fft_forward": 3 * Gc * s + 6 * H * s,
-        "fourier": 6 * K * s + 6 * H * s,
-        "fft_backward": 6 * H * s + 3 * Gc * s,
+        "fft_forward": 3 * Gc * s + 6 * Z * s,
+        "fourier": 12 * Z * s,
+        "fft_backward": 6 * Z * s + 3 * Gc * s,
         "interp": 3 * Gc * s + 6 * Np * s,
     }[stage]
 

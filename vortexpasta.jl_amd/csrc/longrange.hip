@@ -13,13 +13,24 @@
 //
 // Kernels:
 //   lr_tile_keys    tile of each quadrature point on the σ-oversampled grid
-//   spread_kernel   one workgroup per 16×8×8 grid tile GATHERS the kernel-weighted
-//                   charges of the points in the 3×3×3 neighbouring tiles: weights
-//                   staged in LDS, every grid value written exactly once, no atomics,
-//                   no zero-fill pass, deterministic summation order
+//   spread_scatter_kernel  one workgroup per 32×8×8 grid tile takes the points of the
+//                   3×3×3 neighbouring tiles whose support overlaps it and adds their
+//                   kernel-weighted charges into an LDS copy of the tile (each
+//                   wavefront owns a z-slab, points in sorted order): every grid value
+//                   written exactly once, no global atomics, no zero-fill pass,
+//                   deterministic summation order
+//   FFT             batched contiguous 1D c2c transforms on rocFFT along x, y and
+//                   z, with pencil transposes of our own between them:
+//                   x: the real grid holds pairs of y-rows as one complex row, so
+//                      one c2c transforms two real rows;
+//                   pair_split_to_ypencils: separates the pair, A = (Z_k + conj Z_-k)/2,
+//                      B = (Z_k - conj Z_-k)/2i, keeps the retained kx and makes
+//                      y contiguous; ypencils_to_zpencils keeps the retained ky and
+//                      makes z contiguous. The inverse mirrors this and rebuilds
+//                      the Hermitian half kx < 0 and the zero modes.
 //   fourier_kernel  deconvolution (twice: type 1 and type 2), Γ/V, Green function,
-//                   curl, Ewald Gaussian, Hermitian clean-up, zeroing of dropped
-//                   modes — one pass over the half spectrum, in place
+//                   curl, Ewald Gaussian, zeroing of dropped modes — one pass over
+//                   the z-pencils of retained (kx, ky), in place
 //   interp_kernel_wave  W³ ES-weighted gather, one wavefront per node
 // Everything is templated on the grid type (double for the FP64 path, float
 // for the FP32-bulk path).
@@ -61,8 +72,18 @@ struct LRGeom {
     double scale[3];  // G/L
     int ntile[3];
     int M;
-    int64_t Gxp;
-    int64_t cstride;
+    int64_t cstride;  // real grid values per component
+};
+
+// Position of real grid value (x, y, z) of one component: rows y = 2p and
+// y = 2p + 1 are interleaved as the real and imaginary parts of complex row p.
+__device__ inline int64_t gidx(const LRGeom &g, int x, int y, int z) {
+    return (((int64_t)z * (g.G[1] >> 1) + (y >> 1)) * g.G[0] + x) * 2 + (y & 1);
+}
+
+template <typename CT>
+struct alignas(2 * sizeof(CT)) Cx {
+    CT re, im;
 };
 
 // ----------------------------------------------------------------------------
@@ -140,65 +161,52 @@ struct alignas(16) Q4 {
     CT x, y, z, w;
 };
 
-// One workgroup (256 threads) per 32×8×8 tile. Wave w owns the x-octet
-// x ∈ [8w, 8w+8) of the tile and lane l the row (y, z) = (l % 8, l / 8); each
-// thread accumulates its 8 cells × 3 components in registers.
-// For each batch of up to SPB candidate points from the 3×3×3 neighbour tiles:
-//   A) keep the points whose kernel support overlaps the tile (deterministic
-//      ballot compaction, candidate order preserved),
-//   B) expand their precomputed weights to per-tile rows in LDS (zero outside
-//      the support) and record which x-octets the support touches,
-//   C) accumulate Σ_p q_p·wx·wy·wz; a wave skips the points whose support
-//      misses its octet (a wave-uniform branch).
-// The tile is then transposed through LDS and written once, in full rows.
-constexpr int SPREAD_TPB = 256;
-constexpr int SPB = 128;
-
-template <typename CT>
-struct SpreadSmem {
-    CT wx[SPB][TX];
-    CT wy[SPB][TY];
-    CT wz[SPB][TZ];
-};
-
-template <typename CT>
-__global__ __launch_bounds__(SPREAD_TPB) void spread_kernel(LRGeom g, int64_t Nq, const int32_t *__restrict__ tstart,
-                                                            const uint8_t *__restrict__ active,
-                                                            const int32_t *__restrict__ si0,
-                                                            const CT *__restrict__ sw,
-                                                            const CT *__restrict__ schg, CT *__restrict__ grid) {
-    static_assert(sizeof(SpreadSmem<CT>) >= 3 * TX * TY * TZ * sizeof(CT), "epilogue tile must fit");
+// Point-stationary spreading: one workgroup (4 wavefronts) per 32×8×8 tile,
+// wavefront w owning the z-slab [2w, 2w+2) of the tile in LDS.
+// Per round of up to NB candidates from the 3×3×3 neighbour tiles:
+//   A) keep the points whose support overlaps the tile (ballot compaction,
+//      candidate order preserved), recording the support start relative to
+//      the tile start, wrap1(i0 - t0, G), per direction;
+//   B) stage the kept points' 3W weights and charges in LDS;
+//   C) each wavefront walks the kept points in order and adds q·wx·wy·wz into
+//      the cells of its slab inside the support: lane (a, b) takes support
+//      offsets (8i + a, 8j + b) in x and y. A wavefront never shares a cell
+//      with another and adds the points in a fixed order, so the sums are
+//      deterministic; within one point the lanes hit distinct cells.
+// The tile (pair rows interleaved, as in the grid) is then written once.
+template <typename CT, int W>
+__global__ __launch_bounds__(256) void spread_scatter_kernel(LRGeom g, int64_t Nq, const int32_t *__restrict__ tstart,
+                                                             const uint8_t *__restrict__ active,
+                                                             const int32_t *__restrict__ si0,
+                                                             const CT *__restrict__ sw,
+                                                             const CT *__restrict__ schg, CT *__restrict__ grid) {
+    constexpr int NB = sizeof(CT) == 4 ? 128 : 64;  // kept points per round
+    constexpr int NPAIR = TY / 2;
+    constexpr int PITCH = 2 * TX + (sizeof(CT) == 4 ? 16 : 8);  // LDS pitch of a pair row (values)
+    constexpr int NPASS = (W + 7) / 8;                           // 8-wide passes per direction
+    __shared__ CT acc[3][TZ][NPAIR][PITCH];
+    __shared__ CT s_w[NB][3][W];
+    __shared__ CT s_q[NB][3];
+    __shared__ int4 s_r[NB];  // (rx, ry, rz, sorted point index)
     __shared__ int nb_start[27];
     __shared__ int nb_len[27];
     __shared__ int nb_pre[28];
-    __shared__ SpreadSmem<CT> sm;
-    __shared__ Q4<CT> s_q[SPB];
-    __shared__ int s_pt[SPB];  // staged point index (sorted order)
-    __shared__ int s_mask[SPB];  // x-octets of the tile touched by the point's support
-    __shared__ int s_wave_cnt[SPB / 64];
+    __shared__ int s_wave_cnt[4];
 
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
     const int64_t ntl = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
     const int64_t tile = xcd_tile(blockIdx.x, ntl);
-    if (tile >= ntl) return;
+    if (tile >= ntl) return;  // uniform over the workgroup
     const int tix = (int)(tile % g.ntile[0]);
     const int tiy = (int)((tile / g.ntile[0]) % g.ntile[1]);
     const int tiz = (int)(tile / ((int64_t)g.ntile[0] * g.ntile[1]));
     const int t0[3] = {tix * TX, tiy * TY, tiz * TZ};
-    const int lane = tid & 63, wid = tid >> 6;
-    const int ly = lane & 7, lz = lane >> 3;  // this thread's row
 
-    CT acc[8][3];
-#pragma unroll
-    for (int a = 0; a < 8; a++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) acc[a][d] = (CT)0;
+    CT *accf = &acc[0][0][0][0];
+    for (int e = tid; e < 3 * TZ * NPAIR * PITCH; e += 256) accf[e] = (CT)0;
 
     if (active[tile]) {
-        const int W = 2 * g.M;
-        // Neighbour tiles t + (-1..1)³: lane k < 27 loads the point range of one
-        // of them; a direction with fewer than 3 tiles would visit a tile twice,
-        // so the repeated offsets get an empty range.
         if (tid < 27) {
             const int o[3] = {tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1};
             const int tt[3] = {tix, tiy, tiz};
@@ -227,113 +235,271 @@ __global__ __launch_bounds__(SPREAD_TPB) void spread_kernel(LRGeom g, int64_t Nq
         }
         __syncthreads();
         const int ncand = nb_pre[27];
+        const int a8 = lane & 7, b8 = lane >> 3;
 
-        for (int base = 0; base < ncand; base += SPB) {
-            // A) candidate -> overlap test (support [i0, i0+W) ∩ tile, periodic) -> compaction
+        for (int base = 0; base < ncand; base += NB) {
+            // A) overlap test and compaction
             bool keep = false;
-            int sidx = 0, mask = 0;
-            if (tid < SPB && base + tid < ncand) {
-                const int c = base + tid;
-                int lo = 0, hi = 26;  // largest nbi with nb_pre[nbi] <= c
+            int4 rr = {0, 0, 0, 0};
+            if (tid < NB && base + tid < ncand) {
+                const int cnd = base + tid;
+                int lo = 0, hi = 26;  // largest nbi with nb_pre[nbi] <= cnd
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
-                    if (nb_pre[mid] <= c)
+                    if (nb_pre[mid] <= cnd)
                         lo = mid;
                     else
                         hi = mid - 1;
                 }
-                sidx = nb_start[lo] + (c - nb_pre[lo]);
+                const int sidx = nb_start[lo] + (cnd - nb_pre[lo]);
+                int r[3];
                 keep = true;
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
                     const int i0 = si0[d * Nq + sidx];
                     const int T = d == 0 ? TX : (d == 1 ? TY : TZ);
-                    const int dd = wrap1(t0[d] - i0, g.G[d]);  // tile start relative to support start
-                    const int ee = wrap1(i0 - t0[d], g.G[d]);  // support start relative to tile start
-                    keep = keep && (dd < W || ee < T);
-                    if (d == 0) {
-#pragma unroll
-                        for (int o = 0; o < TX / 8; o++) {
-                            const int d8 = wrap1(t0[0] + 8 * o - i0, g.G[0]);
-                            const int e8 = wrap1(i0 - t0[0] - 8 * o + g.G[0], g.G[0]);
-                            mask |= (d8 < W || e8 < 8) ? (1 << o) : 0;
-                        }
-                    }
+                    r[d] = wrap1(i0 - t0[d], g.G[d]);                       // support start rel. to tile
+                    keep = keep && (r[d] < T || wrap1(t0[d] - i0, g.G[d]) < W);  // overlap (periodic)
                 }
+                rr = make_int4(r[0], r[1], r[2], sidx);
             }
             const unsigned long long ball = __ballot(keep);
-            if (tid < SPB && lane == 0) s_wave_cnt[wid] = __popcll(ball);
+            if (lane == 0) s_wave_cnt[wid] = __popcll(ball);
             __syncthreads();
             if (keep) {
                 int pos = __popcll(ball & ((1ull << lane) - 1ull));
                 for (int w = 0; w < wid; w++) pos += s_wave_cnt[w];
-                s_pt[pos] = sidx;
-                s_mask[pos] = mask;
-                s_q[pos] = Q4<CT>{schg[sidx], schg[Nq + sidx], schg[2 * Nq + sidx], (CT)0};
+                s_r[pos] = rr;
             }
             __syncthreads();
-            int ns = 0;
-#pragma unroll
-            for (int w = 0; w < SPB / 64; w++) ns += s_wave_cnt[w];
-            // B) expand the weights of the staged points over the tile rows
-            for (int e = tid; e < ns * (TX + TY + TZ); e += SPREAD_TPB) {
-                const int p = e / (TX + TY + TZ);
-                const int r = e - p * (TX + TY + TZ);
-                const int d = r < TX ? 0 : (r < TX + TY ? 1 : 2);
-                const int cl = d == 0 ? r : (d == 1 ? r - TX : r - TX - TY);
-                const int sp = s_pt[p];
-                const int k = wrap1(t0[d] + cl - si0[d * Nq + sp], g.G[d]);
-                const CT w = k < W ? sw[((int64_t)d * W + k) * Nq + sp] : (CT)0;
-                if (d == 0)
-                    sm.wx[p][cl] = w;
-                else if (d == 1)
-                    sm.wy[p][cl] = w;
-                else
-                    sm.wz[p][cl] = w;
+            const int ns = s_wave_cnt[0] + s_wave_cnt[1] + s_wave_cnt[2] + s_wave_cnt[3];
+            // B) stage weights (component-major source: row d·W + k, point p fastest) and charges
+            for (int e = tid; e < ns * 3 * W; e += 256) {
+                const int p = e % ns, r = e / ns;
+                s_w[p][r / W][r % W] = sw[(int64_t)r * Nq + s_r[p].w];
+            }
+            for (int e = tid; e < ns * 3; e += 256) {
+                const int p = e % ns, d = e / ns;
+                s_q[p][d] = schg[(int64_t)d * Nq + s_r[p].w];
             }
             __syncthreads();
-            // C) accumulate
-            const int bit = 1 << wid;
+            // C) scatter into this wavefront's slab
+            const int zs = 2 * wid;
             for (int p = 0; p < ns; p++) {
-                if (!(s_mask[p] & bit)) continue;  // wave-uniform
-                const CT wyz = sm.wy[p][ly] * sm.wz[p][lz];
-                const Q4<CT> q = s_q[p];
-                const CT a0 = wyz * q.x, a1 = wyz * q.y, a2 = wyz * q.z;
+                const int4 r = s_r[p];
+                const int kz0 = wrap1(zs - r.z, g.G[2]), kz1 = wrap1(zs + 1 - r.z, g.G[2]);
+                if (kz0 >= W && kz1 >= W) continue;  // uniform: support misses the slab
+                const CT q0 = s_q[p][0], q1 = s_q[p][1], q2 = s_q[p][2];
+                const CT wz0 = kz0 < W ? s_w[p][2][kz0] : (CT)0;
+                const CT wz1 = kz1 < W ? s_w[p][2][kz1] : (CT)0;
 #pragma unroll
-                for (int a = 0; a < 8; a++) {
-                    const CT wx = sm.wx[p][8 * wid + a];
-                    acc[a][0] += wx * a0;
-                    acc[a][1] += wx * a1;
-                    acc[a][2] += wx * a2;
+                for (int pi = 0; pi < NPASS; pi++) {
+#pragma unroll
+                    for (int pj = 0; pj < NPASS; pj++) {
+                        const int kx = 8 * pi + a8, ky = 8 * pj + b8;
+                        const int lx = wrap1(r.x + kx, g.G[0]), ly = wrap1(r.y + ky, g.G[1]);
+                        if (kx < W && ky < W && lx < TX && ly < TY) {
+                            const CT wxy = s_w[p][0][kx] * s_w[p][1][ky];
+                            // plain read-modify-write: the slab is this wavefront's alone and
+                            // its LDS accesses execute in program order, so the next point's
+                            // reads see these writes (LDS float atomics are far slower)
+                            CT *cell = &acc[0][zs][ly >> 1][2 * lx + (ly & 1)];
+                            constexpr int CS = TZ * NPAIR * PITCH, ZS = NPAIR * PITCH;
+                            if (kz0 < W && kz1 < W) {
+                                const CT f0 = wxy * wz0, f1 = wxy * wz1;
+                                CT a0 = cell[0], a1 = cell[CS], a2 = cell[2 * CS];
+                                CT b0 = cell[ZS], b1 = cell[CS + ZS], b2 = cell[2 * CS + ZS];
+                                a0 += f0 * q0;
+                                a1 += f0 * q1;
+                                a2 += f0 * q2;
+                                b0 += f1 * q0;
+                                b1 += f1 * q1;
+                                b2 += f1 * q2;
+                                cell[0] = a0;
+                                cell[CS] = a1;
+                                cell[2 * CS] = a2;
+                                cell[ZS] = b0;
+                                cell[CS + ZS] = b1;
+                                cell[2 * CS + ZS] = b2;
+                            } else {
+                                const int zo = kz0 < W ? 0 : ZS;
+                                const CT f = wxy * (kz0 < W ? wz0 : wz1);
+                                CT a0 = cell[zo], a1 = cell[CS + zo], a2 = cell[2 * CS + zo];
+                                a0 += f * q0;
+                                a1 += f * q1;
+                                a2 += f * q2;
+                                cell[zo] = a0;
+                                cell[CS + zo] = a1;
+                                cell[2 * CS + zo] = a2;
+                            }
+                        }
+                    }
                 }
             }
             __syncthreads();
         }
     }
-    // Transpose through LDS and store whole rows: tile[d][z][y][x].
-    CT *tilebuf = reinterpret_cast<CT *>(&sm);
-#pragma unroll
-    for (int a = 0; a < 8; a++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) tilebuf[((d * TZ + lz) * TY + ly) * TX + 8 * wid + a] = acc[a][d];
     __syncthreads();
-    const int x = tid & 31;
-    for (int r = tid >> 5; r < 3 * TY * TZ; r += SPREAD_TPB / 32) {
-        const int d = r / (TY * TZ), yz = r - d * TY * TZ;
-        const int z = yz / TY, y = yz - z * TY;
-        const int64_t off = ((int64_t)(t0[2] + z) * g.G[1] + (t0[1] + y)) * g.Gxp + t0[0] + x;
-        grid[d * g.cstride + off] = tilebuf[r * TX + x];
+    // write the tile: 3·TZ·NPAIR pair rows of 2·TX contiguous values
+    const int v = tid & 63;
+    for (int row = tid >> 6; row < 3 * TZ * NPAIR; row += 4) {
+        const int d = row / (TZ * NPAIR), zp = row - d * TZ * NPAIR;
+        const int z = zp / NPAIR, pp = zp - z * NPAIR;
+        const int64_t off = (((int64_t)(t0[2] + z) * (g.G[1] >> 1) + (t0[1] >> 1) + pp) * g.G[0] + t0[0]) * 2 + v;
+        grid[d * g.cstride + off] = acc[d][z][pp][v];
+    }
+}
+
+template <typename CT>
+void launch_spread(const LRGeom &g, int64_t nb, int64_t Nq, const int32_t *tstart, const uint8_t *active,
+                   const int32_t *si0, const CT *sw, const CT *schg, CT *grid, hipStream_t s) {
+    const unsigned n = (unsigned)nb;
+    switch (g.M) {
+        case 2: spread_scatter_kernel<CT, 4><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
+        case 3: spread_scatter_kernel<CT, 6><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
+        case 4: spread_scatter_kernel<CT, 8><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
+        case 5: spread_scatter_kernel<CT, 10><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
+        case 6: spread_scatter_kernel<CT, 12><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
+        case 7: spread_scatter_kernel<CT, 14><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
+        default: spread_scatter_kernel<CT, 16><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
     }
 }
 
 // ----------------------------------------------------------------------------
-// Fused Fourier-space kernel (in place on the half spectrum).
+// Pencil transposes between the 1D transforms. All four move 32×32 (or
+// 32×16-pair) tiles through LDS so that both the loads and the stores run
+// along contiguous rows.
+
+// Forward, after the x-pass: Z = FFT_x(row 2p + i·row 2p+1) for every pair p.
+// Writes ypen[c][kx][z][y] (y contiguous) for the retained kx ∈ [0, nkx):
+// A_k = (Z_k + conj Z_{-k})/2 for y = 2p, B_k = (Z_k - conj Z_{-k})/(2i) for y = 2p+1.
+// Grid: (⌈nkx/32⌉, G1/32, 3·G2), 256 threads.
+template <typename CT>
+__global__ __launch_bounds__(256) void pair_split_to_ypencils(const Cx<CT> *__restrict__ zrow,
+                                                              Cx<CT> *__restrict__ ypen, int G0, int G1, int G2,
+                                                              int nkx) {
+    __shared__ Cx<CT> sk[16][33], sm[16][33];
+    const int kx0 = blockIdx.x * 32, p0 = blockIdx.y * 16;
+    const int cz = blockIdx.z, c = cz / G2, z = cz - c * G2;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const Cx<CT> *rows = zrow + (int64_t)cz * (G1 >> 1) * G0;
+    const int kx = kx0 + tx;
+    if (kx < nkx) {
+        const int km = kx == 0 ? 0 : G0 - kx;
+        for (int pp = ty; pp < 16; pp += 8) {
+            const Cx<CT> *r = rows + (int64_t)(p0 + pp) * G0;
+            sk[pp][tx] = r[kx];
+            sm[pp][tx] = r[km];
+        }
+    }
+    __syncthreads();
+    const int pp = tx >> 1, par = tx & 1;
+    for (int kk = ty; kk < 32 && kx0 + kk < nkx; kk += 8) {
+        const Cx<CT> a = sk[pp][kk], b = sm[pp][kk];
+        Cx<CT> o;
+        if (par == 0) {
+            o.re = (a.re + b.re) * (CT)0.5;
+            o.im = (a.im - b.im) * (CT)0.5;
+        } else {
+            o.re = (a.im + b.im) * (CT)0.5;
+            o.im = (b.re - a.re) * (CT)0.5;
+        }
+        ypen[(((int64_t)c * nkx + kx0 + kk) * G2 + z) * G1 + 2 * p0 + tx] = o;
+    }
+}
+
+// Forward, after the y-pass: ypen[plane][z][iy] -> zpen[plane][j][z] for the
+// retained ky = j (j ≤ K1) or j - nky (j > K1), plane = c·nkx + kx.
+// Grid: (G2/32, ⌈nky/32⌉, 3·nkx).
+template <typename CT>
+__global__ __launch_bounds__(256) void ypencils_to_zpencils(const Cx<CT> *__restrict__ ypen,
+                                                            Cx<CT> *__restrict__ zpen, int G1, int G2, int nky,
+                                                            int K1) {
+    __shared__ Cx<CT> s[32][33];
+    const int z0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    const int64_t plane = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const Cx<CT> *src = ypen + plane * G2 * G1;
+    Cx<CT> *dst = zpen + plane * nky * G2;
+    const int j = j0 + tx;
+    if (j < nky) {
+        const int iy = j <= K1 ? j : G1 + j - nky;
+        for (int zz = ty; zz < 32; zz += 8) s[zz][tx] = src[(int64_t)(z0 + zz) * G1 + iy];
+    }
+    __syncthreads();
+    for (int jj = ty; jj < 32 && j0 + jj < nky; jj += 8) dst[(int64_t)(j0 + jj) * G2 + z0 + tx] = s[tx][jj];
+}
+
+// Inverse, after the z-pass: zpen[plane][j][z] -> ypen[plane][z][iy] for every
+// iy, zero where ky is not retained. Grid: (G2/32, G1/32, 3·nkx).
+template <typename CT>
+__global__ __launch_bounds__(256) void zpencils_to_ypencils(const Cx<CT> *__restrict__ zpen,
+                                                            Cx<CT> *__restrict__ ypen, int G1, int G2, int nky,
+                                                            int K1) {
+    __shared__ Cx<CT> s[32][33];
+    const int z0 = blockIdx.x * 32, iy0 = blockIdx.y * 32;
+    const int64_t plane = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const Cx<CT> *src = zpen + plane * nky * G2;
+    Cx<CT> *dst = ypen + plane * G2 * G1;
+    for (int jj = ty; jj < 32; jj += 8) {
+        const int iy = iy0 + jj;
+        const int ky = iy <= (G1 >> 1) ? iy : iy - G1;
+        Cx<CT> v{(CT)0, (CT)0};
+        if (ky >= -K1 && ky <= K1) v = src[(int64_t)(ky >= 0 ? ky : ky + nky) * G2 + z0 + tx];
+        s[jj][tx] = v;
+    }
+    __syncthreads();
+    for (int zz = ty; zz < 32; zz += 8) dst[(int64_t)(z0 + zz) * G1 + iy0 + tx] = s[tx][zz];
+}
+
+// Inverse, after the y-pass: rebuild each pair row Z_k = A_k + i·B_k for every
+// k ∈ [0, G0): A, B from ypen for k < nkx, their conjugates at G0 - k for
+// k > G0 - nkx (Hermitian symmetry of real rows), zero otherwise; the DC term
+// keeps only its real part (what a c2r transform uses).
+// Grid: (G0/32, G1/32, 3·G2).
+template <typename CT>
+__global__ __launch_bounds__(256) void ypencils_to_pair_rows(const Cx<CT> *__restrict__ ypen,
+                                                             Cx<CT> *__restrict__ zrow, int G0, int G1, int G2,
+                                                             int nkx) {
+    __shared__ Cx<CT> s[32][33];
+    const int k0 = blockIdx.x * 32, p0 = blockIdx.y * 16;
+    const int cz = blockIdx.z, c = cz / G2, z = cz - c * G2;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int kk = ty; kk < 32; kk += 8) {
+        const int k = k0 + kk;
+        int kx = -1;
+        bool cj = false;
+        if (k < nkx) {
+            kx = k;
+        } else if (k > G0 - nkx) {
+            kx = G0 - k;
+            cj = true;
+        }
+        Cx<CT> v{(CT)0, (CT)0};
+        if (kx >= 0) {
+            v = ypen[(((int64_t)c * nkx + kx) * G2 + z) * G1 + 2 * p0 + tx];
+            if (cj) v.im = -v.im;
+            if (k == 0) v.im = (CT)0;
+        }
+        s[kk][tx] = v;
+    }
+    __syncthreads();
+    Cx<CT> *rows = zrow + (int64_t)cz * (G1 >> 1) * G0;
+    for (int pp = ty; pp < 16; pp += 8) {
+        const Cx<CT> A = s[tx][2 * pp], B = s[tx][2 * pp + 1];
+        rows[(int64_t)(p0 + pp) * G0 + k0 + tx] = Cx<CT>{A.re - B.im, A.im + B.re};
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Fused Fourier-space kernel, in place on the z-pencils [3][nkx][nky][G2].
 
 template <typename CT>
 struct FourierArgs {
     int G[3];
-    int64_t Hx;       // G0/2 + 1
-    int64_t cstride;  // complex elements per component
+    int nkx, nky;
+    int64_t cstride;  // complex values per component
     int kmax[3];      // retained: kx ∈ [0, kmax0], |ky| ≤ kmax1, |kz| ≤ kmax2
     const double *dx, *dy, *dz;  // 1/ψ̂ per retained mode (dy, dz offset by kmax)
     double kscale[3];            // 2π/L
@@ -346,15 +512,13 @@ struct FourierArgs {
 template <typename CT>
 __global__ void fourier_kernel(FourierArgs<CT> a, CT *__restrict__ spec, CT *__restrict__ spec2) {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t total = a.Hx * a.G[1] * a.G[2];
-    if (idx >= total) return;
-    const int ix = (int)(idx % a.Hx);
-    const int iy = (int)((idx / a.Hx) % a.G[1]);
-    const int iz = (int)(idx / (a.Hx * a.G[1]));
-    const int kx = ix;
-    const int ky = iy <= a.G[1] / 2 ? iy : iy - a.G[1];
+    if (idx >= a.cstride) return;
+    const int iz = (int)(idx % a.G[2]);
+    const int j = (int)((idx / a.G[2]) % a.nky);
+    const int kx = (int)(idx / ((int64_t)a.G[2] * a.nky));
+    const int ky = j <= a.kmax[1] ? j : j - a.nky;
     const int kz = iz <= a.G[2] / 2 ? iz : iz - a.G[2];
-    bool keep = kx <= a.kmax[0] && ky >= -a.kmax[1] && ky <= a.kmax[1] && kz >= -a.kmax[2] && kz <= a.kmax[2];
+    bool keep = kz >= -a.kmax[2] && kz <= a.kmax[2];
     const double KX = a.kscale[0] * kx, KY = a.kscale[1] * ky, KZ = a.kscale[2] * kz;
     const double k2 = KX * KX + KY * KY + KZ * KZ;
     if (a.kmax2_trunc >= 0 && k2 > a.kmax2_trunc) keep = false;
@@ -442,23 +606,25 @@ __global__ __launch_bounds__(256) void interp_kernel_wave(LRGeom g, int64_t Np, 
         const CT wx = es_kernel<CT>((CT)(((double)(i0x + a) - ux) / (double)M), beta);
         CT wz[W];
         int64_t gzo[W];
+        const int64_t plane = (int64_t)g.G[0] * g.G[1];  // values per z-plane
 #pragma unroll
         for (int c = 0; c < W; c++) {
             wz[c] = es_kernel<CT>((CT)(((double)(i0z + c) - uz) / (double)M), beta);
-            gzo[c] = (int64_t)wrapi(i0z + c, g.G[2]) * g.G[1];
+            gzo[c] = (int64_t)wrapi(i0z + c, g.G[2]) * plane;
         }
-        const int64_t gx = wrapi(i0x + a, g.G[0]);
+        const int64_t gx = 2 * (int64_t)wrapi(i0x + a, g.G[0]);
         CT acc0 = 0, acc1 = 0, acc2 = 0;
 #pragma unroll
         for (int pass = 0; pass < PASSES; pass++) {
             const int bb = b + R * pass;
             if (lane_on && bb < W) {
                 const CT wy = es_kernel<CT>((CT)(((double)(i0y + bb) - uy) / (double)M), beta);
-                const int64_t gy = wrapi(i0y + bb, g.G[1]);
+                const int iy = wrapi(i0y + bb, g.G[1]);
+                const int64_t gy = (int64_t)(iy >> 1) * 2 * g.G[0] + (iy & 1);
                 const CT wxy = wx * wy;
 #pragma unroll
                 for (int c = 0; c < W; c++) {
-                    const int64_t off = (gzo[c] + gy) * g.Gxp + gx;
+                    const int64_t off = gzo[c] + gy + gx;
                     const CT w = wxy * wz[c];
                     acc0 += w * grid[off];
                     acc1 += w * grid[g.cstride + off];
@@ -550,7 +716,6 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
         g.ntile[d] = c->ntile[d];
     }
     g.M = c->p.nufft_m;
-    g.Gxp = c->Gxp;
     g.cstride = c->comp_stride;
     const CT beta = (CT)c->es_beta;
 
@@ -562,21 +727,45 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
     stage_end(c, ST_LRSORT, s);
 
     CT *grid = (CT *)c->grid;
-    CT *spec = (CT *)c->spec;
-    CT *spec2 = (CT *)c->spec2;
     stage_begin(c, ST_SPREAD, s);
     tile_active<<<nblocks(c->ntiles, 256), 256, 0, s>>>(g, c->ntiles, c->lr_start, c->lr_active);
     const int64_t nb_spread = ((c->ntiles + 7) / 8) * 8;  // xcd_tile() maps 8·⌈n/8⌉ blocks onto the tiles
-    spread_kernel<CT><<<(unsigned)nb_spread, SPREAD_TPB, 0, s>>>(g, Nq, c->lr_start, c->lr_active, c->lr_i0,
-                                                                  (const CT *)c->lr_w, (const CT *)c->lr_chg, grid);
+    launch_spread<CT>(g, nb_spread, Nq, c->lr_start, c->lr_active, c->lr_i0, (const CT *)c->lr_w,
+                      (const CT *)c->lr_chg, grid, s);
     stage_end(c, ST_SPREAD, s);
     VPBS_HIP(c, hipGetLastError());
 
+    const int G0 = c->G[0], G1 = c->G[1], G2 = c->G[2];
+    const int nkx = c->nkx, nky = c->nky, K1 = c->kmax_mode[1];
+    Cx<CT> *rows = (Cx<CT> *)c->grid;
+    Cx<CT> *ypen = (Cx<CT> *)c->ypen;
+    auto fft = [&](int dim, int dir, void *buf) -> int {
+        void *io[1] = {buf};
+        if (rocfft_execute(c->plan[dim][dir], io, nullptr, c->fft_info) != rocfft_status_success)
+            return fail(c, VPBS_ERR_FFT, "rocfft_execute failed (dimension %d, %s)", dim,
+                        dir ? "inverse" : "forward");
+        return VPBS_OK;
+    };
+    // z-pencils -> ypen -> pair rows -> real grid in place
+    auto inverse = [&](Cx<CT> *zp) -> int {
+        VPBS_TRY(fft(2, 1, zp));
+        zpencils_to_ypencils<CT><<<dim3(G2 / 32, G1 / 32, 3 * nkx), 256, 0, s>>>(zp, ypen, G1, G2, nky, K1);
+        VPBS_TRY(fft(1, 1, ypen));
+        ypencils_to_pair_rows<CT><<<dim3(G0 / 32, G1 / 32, 3 * G2), 256, 0, s>>>(ypen, rows, G0, G1, G2, nkx);
+        VPBS_TRY(fft(0, 1, rows));
+        VPBS_HIP(c, hipGetLastError());
+        return VPBS_OK;
+    };
+
     stage_begin(c, ST_FFTF, s);
-    void *fin[1] = {grid}, *fout[1] = {spec};
-    if (rocfft_execute(c->plan_fwd, fin, fout, c->info_fwd) != rocfft_status_success)
-        return fail(c, VPBS_ERR_FFT, "rocfft_execute (forward) failed");
+    Cx<CT> *zpen = (Cx<CT> *)c->zpen;
+    VPBS_TRY(fft(0, 0, rows));
+    pair_split_to_ypencils<CT><<<dim3((nkx + 31) / 32, G1 / 32, 3 * G2), 256, 0, s>>>(rows, ypen, G0, G1, G2, nkx);
+    VPBS_TRY(fft(1, 0, ypen));
+    ypencils_to_zpencils<CT><<<dim3(G2 / 32, (nky + 31) / 32, 3 * nkx), 256, 0, s>>>(ypen, zpen, G1, G2, nky, K1);
+    VPBS_TRY(fft(2, 0, zpen));
     stage_end(c, ST_FFTF, s);
+    VPBS_HIP(c, hipGetLastError());
 
     FourierArgs<CT> fa;
     for (int d = 0; d < 3; d++) {
@@ -584,8 +773,9 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
         fa.kmax[d] = c->kmax_mode[d];
         fa.kscale[d] = 2 * M_PI / c->p.Ls[d];
     }
-    fa.Hx = c->G[0] / 2 + 1;
-    fa.cstride = c->spec_stride;
+    fa.nkx = nkx;
+    fa.nky = nky;
+    fa.cstride = c->zpen_stride;
     fa.dx = c->deconv;
     fa.dy = c->deconv + (c->kmax_mode[0] + 1);
     fa.dz = fa.dy + (2 * c->kmax_mode[1] + 1);
@@ -599,26 +789,20 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
     }
     fa.do_vel = vel;
     fa.do_psi = psi;
-    const int64_t nspec = fa.Hx * c->G[1] * c->G[2];
+    Cx<CT> *zpen2 = psi ? (Cx<CT> *)c->zpen2 : zpen;
     stage_begin(c, ST_FOURIER, s);
-    fourier_kernel<CT><<<nblocks(nspec, 256), 256, 0, s>>>(fa, spec, psi ? spec2 : spec);
+    fourier_kernel<CT><<<nblocks(fa.cstride, 256), 256, 0, s>>>(fa, (CT *)zpen, (CT *)zpen2);
     stage_end(c, ST_FOURIER, s);
     VPBS_HIP(c, hipGetLastError());
 
-    // velocity: spectrum -> real grid -> nodes; then the streamfunction reuses the grid
+    // velocity: z-pencils -> real grid -> nodes; then the streamfunction reuses the grid
     stage_begin(c, ST_FFTB, s);
-    if (vel) {
-        void *bin[1] = {spec}, *bout[1] = {grid};
-        if (rocfft_execute(c->plan_bwd, bin, bout, c->info_bwd) != rocfft_status_success)
-            return fail(c, VPBS_ERR_FFT, "rocfft_execute (backward) failed");
-    }
+    if (vel) VPBS_TRY(inverse(zpen));
     stage_end(c, ST_FFTB, s);
     stage_begin(c, ST_INTERP, s);
     if (vel) launch_interp<CT>(g, Np, c->nodes, beta, grid, c->v_lr, s);
     if (psi) {
-        void *bin[1] = {spec2}, *bout[1] = {grid};
-        if (rocfft_execute(c->plan_bwd, bin, bout, c->info_bwd) != rocfft_status_success)
-            return fail(c, VPBS_ERR_FFT, "rocfft_execute (backward, psi) failed");
+        VPBS_TRY(inverse(zpen2));
         launch_interp<CT>(g, Np, c->nodes, beta, grid, c->psi_lr, s);
     }
     stage_end(c, ST_INTERP, s);
@@ -652,14 +836,16 @@ int longrange_setup(vpbs_ctx *c) {
         c->ntile[d] = c->G[d] / (d == 0 ? TX : (d == 1 ? TY : TZ));
     }
     c->ntiles = (int64_t)c->ntile[0] * c->ntile[1] * c->ntile[2];
-    c->Gxp = c->G[0];
     c->comp_stride = (int64_t)c->G[0] * c->G[1] * c->G[2];
-    c->spec_stride = (int64_t)(c->G[0] / 2 + 1) * c->G[1] * c->G[2];
     c->es_beta = es_beta(m, c->p.nufft_sigma);
     // retained modes (longrange.jl:556-583)
     c->kmax_mode[0] = c->p.Ns[0] / 2 - 1;
     for (int d = 1; d < 3; d++)
         c->kmax_mode[d] = (c->p.Ns[d] % 2 == 0) ? c->p.Ns[d] / 2 - 1 : (c->p.Ns[d] - 1) / 2;
+    c->nkx = c->kmax_mode[0] + 1;
+    c->nky = 2 * c->kmax_mode[1] + 1;
+    c->ypen_stride = (int64_t)c->nkx * c->G[2] * c->G[1];
+    c->zpen_stride = (int64_t)c->nkx * c->nky * c->G[2];
 
     // deconvolution tables 1/ψ̂(k)
     std::vector<double> gx, gw;
@@ -672,50 +858,43 @@ int longrange_setup(vpbs_ctx *c) {
     VPBS_HIP(c, hipMalloc((void **)&c->deconv, sizeof(double) * tab.size()));
     VPBS_HIP(c, hipMemcpy(c->deconv, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
 
-    // grid buffers: real grid and half spectrum (out-of-place transforms keep the
-    // real rows unpadded, i.e. aligned to 128 B for spreading and interpolation)
     const size_t es = c->p.precision == VPBS_F32 ? sizeof(float) : sizeof(double);
     VPBS_HIP(c, hipMalloc(&c->grid, es * 3 * (size_t)c->comp_stride));
-    VPBS_HIP(c, hipMalloc(&c->spec, 2 * es * 3 * (size_t)c->spec_stride));
+    VPBS_HIP(c, hipMalloc(&c->ypen, 2 * es * 3 * (size_t)c->ypen_stride));
+    VPBS_HIP(c, hipMalloc(&c->zpen, 2 * es * 3 * (size_t)c->zpen_stride));
 
-    // rocFFT plans: 3 transforms (components) of G0×G1×G2, x fastest, out of place
-    rocfft_precision prec = c->p.precision == VPBS_F32 ? rocfft_precision_single : rocfft_precision_double;
-    size_t len[3] = {(size_t)c->G[0], (size_t)c->G[1], (size_t)c->G[2]};
-    const size_t Hx = c->G[0] / 2 + 1;
-    size_t rstr[3] = {1, (size_t)c->Gxp, (size_t)c->Gxp * c->G[1]};
-    size_t cstr[3] = {1, Hx, Hx * c->G[1]};
-    size_t rdist = (size_t)c->comp_stride, cdist = (size_t)c->spec_stride;
-    rocfft_plan_description df = nullptr, db = nullptr;
-    if (!rocfft_ok(rocfft_plan_description_create(&df)) || !rocfft_ok(rocfft_plan_description_create(&db)))
-        return fail(c, VPBS_ERR_FFT, "rocfft_plan_description_create failed");
-    if (!rocfft_ok(rocfft_plan_description_set_data_layout(df, rocfft_array_type_real,
-                                                           rocfft_array_type_hermitian_interleaved, nullptr,
-                                                           nullptr, 3, rstr, rdist, 3, cstr, cdist)) ||
-        !rocfft_ok(rocfft_plan_description_set_data_layout(db, rocfft_array_type_hermitian_interleaved,
-                                                           rocfft_array_type_real, nullptr, nullptr, 3, cstr,
-                                                           cdist, 3, rstr, rdist)))
-        return fail(c, VPBS_ERR_FFT, "rocfft_plan_description_set_data_layout failed");
-    if (!rocfft_ok(rocfft_plan_create(&c->plan_fwd, rocfft_placement_notinplace,
-                                      rocfft_transform_type_real_forward, prec, 3, len, 3, df)) ||
-        !rocfft_ok(rocfft_plan_create(&c->plan_bwd, rocfft_placement_notinplace,
-                                      rocfft_transform_type_real_inverse, prec, 3, len, 3, db)))
-        return fail(c, VPBS_ERR_FFT, "rocfft_plan_create failed (grid %d x %d x %d)", c->G[0], c->G[1], c->G[2]);
-    rocfft_plan_description_destroy(df);
-    rocfft_plan_description_destroy(db);
-    size_t wf = 0, wb = 0;
-    rocfft_plan_get_work_buffer_size(c->plan_fwd, &wf);
-    rocfft_plan_get_work_buffer_size(c->plan_bwd, &wb);
-    c->fft_work_size = wf > wb ? wf : wb;
-    if (c->fft_work_size) VPBS_HIP(c, hipMalloc(&c->fft_work, c->fft_work_size));
-    if (!rocfft_ok(rocfft_execution_info_create(&c->info_fwd)) ||
-        !rocfft_ok(rocfft_execution_info_create(&c->info_bwd)))
-        return fail(c, VPBS_ERR_FFT, "rocfft_execution_info_create failed");
-    rocfft_execution_info_set_stream(c->info_fwd, c->s_lr);
-    rocfft_execution_info_set_stream(c->info_bwd, c->s_lr);
-    if (c->fft_work_size) {
-        rocfft_execution_info_set_work_buffer(c->info_fwd, c->fft_work, c->fft_work_size);
-        rocfft_execution_info_set_work_buffer(c->info_bwd, c->fft_work, c->fft_work_size);
+    // rocFFT: in-place batched contiguous c2c along each direction
+    const rocfft_precision prec = c->p.precision == VPBS_F32 ? rocfft_precision_single : rocfft_precision_double;
+    const size_t len[3] = {(size_t)c->G[0], (size_t)c->G[1], (size_t)c->G[2]};
+    const size_t batch[3] = {3 * (size_t)c->G[2] * (c->G[1] / 2), 3 * (size_t)c->nkx * c->G[2],
+                             3 * (size_t)c->nkx * c->nky};
+    c->fft_work_size = 0;
+    for (int d = 0; d < 3; d++) {
+        for (int dir = 0; dir < 2; dir++) {
+            rocfft_plan_description desc = nullptr;
+            if (!rocfft_ok(rocfft_plan_description_create(&desc)))
+                return fail(c, VPBS_ERR_FFT, "rocfft_plan_description_create failed");
+            const size_t stride[1] = {1};
+            const bool ok =
+                rocfft_ok(rocfft_plan_description_set_data_layout(
+                    desc, rocfft_array_type_complex_interleaved, rocfft_array_type_complex_interleaved, nullptr,
+                    nullptr, 1, stride, len[d], 1, stride, len[d])) &&
+                rocfft_ok(rocfft_plan_create(&c->plan[d][dir], rocfft_placement_inplace,
+                                             dir ? rocfft_transform_type_complex_inverse
+                                                 : rocfft_transform_type_complex_forward,
+                                             prec, 1, &len[d], batch[d], desc));
+            rocfft_plan_description_destroy(desc);
+            if (!ok) return fail(c, VPBS_ERR_FFT, "rocfft_plan_create failed (length %zu, batch %zu)", len[d], batch[d]);
+            size_t w = 0;
+            rocfft_plan_get_work_buffer_size(c->plan[d][dir], &w);
+            c->fft_work_size = std::max(c->fft_work_size, w);
+        }
     }
+    if (c->fft_work_size) VPBS_HIP(c, hipMalloc(&c->fft_work, c->fft_work_size));
+    if (!rocfft_ok(rocfft_execution_info_create(&c->fft_info)))
+        return fail(c, VPBS_ERR_FFT, "rocfft_execution_info_create failed");
+    rocfft_execution_info_set_stream(c->fft_info, c->s_lr);
+    if (c->fft_work_size) rocfft_execution_info_set_work_buffer(c->fft_info, c->fft_work, c->fft_work_size);
     c->plans_ready = true;
     return VPBS_OK;
 }
@@ -723,24 +902,23 @@ int longrange_setup(vpbs_ctx *c) {
 int longrange_compute(vpbs_ctx *c, uint32_t flags) {
     const bool psi = flags & VPBS_STREAMFUNCTION;
     const size_t es = c->p.precision == VPBS_F32 ? sizeof(float) : sizeof(double);
-    if (psi && !c->spec2) VPBS_HIP(c, hipMalloc(&c->spec2, 2 * es * 3 * (size_t)c->spec_stride));
+    if (psi && !c->zpen2) VPBS_HIP(c, hipMalloc(&c->zpen2, 2 * es * 3 * (size_t)c->zpen_stride));
     if (c->p.precision == VPBS_F32) return launch_lr<float>(c, flags);
     return launch_lr<double>(c, flags);
 }
 
 void longrange_release(vpbs_ctx *c) {
-    if (c->plan_fwd) rocfft_plan_destroy(c->plan_fwd);
-    if (c->plan_bwd) rocfft_plan_destroy(c->plan_bwd);
-    if (c->info_fwd) rocfft_execution_info_destroy(c->info_fwd);
-    if (c->info_bwd) rocfft_execution_info_destroy(c->info_bwd);
-    if (c->fft_work) (void)hipFree(c->fft_work);
-    if (c->grid) (void)hipFree(c->grid);
-    if (c->spec) (void)hipFree(c->spec);
-    if (c->spec2) (void)hipFree(c->spec2);
-    if (c->deconv) (void)hipFree(c->deconv);
-    c->plan_fwd = c->plan_bwd = nullptr;
-    c->info_fwd = c->info_bwd = nullptr;
-    c->fft_work = c->grid = c->spec = c->spec2 = nullptr;
+    for (auto &pd : c->plan)
+        for (auto &p : pd) {
+            if (p) rocfft_plan_destroy(p);
+            p = nullptr;
+        }
+    if (c->fft_info) rocfft_execution_info_destroy(c->fft_info);
+    c->fft_info = nullptr;
+    void *bufs[] = {c->fft_work, c->grid, c->ypen, c->zpen, c->zpen2, c->deconv};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    c->fft_work = c->grid = c->ypen = c->zpen = c->zpen2 = nullptr;
     c->deconv = nullptr;
 }
 

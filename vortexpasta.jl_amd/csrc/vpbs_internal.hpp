@@ -90,14 +90,19 @@ struct vpbs_ctx {
 
     // long range
     int32_t G[3] = {0, 0, 0};   // oversampled grid
-    int64_t Gxp = 0;            // row pitch of the real grid (= G0: rows are 128-B aligned)
-    int64_t comp_stride = 0;    // real elements per component (G0·G1·G2)
-    int64_t spec_stride = 0;    // complex elements per component of the half spectrum ((G0/2+1)·G1·G2)
+    int64_t comp_stride = 0;    // real grid values per component (G0·G1·G2)
     int32_t ntile[3] = {0, 0, 0};
     int64_t ntiles = 0;
-    void *grid = nullptr;       // real grid, 3 components (spreading output, c2r output)
-    void *spec = nullptr;       // half spectrum, 3 components (vorticity, then velocity, in place)
-    void *spec2 = nullptr;      // half spectrum of the streamfunction (allocated on demand)
+    // Real grid, 3 components, rows of y = 2p and y = 2p+1 interleaved as one
+    // complex row (pair p): value (x, y, z) at ((z·G1/2 + y/2)·G0 + x)·2 + (y & 1).
+    // Spreading writes it, the x-pass transforms it in place, interpolation reads it.
+    void *grid = nullptr;
+    // Pencils of retained modes (complex): y-pencils [3][nkx][G2][G1] and
+    // z-pencils [3][nkx][nky][G2] (vorticity, then velocity, in place); the
+    // streamfunction gets its own z-pencils (allocated on demand).
+    int32_t nkx = 0, nky = 0;
+    int64_t ypen_stride = 0, zpen_stride = 0;  // complex values per component
+    void *ypen = nullptr, *zpen = nullptr, *zpen2 = nullptr;
     int32_t *lr_key = nullptr, *lr_start = nullptr, *lr_perm = nullptr;
     uint8_t *lr_active = nullptr;  // per tile: 1 if any neighbour tile holds a point
     int32_t *lr_i0 = nullptr;   // tile-sorted points: first stencil index per direction, 3*Nq
@@ -106,8 +111,9 @@ struct vpbs_ctx {
     double *deconv = nullptr;   // 1/ψ̂ tables for retained kx, ky, kz (concatenated)
     int32_t kmax_mode[3] = {0, 0, 0};
     double es_beta = 0;
-    rocfft_plan plan_fwd = nullptr, plan_bwd = nullptr;
-    rocfft_execution_info info_fwd = nullptr, info_bwd = nullptr;
+    // rocFFT: batched contiguous 1D c2c transforms along x, y and z, forward and inverse
+    rocfft_plan plan[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    rocfft_execution_info fft_info = nullptr;
     void *fft_work = nullptr;
     size_t fft_work_size = 0;
     bool plans_ready = false;
