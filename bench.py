@@ -47,7 +47,9 @@ def build_workload(vp, cfg: int, sigma: float = 1.5, m: int = 4):
     p32 = vp.ParamsBiotSavart.build(np.float32, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl, backend_long=bl)
     p64 = vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl, backend_long=bl)
     batch = vp.FilamentBatch(fs)
+    t = time.perf_counter()
     pd = vp.add_point_charges(fs, p64, batch=batch)
+    pd.host_build_s = time.perf_counter() - t
     return fs, spl, p32, p64, pd
 
 
@@ -72,6 +74,31 @@ def time_path(vp, params, pd, steps: int, warmup: int):
     grid, cells = ctx.sizes()
     ctx.close()
     return ms, stages, v, grid, cells
+
+
+def time_point_data(vp, params, pd, reps: int = 3):
+    """Device point-data build (vpbs_set_filaments) against the host add_point_charges.
+
+    Reports the kernel time (HIP events), the wall time of the call (upload of
+    nodes, knots and spline coefficients included) and whether the device point
+    data is bit-identical to the host-built one.
+    """
+    ctx = vp._lib.Context(params)
+    ctx.set_filaments(pd.batch, params)  # first call allocates
+    walls, kern = [], []
+    for _ in range(reps):
+        t = time.perf_counter()
+        ctx.set_filaments(pd.batch, params)
+        walls.append(time.perf_counter() - t)
+        kern.append(ctx.timings()["point_data"])
+    dev = ctx.get_points()
+    ctx.close()
+    same = all(np.array_equal(dev[k], getattr(pd, k))
+               for k in ("nodes", "node_idx_prev", "sp", "spp", "dlm", "dlp", "points", "charges"))
+    return {"device_kernel_ms": round(float(np.median(kern)), 4),
+            "device_call_ms": round(float(np.median(walls)) * 1e3, 3),
+            "host_add_point_charges_s": round(pd.host_build_s, 3),
+            "bit_identical_to_host": bool(same)}
 
 
 def stage_bytes(stage: str, s: int, G, Ns, Np: int, Nq: int) -> float:
@@ -202,6 +229,9 @@ def main():
         res64 = {"value": pd.Np / (ms64 * 1e-3), "ms_per_step": ms64, "dtype": "f64",
                  "stages_ms": {k: round(v, 4) for k, v in st64.items()}}
 
+    pdt = time_point_data(vp, p64, pd)
+    log(f"point data: {json.dumps(pdt)}")
+
     # roofline of the dominant long-range kernel of the FP32 path (our own kernels only)
     lr_kernels = ["spread", "fourier", "interp"]
     dom = max(lr_kernels, key=lambda k: st32.get(k, 0.0))
@@ -232,7 +262,8 @@ def main():
             "alpha": spl.α, "rcut": spl.rcut,
         },
         "roofline": {
-            "kernel": f"{dom}_kernel<float>",
+            "kernel": {"spread": f"spread_rows_kernel<float, {2 * args.m}>", "fourier": "fourier_kernel<float>",
+                       "interp": f"interp_kernel_wave<float, {2 * args.m}>"}[dom],
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -244,6 +275,7 @@ def main():
         },
         "stages_ms": {k: round(v, 4) for k, v in st32.items()},
         "stage_algorithmic_GBps": per_stage,
+        "point_data": pdt,
     }
     if res64 is not None:
         result["fp64_path"] = res64

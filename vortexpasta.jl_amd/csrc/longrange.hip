@@ -13,12 +13,11 @@
 //
 // Kernels:
 //   lr_tile_keys    tile of each quadrature point on the σ-oversampled grid
-//   spread_scatter_kernel  one workgroup per 32×8×8 grid tile takes the points of the
-//                   3×3×3 neighbouring tiles whose support overlaps it and adds their
-//                   kernel-weighted charges into an LDS copy of the tile (each
-//                   wavefront owns a z-slab, points in sorted order): every grid value
-//                   written exactly once, no global atomics, no zero-fill pass,
-//                   deterministic summation order
+//   spread_rows_kernel  one wavefront per grid tile (one 128-B row wide, 8×8 rows)
+//                   takes the points of the 3×3×3 neighbouring tiles whose support
+//                   overlaps it and adds their kernel-weighted charges into the tile
+//                   rows held in registers (lane = row): every grid value written
+//                   exactly once, no atomics, no zero-fill pass, deterministic order
 //   FFT             batched contiguous 1D c2c transforms on rocFFT along x, y and
 //                   z, with pencil transposes of our own between them:
 //                   x: the real grid holds pairs of y-rows as one complex row, so
@@ -38,13 +37,21 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace vpbs {
 
 namespace {
 
-constexpr int TX = 32, TY = 8, TZ = 8;  // spread tile (32 cells = one 128-B row in FP32)
+// Spreading tile: 16 × 8 × 8 cells. A pair row of a tile (rows y and y+1
+// interleaved) is 32 values, 128 B in FP32; 16 x-cells keep the per-lane
+// accumulators (16 × 3) in registers at 4 wavefronts per SIMD.
+template <typename CT>
+constexpr int tile_x() {
+    return 16;
+}
+constexpr int TY = 8, TZ = 8;
 constexpr int MAXM = 8;     // largest supported half-support
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -71,9 +78,14 @@ struct LRGeom {
     int G[3];
     double scale[3];  // G/L
     int ntile[3];
+    int tx;           // spreading tile width along x (cells)
     int M;
     int64_t cstride;  // real grid values per component
 };
+
+// Per tile-sorted point, the record read by spreading: the 2m kernel weights
+// along x, y and z, then the 3 charge components, padded to 16 bytes.
+constexpr int rec_len(int W) { return (3 * W + 3 + 3) & ~3; }
 
 // Position of real grid value (x, y, z) of one component: rows y = 2p and
 // y = 2p + 1 are interleaved as the real and imaginary parts of complex row p.
@@ -96,32 +108,34 @@ __global__ void lr_tile_keys(int64_t Nq, const double *__restrict__ pts, LRGeom 
     for (int d = 0; d < 3; d++) {
         double u = pts[d * Nq + j] * g.scale[d];
         int i = wrapi((int)floor(u), g.G[d]);
-        t[d] = i / (d == 0 ? TX : (d == 1 ? TY : TZ));
+        t[d] = i / (d == 0 ? g.tx : (d == 1 ? TY : TZ));
     }
     key[j] = (t[2] * g.ntile[1] + t[1]) * g.ntile[0] + t[0];
 }
 
-// Tile-sorted copy of the points with, per point and direction d, the first
-// stencil index i0_d = ⌊u_d⌋ - m + 1 (wrapped into [0, G_d)) and the 2m kernel
-// weights φ((i0_d + k - u_d)/m), k = 0..2m-1, computed once here instead of
-// once per tile in the spreading kernel.
+// Tile-sorted copy of the points: per point and direction d the first stencil
+// index i0_d = ⌊u_d⌋ - m + 1 (wrapped into [0, G_d), component-major for the
+// overlap tests) and one record (rec_len(2m) values) holding the 2m kernel
+// weights φ((i0_d + k - u_d)/m) per direction and the charge, computed once
+// here instead of once per tile in the spreading kernel.
 template <typename CT>
 __global__ void gather_points(int64_t Nq, const int32_t *__restrict__ perm, const double *__restrict__ pts,
                               const double *__restrict__ chg, LRGeom g, CT beta, int32_t *__restrict__ si0,
-                              CT *__restrict__ sw, CT *__restrict__ schg) {
+                              CT *__restrict__ rec) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= Nq) return;
     const int32_t j = perm[s];
     const int M = g.M, W = 2 * M;
+    CT *r = rec + s * rec_len(W);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const double u = pts[d * Nq + j] * g.scale[d];
         const int i0 = (int)floor(u) - M + 1;
         si0[d * Nq + s] = wrapi(i0, g.G[d]);
-        for (int k = 0; k < W; k++)
-            sw[((int64_t)d * W + k) * Nq + s] = es_kernel<CT>((CT)(((double)(i0 + k) - u) / (double)M), beta);
-        schg[d * Nq + s] = (CT)chg[d * Nq + j];
+        for (int k = 0; k < W; k++) r[d * W + k] = es_kernel<CT>((CT)(((double)(i0 + k) - u) / (double)M), beta);
+        r[3 * W + d] = (CT)chg[d * Nq + j];
     }
+    for (int k = 3 * W + 3; k < rec_len(W); k++) r[k] = (CT)0;
 }
 
 // ----------------------------------------------------------------------------
@@ -161,54 +175,64 @@ struct alignas(16) Q4 {
     CT x, y, z, w;
 };
 
-// Point-stationary spreading: one workgroup (4 wavefronts) per 32×8×8 tile,
-// wavefront w owning the z-slab [2w, 2w+2) of the tile in LDS.
-// Per round of up to NB candidates from the 3×3×3 neighbour tiles:
-//   A) keep the points whose support overlaps the tile (ballot compaction,
-//      candidate order preserved), recording the support start relative to
-//      the tile start, wrap1(i0 - t0, G), per direction;
-//   B) stage the kept points' 3W weights and charges in LDS;
-//   C) each wavefront walks the kept points in order and adds q·wx·wy·wz into
-//      the cells of its slab inside the support: lane (a, b) takes support
-//      offsets (8i + a, 8j + b) in x and y. A wavefront never shares a cell
-//      with another and adds the points in a fixed order, so the sums are
-//      deterministic; within one point the lanes hit distinct cells.
-// The tile (pair rows interleaved, as in the grid) is then written once.
-template <typename CT, int W>
-__global__ __launch_bounds__(256) void spread_scatter_kernel(LRGeom g, int64_t Nq, const int32_t *__restrict__ tstart,
-                                                             const uint8_t *__restrict__ active,
-                                                             const int32_t *__restrict__ si0,
-                                                             const CT *__restrict__ sw,
-                                                             const CT *__restrict__ schg, CT *__restrict__ grid) {
-    constexpr int NB = sizeof(CT) == 4 ? 128 : 64;  // kept points per round
-    constexpr int NPAIR = TY / 2;
-    constexpr int PITCH = 2 * TX + (sizeof(CT) == 4 ? 16 : 8);  // LDS pitch of a pair row (values)
-    constexpr int NPASS = (W + 7) / 8;                           // 8-wide passes per direction
-    __shared__ CT acc[3][TZ][NPAIR][PITCH];
-    __shared__ CT s_w[NB][3][W];
-    __shared__ CT s_q[NB][3];
-    __shared__ int4 s_r[NB];  // (rx, ry, rz, sorted point index)
-    __shared__ int nb_start[27];
-    __shared__ int nb_len[27];
-    __shared__ int nb_pre[28];
-    __shared__ int s_wave_cnt[4];
+// f(std::integral_constant<int, I>) for I = B .. E-1: register arrays indexed
+// through it keep compile-time indices and stay in registers.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
 
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wid = tid >> 6;
+// Spreading (type 1): one wavefront per TX×8×8 tile, lane (y, z) = (lane % 8,
+// lane / 8) owning tile row (y, z) and accumulating its TX cells × 3
+// components in registers.
+//  1) lanes 0..26 read the point ranges of the 3×3×3 neighbour tiles at once;
+//  2) the candidates are tested 64 at a time for support overlap; the kept ones
+//     are compacted in order and their records (weights, charge) copied to LDS;
+//  3) the wavefront walks the kept points: each lane weights the charge by its
+//     row's wy·wz, and adds wx[k]·(…) to cells sx + k through a switch on the
+//     uniform offset sx whose cases use compile-time register indices.
+// Summation order is fixed (neighbour tile, then sorted order): results are
+// deterministic. The tile is written once, transposed through LDS into
+// contiguous pair rows.
+template <typename CT, int W>
+__global__ __launch_bounds__(64) void spread_rows_kernel(LRGeom g, int64_t Nq, const int32_t *__restrict__ tstart,
+                                                         const uint8_t *__restrict__ active,
+                                                         const int32_t *__restrict__ si0,
+                                                         const CT *__restrict__ rec, CT *__restrict__ grid) {
+    constexpr int TX = tile_x<CT>();
+    constexpr int REC = rec_len(W);
+    constexpr int TXP = TX + 1;  // LDS row pitch (odd: conflict-free column writes)
+    constexpr int NPAIR = TY / 2;
+    __shared__ CT buf[TY * TZ * TXP];
+    __shared__ CT s_rec[64][REC];
+    __shared__ int4 s_meta[64];  // (sx, ry, rz, -) of the kept points
+    __shared__ int nb_start[27];
+    __shared__ int nb_pre[28];
+
+    const int lane = threadIdx.x;
     const int64_t ntl = (int64_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
     const int64_t tile = xcd_tile(blockIdx.x, ntl);
-    if (tile >= ntl) return;  // uniform over the workgroup
+    if (tile >= ntl) return;  // the whole (one-wavefront) block
     const int tix = (int)(tile % g.ntile[0]);
     const int tiy = (int)((tile / g.ntile[0]) % g.ntile[1]);
     const int tiz = (int)(tile / ((int64_t)g.ntile[0] * g.ntile[1]));
-    const int t0[3] = {tix * TX, tiy * TY, tiz * TZ};
+    const int t0x = tix * TX, t0y = tiy * TY, t0z = tiz * TZ;
+    const int ly = lane & 7, lz = lane >> 3;
 
-    CT *accf = &acc[0][0][0][0];
-    for (int e = tid; e < 3 * TZ * NPAIR * PITCH; e += 256) accf[e] = (CT)0;
+    CT acc[TX][3];
+    static_for<0, TX>([&](auto X) {
+        constexpr int x = decltype(X)::value;
+        acc[x][0] = acc[x][1] = acc[x][2] = (CT)0;
+    });
 
     if (active[tile]) {
-        if (tid < 27) {
-            const int o[3] = {tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1};
+        // 1) neighbour ranges; with fewer than 3 tiles along d a neighbour repeats: empty range
+        int len = 0;
+        if (lane < 27) {
+            const int o[3] = {lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1};
             const int tt[3] = {tix, tiy, tiz};
             int tn[3];
             bool dup = false;
@@ -219,31 +243,27 @@ __global__ __launch_bounds__(256) void spread_scatter_kernel(LRGeom g, int64_t N
             }
             const int64_t tl = ((int64_t)tn[2] * g.ntile[1] + tn[1]) * g.ntile[0] + tn[0];
             const int s0 = tstart[tl];
-            nb_start[tid] = s0;
-            nb_len[tid] = dup ? 0 : tstart[tl + 1] - s0;
+            nb_start[lane] = s0;
+            len = dup ? 0 : tstart[tl + 1] - s0;
         }
-        __syncthreads();
-        if (tid < 64) {
-            int incl = tid < 27 ? nb_len[tid] : 0;
+        int incl = len;
 #pragma unroll
-            for (int off = 1; off < 32; off <<= 1) {
-                int y = __shfl_up(incl, off, 64);
-                if (tid >= off) incl += y;
-            }
-            if (tid < 27) nb_pre[tid + 1] = incl;
-            if (tid == 0) nb_pre[0] = 0;
+        for (int off = 1; off < 32; off <<= 1) {
+            const int y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
         }
+        if (lane < 27) nb_pre[lane + 1] = incl;
+        if (lane == 0) nb_pre[0] = 0;
+        const int ncand = __shfl(incl, 26, 64);
         __syncthreads();
-        const int ncand = nb_pre[27];
-        const int a8 = lane & 7, b8 = lane >> 3;
 
-        for (int base = 0; base < ncand; base += NB) {
-            // A) overlap test and compaction
+        for (int base = 0; base < ncand; base += 64) {
+            // 2) overlap test, compaction, record staging
+            const int cnd = base + lane;
             bool keep = false;
-            int4 rr = {0, 0, 0, 0};
-            if (tid < NB && base + tid < ncand) {
-                const int cnd = base + tid;
-                int lo = 0, hi = 26;  // largest nbi with nb_pre[nbi] <= cnd
+            int sidx = 0, sx = 0, ry = 0, rz = 0;
+            if (cnd < ncand) {
+                int lo = 0, hi = 26;  // largest neighbour nb with nb_pre[nb] <= cnd
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
                     if (nb_pre[mid] <= cnd)
@@ -251,117 +271,104 @@ __global__ __launch_bounds__(256) void spread_scatter_kernel(LRGeom g, int64_t N
                     else
                         hi = mid - 1;
                 }
-                const int sidx = nb_start[lo] + (cnd - nb_pre[lo]);
-                int r[3];
-                keep = true;
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const int i0 = si0[d * Nq + sidx];
-                    const int T = d == 0 ? TX : (d == 1 ? TY : TZ);
-                    r[d] = wrap1(i0 - t0[d], g.G[d]);                       // support start rel. to tile
-                    keep = keep && (r[d] < T || wrap1(t0[d] - i0, g.G[d]) < W);  // overlap (periodic)
-                }
-                rr = make_int4(r[0], r[1], r[2], sidx);
+                sidx = nb_start[lo] + (cnd - nb_pre[lo]);
+                const int ix = si0[sidx], iy = si0[Nq + sidx], iz = si0[2 * Nq + sidx];
+                const int rx = wrap1(ix - t0x, g.G[0]);  // support start relative to the tile start
+                ry = wrap1(iy - t0y, g.G[1]);
+                rz = wrap1(iz - t0z, g.G[2]);
+                keep = (rx < TX || wrap1(t0x - ix, g.G[0]) < W) && (ry < TY || wrap1(t0y - iy, g.G[1]) < W) &&
+                       (rz < TZ || wrap1(t0z - iz, g.G[2]) < W);
+                sx = rx < TX ? rx : rx - g.G[0];  // ∈ (-W, TX)
             }
-            const unsigned long long ball = __ballot(keep);
-            if (lane == 0) s_wave_cnt[wid] = __popcll(ball);
-            __syncthreads();
+            const unsigned long long mask = __ballot(keep);
+            const int nkept = __popcll(mask);
             if (keep) {
-                int pos = __popcll(ball & ((1ull << lane) - 1ull));
-                for (int w = 0; w < wid; w++) pos += s_wave_cnt[w];
-                s_r[pos] = rr;
+                const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+                s_meta[pos] = make_int4(sx, ry, rz, 0);
+                const CT *pr = rec + (int64_t)sidx * REC;
+#pragma unroll
+                for (int k = 0; k < REC; k++) s_rec[pos][k] = pr[k];
             }
             __syncthreads();
-            const int ns = s_wave_cnt[0] + s_wave_cnt[1] + s_wave_cnt[2] + s_wave_cnt[3];
-            // B) stage weights (component-major source: row d·W + k, point p fastest) and charges
-            for (int e = tid; e < ns * 3 * W; e += 256) {
-                const int p = e % ns, r = e / ns;
-                s_w[p][r / W][r % W] = sw[(int64_t)r * Nq + s_r[p].w];
-            }
-            for (int e = tid; e < ns * 3; e += 256) {
-                const int p = e % ns, d = e / ns;
-                s_q[p][d] = schg[(int64_t)d * Nq + s_r[p].w];
-            }
-            __syncthreads();
-            // C) scatter into this wavefront's slab
-            const int zs = 2 * wid;
-            for (int p = 0; p < ns; p++) {
-                const int4 r = s_r[p];
-                const int kz0 = wrap1(zs - r.z, g.G[2]), kz1 = wrap1(zs + 1 - r.z, g.G[2]);
-                if (kz0 >= W && kz1 >= W) continue;  // uniform: support misses the slab
-                const CT q0 = s_q[p][0], q1 = s_q[p][1], q2 = s_q[p][2];
-                const CT wz0 = kz0 < W ? s_w[p][2][kz0] : (CT)0;
-                const CT wz1 = kz1 < W ? s_w[p][2][kz1] : (CT)0;
-#pragma unroll
-                for (int pi = 0; pi < NPASS; pi++) {
-#pragma unroll
-                    for (int pj = 0; pj < NPASS; pj++) {
-                        const int kx = 8 * pi + a8, ky = 8 * pj + b8;
-                        const int lx = wrap1(r.x + kx, g.G[0]), ly = wrap1(r.y + ky, g.G[1]);
-                        if (kx < W && ky < W && lx < TX && ly < TY) {
-                            const CT wxy = s_w[p][0][kx] * s_w[p][1][ky];
-                            // plain read-modify-write: the slab is this wavefront's alone and
-                            // its LDS accesses execute in program order, so the next point's
-                            // reads see these writes (LDS float atomics are far slower)
-                            CT *cell = &acc[0][zs][ly >> 1][2 * lx + (ly & 1)];
-                            constexpr int CS = TZ * NPAIR * PITCH, ZS = NPAIR * PITCH;
-                            if (kz0 < W && kz1 < W) {
-                                const CT f0 = wxy * wz0, f1 = wxy * wz1;
-                                CT a0 = cell[0], a1 = cell[CS], a2 = cell[2 * CS];
-                                CT b0 = cell[ZS], b1 = cell[CS + ZS], b2 = cell[2 * CS + ZS];
-                                a0 += f0 * q0;
-                                a1 += f0 * q1;
-                                a2 += f0 * q2;
-                                b0 += f1 * q0;
-                                b1 += f1 * q1;
-                                b2 += f1 * q2;
-                                cell[0] = a0;
-                                cell[CS] = a1;
-                                cell[2 * CS] = a2;
-                                cell[ZS] = b0;
-                                cell[CS + ZS] = b1;
-                                cell[2 * CS + ZS] = b2;
-                            } else {
-                                const int zo = kz0 < W ? 0 : ZS;
-                                const CT f = wxy * (kz0 < W ? wz0 : wz1);
-                                CT a0 = cell[zo], a1 = cell[CS + zo], a2 = cell[2 * CS + zo];
-                                a0 += f * q0;
-                                a1 += f * q1;
-                                a2 += f * q2;
-                                cell[zo] = a0;
-                                cell[CS + zo] = a1;
-                                cell[2 * CS + zo] = a2;
-                            }
+            // 3) accumulate the kept points in order
+            for (int p = 0; p < nkept; p++) {
+                const int4 m = s_meta[p];
+                const int ky = wrap1(ly - m.y, g.G[1]), kz = wrap1(lz - m.z, g.G[2]);
+                CT wyz = (CT)0;
+                if (ky < W && kz < W) wyz = s_rec[p][W + ky] * s_rec[p][2 * W + kz];
+                const CT a0 = wyz * s_rec[p][3 * W], a1 = wyz * s_rec[p][3 * W + 1], a2 = wyz * s_rec[p][3 * W + 2];
+                CT wx[W];
+                static_for<0, W>([&](auto K) { wx[decltype(K)::value] = s_rec[p][decltype(K)::value]; });
+                // acc[sx + k] += wx[k]·a: one straight-line case per uniform offset,
+                // all indices compile-time constants (the accumulators stay in registers)
+                auto leaf = [&](auto S) {
+                    constexpr int S0 = decltype(S)::value;
+                    static_for<0, W>([&](auto K) {
+                        constexpr int k = decltype(K)::value, x = S0 + k;
+                        if constexpr (x >= 0 && x < TX) {
+                            acc[x][0] += wx[k] * a0;
+                            acc[x][1] += wx[k] * a1;
+                            acc[x][2] += wx[k] * a2;
                         }
+                    });
+                };
+                // a single tile along x: the support can also wrap back into the same tile
+                const int sx2 = (g.G[0] == TX && m.x + W > TX) ? m.x - TX : TX;
+                for (int o = m.x; o < TX; o = (o == m.x ? sx2 : TX)) {
+#define VPBS_SX_CASE(v) \
+    case v: leaf(std::integral_constant<int, v>{}); break;
+                    switch (o) {
+                        VPBS_SX_CASE(-15) VPBS_SX_CASE(-14) VPBS_SX_CASE(-13) VPBS_SX_CASE(-12)
+                        VPBS_SX_CASE(-11) VPBS_SX_CASE(-10) VPBS_SX_CASE(-9) VPBS_SX_CASE(-8)
+                        VPBS_SX_CASE(-7) VPBS_SX_CASE(-6) VPBS_SX_CASE(-5) VPBS_SX_CASE(-4)
+                        VPBS_SX_CASE(-3) VPBS_SX_CASE(-2) VPBS_SX_CASE(-1) VPBS_SX_CASE(0)
+                        VPBS_SX_CASE(1) VPBS_SX_CASE(2) VPBS_SX_CASE(3) VPBS_SX_CASE(4)
+                        VPBS_SX_CASE(5) VPBS_SX_CASE(6) VPBS_SX_CASE(7) VPBS_SX_CASE(8)
+                        VPBS_SX_CASE(9) VPBS_SX_CASE(10) VPBS_SX_CASE(11) VPBS_SX_CASE(12)
+                        VPBS_SX_CASE(13) VPBS_SX_CASE(14) VPBS_SX_CASE(15)
+                        default: break;
                     }
+#undef VPBS_SX_CASE
                 }
+                static_assert(W <= 16 && TX <= 16, "offset cases cover (-16, 16)");
             }
             __syncthreads();
         }
     }
-    __syncthreads();
-    // write the tile: 3·TZ·NPAIR pair rows of 2·TX contiguous values
-    const int v = tid & 63;
-    for (int row = tid >> 6; row < 3 * TZ * NPAIR; row += 4) {
-        const int d = row / (TZ * NPAIR), zp = row - d * TZ * NPAIR;
-        const int z = zp / NPAIR, pp = zp - z * NPAIR;
-        const int64_t off = (((int64_t)(t0[2] + z) * (g.G[1] >> 1) + (t0[1] >> 1) + pp) * g.G[0] + t0[0]) * 2 + v;
-        grid[d * g.cstride + off] = acc[d][z][pp][v];
-    }
+    // write: per component, rows to LDS, then contiguous pair rows (2·TX values) to the grid
+    constexpr int VPR = 2 * TX;     // values per pair row
+    constexpr int RPI = 64 / VPR;   // pair rows per store instruction
+    static_for<0, 3>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        static_for<0, TX>([&](auto X) {
+            constexpr int x = decltype(X)::value;
+            buf[(lz * TY + ly) * TXP + x] = acc[x][c];
+        });
+        __syncthreads();
+        for (int it = 0; it < TZ * NPAIR / RPI; it++) {
+            const int row = it * RPI + lane / VPR;  // z·NPAIR + pair
+            const int v = lane % VPR;
+            const int z = row / NPAIR, pp = row - z * NPAIR;
+            const CT val = buf[(z * TY + 2 * pp + (v & 1)) * TXP + (v >> 1)];
+            grid[c * g.cstride + (((int64_t)(t0z + z) * (g.G[1] >> 1) + (t0y >> 1) + pp) * g.G[0] + t0x) * 2 + v] =
+                val;
+        }
+        __syncthreads();
+    });
 }
 
 template <typename CT>
 void launch_spread(const LRGeom &g, int64_t nb, int64_t Nq, const int32_t *tstart, const uint8_t *active,
-                   const int32_t *si0, const CT *sw, const CT *schg, CT *grid, hipStream_t s) {
+                   const int32_t *si0, const CT *rec, CT *grid, hipStream_t s) {
     const unsigned n = (unsigned)nb;
     switch (g.M) {
-        case 2: spread_scatter_kernel<CT, 4><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
-        case 3: spread_scatter_kernel<CT, 6><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
-        case 4: spread_scatter_kernel<CT, 8><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
-        case 5: spread_scatter_kernel<CT, 10><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
-        case 6: spread_scatter_kernel<CT, 12><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
-        case 7: spread_scatter_kernel<CT, 14><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
-        default: spread_scatter_kernel<CT, 16><<<n, 256, 0, s>>>(g, Nq, tstart, active, si0, sw, schg, grid); break;
+        case 2: spread_rows_kernel<CT, 4><<<n, 64, 0, s>>>(g, Nq, tstart, active, si0, rec, grid); break;
+        case 3: spread_rows_kernel<CT, 6><<<n, 64, 0, s>>>(g, Nq, tstart, active, si0, rec, grid); break;
+        case 4: spread_rows_kernel<CT, 8><<<n, 64, 0, s>>>(g, Nq, tstart, active, si0, rec, grid); break;
+        case 5: spread_rows_kernel<CT, 10><<<n, 64, 0, s>>>(g, Nq, tstart, active, si0, rec, grid); break;
+        case 6: spread_rows_kernel<CT, 12><<<n, 64, 0, s>>>(g, Nq, tstart, active, si0, rec, grid); break;
+        case 7: spread_rows_kernel<CT, 14><<<n, 64, 0, s>>>(g, Nq, tstart, active, si0, rec, grid); break;
+        default: spread_rows_kernel<CT, 16><<<n, 64, 0, s>>>(g, Nq, tstart, active, si0, rec, grid); break;
     }
 }
 
@@ -716,6 +723,7 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
         g.ntile[d] = c->ntile[d];
     }
     g.M = c->p.nufft_m;
+    g.tx = c->p.precision == VPBS_F32 ? tile_x<float>() : tile_x<double>();
     g.cstride = c->comp_stride;
     const CT beta = (CT)c->es_beta;
 
@@ -723,15 +731,14 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
     lr_tile_keys<<<nblocks(Nq, 256), 256, 0, s>>>(Nq, c->points, g, c->lr_key);
     VPBS_TRY(counting_sort(c, c->sort_lr, c->lr_key, Nq, c->ntiles, c->lr_start, c->lr_perm, s));
     gather_points<CT><<<nblocks(Nq, 256), 256, 0, s>>>(Nq, c->lr_perm, c->points, c->charges, g, beta, c->lr_i0,
-                                                        (CT *)c->lr_w, (CT *)c->lr_chg);
+                                                        (CT *)c->lr_w);
     stage_end(c, ST_LRSORT, s);
 
     CT *grid = (CT *)c->grid;
     stage_begin(c, ST_SPREAD, s);
     tile_active<<<nblocks(c->ntiles, 256), 256, 0, s>>>(g, c->ntiles, c->lr_start, c->lr_active);
     const int64_t nb_spread = ((c->ntiles + 7) / 8) * 8;  // xcd_tile() maps 8·⌈n/8⌉ blocks onto the tiles
-    launch_spread<CT>(g, nb_spread, Nq, c->lr_start, c->lr_active, c->lr_i0, (const CT *)c->lr_w,
-                      (const CT *)c->lr_chg, grid, s);
+    launch_spread<CT>(g, nb_spread, Nq, c->lr_start, c->lr_active, c->lr_i0, (const CT *)c->lr_w, grid, s);
     stage_end(c, ST_SPREAD, s);
     VPBS_HIP(c, hipGetLastError());
 
@@ -833,7 +840,8 @@ int longrange_setup(vpbs_ctx *c) {
     for (int d = 0; d < 3; d++) {
         c->G[d] = oversampled_size(c->p.Ns[d], c->p.nufft_sigma);
         if (c->G[d] < 4 * m) c->G[d] = ((4 * m + 31) / 32) * 32;
-        c->ntile[d] = c->G[d] / (d == 0 ? TX : (d == 1 ? TY : TZ));
+        const int tx = c->p.precision == VPBS_F32 ? tile_x<float>() : tile_x<double>();
+        c->ntile[d] = c->G[d] / (d == 0 ? tx : (d == 1 ? TY : TZ));
     }
     c->ntiles = (int64_t)c->ntile[0] * c->ntile[1] * c->ntile[2];
     c->comp_stride = (int64_t)c->G[0] * c->G[1] * c->G[2];

@@ -109,7 +109,7 @@ static void release(vpbs_ctx *c) {
                     c->points,    c->charges,   c->v_sr,     c->psi_sr,    c->v_lr,       c->psi_lr,
                     c->v_out,     c->psi_out,   c->src_key,  c->src_start, c->src_perm,   c->tgt_key,
                     c->tgt_start, c->tgt_perm,  c->src_pos,  c->src_chg,   c->src_idx,    c->stat_counts,
-                    c->stat_hashes, c->lr_key,  c->lr_start, c->lr_active, c->lr_perm,   c->lr_i0,      c->lr_w,    c->lr_chg,
+                    c->stat_hashes, c->lr_key,  c->lr_start, c->lr_active, c->lr_perm,   c->lr_i0,      c->lr_w,
                     c->sort_sr.rank, c->sort_sr.scan_tmp, c->sort_tg.rank, c->sort_tg.scan_tmp,
                     c->sort_lr.rank, c->sort_lr.scan_tmp, c->fil_x, c->fil_ts, c->fil_c, c->fil_d, c->fil_e,
                     c->fil_fid, c->fil_n, c->fil_off, c->fil_T, c->fil_xoff};
@@ -167,12 +167,11 @@ int ensure_capacity(vpbs_ctx *c, int64_t Np, int64_t Nq) {
         VPBS_TRY(alloc(c, &c->lr_perm, Nq));
         VPBS_TRY(alloc(c, &c->lr_i0, 3 * Nq));
         if (c->src_chg) VPBS_HIP(c, hipFree(c->src_chg));
-        if (c->lr_chg) VPBS_HIP(c, hipFree(c->lr_chg));
         if (c->lr_w) VPBS_HIP(c, hipFree(c->lr_w));
-        c->src_chg = c->lr_chg = c->lr_w = nullptr;
+        c->src_chg = c->lr_w = nullptr;
         VPBS_HIP(c, hipMalloc(&c->src_chg, cs * 3 * (size_t)Nq));
-        VPBS_HIP(c, hipMalloc(&c->lr_chg, cs * 3 * (size_t)Nq));
-        VPBS_HIP(c, hipMalloc(&c->lr_w, cs * 3 * 2 * (size_t)c->p.nufft_m * (size_t)Nq));
+        const size_t rec = (3 * 2 * (size_t)c->p.nufft_m + 3 + 3) & ~(size_t)3;  // rec_len(2m) in longrange.hip
+        VPBS_HIP(c, hipMalloc(&c->lr_w, cs * rec * (size_t)Nq));
         c->cap_nq = Nq;
     }
     return VPBS_OK;

@@ -106,8 +106,7 @@ struct vpbs_ctx {
     int32_t *lr_key = nullptr, *lr_start = nullptr, *lr_perm = nullptr;
     uint8_t *lr_active = nullptr;  // per tile: 1 if any neighbour tile holds a point
     int32_t *lr_i0 = nullptr;   // tile-sorted points: first stencil index per direction, 3*Nq
-    void *lr_w = nullptr;       // tile-sorted points: 2m kernel weights per direction, 3*2m*Nq (compute type)
-    void *lr_chg = nullptr;     // tile-sorted charges, 3*Nq (compute type)
+    void *lr_w = nullptr;       // tile-sorted points: one record per point (3·2m weights, charge; compute type)
     double *deconv = nullptr;   // 1/ψ̂ tables for retained kx, ky, kz (concatenated)
     int32_t kmax_mode[3] = {0, 0, 0};
     double es_beta = 0;
