@@ -101,6 +101,32 @@ def time_point_data(vp, params, pd, reps: int = 3):
             "bit_identical_to_host": bool(same)}
 
 
+def time_end_to_end(vp, params, pd, reps: int = 3):
+    """Wall time of one evaluation including the host side and PCIe, FP32 path.
+
+    device: upload of nodes, knots and spline coefficients (vpbs_set_filaments),
+    compute, copy of the velocities back; host: the reference's split, with
+    add_point_charges on the host, upload of the point data, compute, copy back.
+    The spline coefficients are solved once beforehand (filament state).
+    """
+    ctx = vp._lib.Context(params)
+    res = {}
+    for mode in ("device", "host"):
+        walls = []
+        for r in range(reps + 1):
+            t = time.perf_counter()
+            if mode == "device":
+                ctx.set_filaments(pd.batch, params)
+            else:
+                ctx.set_points(vp.add_point_charges(pd.batch.fs, params, batch=pd.batch))
+            v, _ = ctx.compute(vp._lib.VELOCITY, fetch=True)
+            if r:
+                walls.append(time.perf_counter() - t)
+        res[f"{mode}_point_data_ms"] = round(float(np.median(walls)) * 1e3, 2)
+    ctx.close()
+    return res
+
+
 def stage_bytes(stage: str, s: int, G, Ns, Np: int, Nq: int) -> float:
     """Algorithmic HBM bytes per launch of each long-range stage (velocity only).
 
@@ -231,6 +257,8 @@ def main():
 
     pdt = time_point_data(vp, p64, pd)
     log(f"point data: {json.dumps(pdt)}")
+    e2e = time_end_to_end(vp, p32, pd, reps=2)
+    log(f"end to end: {json.dumps(e2e)}")
 
     # roofline of the dominant long-range kernel of the FP32 path (our own kernels only)
     lr_kernels = ["spread", "fourier", "interp"]
@@ -276,6 +304,7 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in st32.items()},
         "stage_algorithmic_GBps": per_stage,
         "point_data": pdt,
+        "end_to_end_ms": e2e,
     }
     if res64 is not None:
         result["fp64_path"] = res64

@@ -47,6 +47,12 @@ extern "C" {
 #define VPBS_NO_SHORTRANGE 16u /* shortrange = false */
 #define VPBS_NO_LONGRANGE 32u  /* longrange = false */
 #define VPBS_NEIGHBOUR_STATS 64u /* record per-node accepted-pair counts and hashes */
+#define VPBS_KEEP_UHAT 128u      /* keep the unsmoothed long-range field in Fourier space (see vpbs_get_uhat) */
+
+/* quantity held by the kept Fourier field (LongRangeCacheState.quantity) */
+#define VPBS_UHAT_NONE 0
+#define VPBS_UHAT_VELOCITY 1
+#define VPBS_UHAT_STREAMFUNCTION 2
 
 /*
  * Parameters. Mirrors ParamsBiotSavart (src/BiotSavart/params.jl:326-376)
@@ -148,10 +154,17 @@ int vpbs_set_points_device(vpbs_ctx *ctx, int64_t Np, const double *nodes, const
  *                     both NULL for NoQuadrature (straight segments, nquad = 1)
  *   lia_segment_fraction  NaN for `nothing`, else γ ∈ (0, 1]: δ± then cover the
  *                     fraction γ of each segment (local_integrals.jl:100-101)
+ *   nquad_ns, quad_ns_nodes, quad_ns_weights  quadrature_near_singularity rule on [0, 1]
+ *                     (≤ 16 points). With lia_segment_fraction set and nquad_ns > 0,
+ *                     vpbs_compute also adds the near-singular local integrals of
+ *                     add_local_integrals! (local_integrals.jl:209-305) on the device,
+ *                     in FP64, whenever the short range runs without VPBS_LIA_ONLY
+ *                     (BiotSavart.jl:570-576); otherwise they stay with the caller.
  */
 int vpbs_set_filaments(vpbs_ctx *ctx, int64_t Nf, const int64_t *counts, const double *X, const double *knots,
                        const double *periods, const double *offsets, const double *coefs, const double *quad_nodes,
-                       const double *quad_weights, double lia_segment_fraction);
+                       const double *quad_weights, double lia_segment_fraction, int32_t nquad_ns,
+                       const double *quad_ns_nodes, const double *quad_ns_weights);
 
 /* Copy the device point data (from vpbs_set_points* or vpbs_set_filaments) to host arrays; any may be NULL. */
 int vpbs_get_points(vpbs_ctx *ctx, double *nodes, int32_t *idx_prev, double *sp, double *spp, double *dlm,
@@ -161,8 +174,9 @@ int vpbs_get_points(vpbs_ctx *ctx, double *nodes, int32_t *idx_prev, double *sp,
  * Compute velocity and/or streamfunction on nodes: the device part of
  * _compute_on_nodes! (BiotSavart.jl:515-612) — LIA term, short-range pair sum,
  * self-interaction removal and long-range NUFFT path, summed. The host-side
- * background-vorticity correction of ψ (BiotSavart.jl:255-286) and
- * lia_segment_fraction integrals stay with the caller.
+ * background-vorticity correction of ψ (BiotSavart.jl:255-286) stays with the
+ * caller, and so do the lia_segment_fraction integrals unless the point data
+ * came from vpbs_set_filaments with a near-singularity rule.
  * integration_cutoff: NaN for the default exp(Δ)a/2 / exp(Δ-1)a/2 cut-offs
  * (local_integrals.jl:115-125). v_out / psi_out: host arrays of 3*Np doubles,
  * or NULL to leave the result on the device (see vpbs_get_outputs).
@@ -182,6 +196,34 @@ int vpbs_output_device_ptrs(vpbs_ctx *ctx, const double **v_dev, const double **
  * Bit-exact comparison target for the cell-list neighbour sets.
  */
 int vpbs_neighbour_stats(vpbs_ctx *ctx, int64_t *counts, uint64_t *hashes);
+
+/*
+ * Long-range field in Fourier space kept by the last vpbs_compute run with
+ * VPBS_KEEP_UHAT: replaces get_longrange_field_fourier (cache.jl:77-83).
+ * Like the reference's cache after compute_on_nodes!, it holds the unsmoothed
+ * velocity v̂ = i k × ω̂/k² (or the streamfunction ψ̂ = ω̂/k² when only ψ was
+ * computed; longrange.jl:102-224), without the Ewald factor. quantity:
+ * VPBS_UHAT_*; dims3 = (N0/2+1, N1, N2); out (may be NULL): 3 components, each
+ * dims3 complex values as (re, im) FP64 pairs with kx fastest (the reference's
+ * uhat layout); modes outside the retained set are zero.
+ */
+int vpbs_get_uhat(vpbs_ctx *ctx, int32_t *quantity, int64_t *dims3, double *out);
+
+/*
+ * Energy spectrum of the kept field on the device: Diagnostics.energy_spectrum!
+ * (Diagnostics/spectra.jl:89-196). Bins k_n = n·dk, n < nk ≤ 8192; mode k adds
+ * f·|v̂|²/dk to bin trunc(|k|/dk + 1/2), f = 1/2 on the kx = 0 plane, else 1.
+ */
+int vpbs_energy_spectrum(vpbs_ctx *ctx, int64_t nk, double dk, double *Ek);
+
+/*
+ * Kept field restricted to kmin² ≤ k² ≤ kmax², interpolated at the current
+ * nodes (type-2 NUFFT): the band velocity behind Forcing.FourierBandForcingBS
+ * (fourier_band_bs.jl; band as SyntheticFields.set_wavevector_band!) and, with
+ * kmin = 0, the low-pass velocity of SmallScaleDissipationBS
+ * (dissipation_bs_small_scales.jl:109-128). v_out: 3*Np doubles (host).
+ */
+int vpbs_interpolate_band(vpbs_ctx *ctx, double kmin, double kmax, double *v_out);
 
 /* Per-stage timings of the last vpbs_compute. */
 int vpbs_get_timings(vpbs_ctx *ctx, vpbs_timings *t);

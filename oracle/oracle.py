@@ -227,6 +227,137 @@ def longrange(pd, params, quantities, method: str = "nufft", workers: int = 8) -
     return out
 
 
+def _wavenumbers(Ns, Ls):
+    """Reference wavenumber vectors (kx: r2c half axis; ky, kz: FFT order), cache_common.jl."""
+    kx = 2 * np.pi / Ls[0] * np.arange(Ns[0] // 2 + 1)
+    ky = 2 * np.pi / Ls[1] * np.where(np.arange(Ns[1]) < (Ns[1] + 1) // 2, np.arange(Ns[1]), np.arange(Ns[1]) - Ns[1])
+    kz = 2 * np.pi / Ls[2] * np.where(np.arange(Ns[2]) < (Ns[2] + 1) // 2, np.arange(Ns[2]), np.arange(Ns[2]) - Ns[2])
+    return kx, ky, kz
+
+
+def longrange_fourier(pd, params, quantity: str = "velocity", method: str = "nufft", workers: int = 8) -> np.ndarray:
+    """Unsmoothed long-range field in Fourier space, layout (3, N2, N1, N0//2 + 1) complex.
+
+    What ``get_longrange_field_fourier`` (cache.jl:77-83) holds after
+    ``compute_on_nodes!``: ω̂ = (Γ/V) Σ q e^{-ik·s} (type-1 NUFFT, longrange.jl:488-495),
+    then ψ̂ = ω̂/k² (longrange.jl:116-124) or v̂ = i k × ω̂/k² (longrange.jl:128-142),
+    without the Ewald factor (applied only when interpolating, cache_common.jl:64-76).
+    Modes outside the retained set (longrange.jl:556-583) are zero. ``quantity``:
+    "vorticity", "streamfunction" or "velocity".
+    """
+    L = lib()
+    Ls = np.array(params.Ls, dtype=np.float64)
+    Ns = np.array(params.Ns, dtype=np.int32)
+    pref = params.Γ / float(np.prod(Ls))
+    kxmax, kym, kzm = mode_limits(params.Ns)
+    kx = np.arange(0, kxmax + 1)
+    ky = np.arange(-kym, kym + 1)
+    kz = np.arange(-kzm, kzm + 1)
+    pts = _c(pd.points)
+    chg = _c(pd.charges)
+    Nq = pts.shape[1]
+    Nkx = Ns[0] // 2 + 1
+    iy = ky % Ns[1]
+    iz = kz % Ns[2]
+    if method == "exact":
+        uh = np.zeros((3, Ns[2], Ns[1], Nkx, 2))
+        L.or_exact_type1(Nq, _p(pts), _p(chg), _p(Ls), _p(Ns, _ip), pref, _p(uh))
+        uh = uh[..., 0] + 1j * uh[..., 1]
+        w = uh[:, iz][:, :, iy][:, :, :, kx]
+    else:
+        m = params.backend_long.m
+        sigma = params.backend_long.σ
+        beta = es_beta(m, sigma)
+        G = np.array([oversampled_size(int(n), sigma, m) for n in Ns], dtype=np.int32)
+        grid = np.zeros((3, G[2], G[1], G[0]))
+        L.or_spread(Nq, _p(pts), _p(chg), _p(Ls), _p(G, _ip), m, beta, _p(grid))
+        spec = scipy.fft.rfftn(grid, axes=(1, 2, 3), workers=workers)
+        del grid
+        D = 1.0 / (es_fourier(kz, int(G[2]), m, beta)[:, None, None] * es_fourier(ky, int(G[1]), m, beta)[None, :, None]
+                   * es_fourier(kx, int(G[0]), m, beta)[None, None, :])
+        w = spec[:, kz % G[2]][:, :, ky % G[1]][:, :, :, kx] * (pref * D)[None]
+        del spec
+    if quantity == "vorticity":
+        u = w
+    else:
+        KX = 2 * np.pi * kx[None, None, :] / Ls[0]
+        KY = 2 * np.pi * ky[None, :, None] / Ls[1]
+        KZ = 2 * np.pi * kz[:, None, None] / Ls[2]
+        k2 = KX * KX + KY * KY + KZ * KZ
+        scal = np.where(k2 == 0, 0.0, 1.0 / np.where(k2 == 0, 1.0, k2))
+        if params.longrange_truncate_spherical:
+            scal = np.where(k2 > params.maximum_wavenumber() ** 2, 0.0, scal)
+        u = _apply_operator(w, scal, (KX, KY, KZ), quantity)
+    full = np.zeros((3, Ns[2], Ns[1], Nkx), dtype=np.complex128)
+    full[np.ix_(range(3), iz, iy, kx)] = u
+    return full
+
+
+def energy_spectrum(uhat: np.ndarray, Ls, nk: int, dk: float, quantity: str = "velocity") -> np.ndarray:
+    """Energy spectrum from a half-spectrum field (Diagnostics/spectra.jl:89-196).
+
+    Bin n = trunc(|k|/Δk + 1/2) (0-based), E[n] += f·|û|²/Δk with f = 1/2 on the
+    kx = 0 plane and 1 elsewhere (Hermitian symmetry); from vorticity |ω̂|²/k².
+    """
+    Ns = (2 * (uhat.shape[3] - 1), uhat.shape[2], uhat.shape[1])
+    kx, ky, kz = _wavenumbers(Ns, np.asarray(Ls, dtype=np.float64))
+    k2 = kx[None, None, :] ** 2 + ky[None, :, None] ** 2 + kz[:, None, None] ** 2
+    u2 = np.sum(np.abs(uhat) ** 2, axis=0)
+    if quantity == "vorticity":
+        u2 = u2 * np.where(k2 == 0, 1.0, 1.0 / np.where(k2 == 0, 1.0, k2))
+    elif quantity != "velocity":
+        raise ValueError("the energy spectrum needs the velocity or the vorticity")
+    factor = np.where(kx == 0, 0.5, 1.0)[None, None, :]
+    n = np.trunc(np.sqrt(k2) / dk + 0.5).astype(np.int64)
+    E = np.zeros(nk)
+    sel = n < nk
+    np.add.at(E, n[sel], (factor * u2 / dk)[sel])
+    return E
+
+
+def band_field(uhat: np.ndarray, params, nodes: np.ndarray, kmin: float, kmax: float,
+               method: str = "nufft", workers: int = 8) -> np.ndarray:
+    """Σ over kmin² ≤ k² ≤ kmax² of û(k) e^{ik·x} at the nodes (the full Hermitian sum).
+
+    The field behind FourierBandForcingBS (fourier_band_bs.jl; band selection as
+    SyntheticFields.set_wavevector_band!, fourier.jl:241-266) and, with kmin = 0,
+    the low-pass velocity of SmallScaleDissipationBS
+    (dissipation_bs_small_scales.jl:109-128). ``method``: "exact" (direct sum) or
+    "nufft" (deconvolution, inverse FFT, ES interpolation: the library's algorithm).
+    """
+    L = lib()
+    Ls = np.array(params.Ls, dtype=np.float64)
+    Ns = np.array(params.Ns, dtype=np.int32)
+    kx, ky, kz = _wavenumbers(Ns, Ls)
+    k2 = kx[None, None, :] ** 2 + ky[None, :, None] ** 2 + kz[:, None, None] ** 2
+    mask = (k2 >= kmin * kmin) & (k2 <= kmax * kmax)
+    u = uhat * mask[None]
+    nodes = _c(nodes)
+    Np = nodes.shape[1]
+    res = np.zeros((3, Np))
+    if method == "exact":
+        buf = np.stack([u.real, u.imag], axis=-1).copy()
+        L.or_exact_type2(Np, _p(nodes), _p(Ls), _p(Ns, _ip), _p(buf), _p(res))
+        return res
+    m = params.backend_long.m
+    sigma = params.backend_long.σ
+    beta = es_beta(m, sigma)
+    G = np.array([oversampled_size(int(n), sigma, m) for n in Ns], dtype=np.int32)
+    kxmax, kym, kzm = mode_limits(params.Ns)
+    qx = np.arange(0, kxmax + 1)
+    qy = np.arange(-kym, kym + 1)
+    qz = np.arange(-kzm, kzm + 1)
+    D = 1.0 / (es_fourier(qz, int(G[2]), m, beta)[:, None, None] * es_fourier(qy, int(G[1]), m, beta)[None, :, None]
+               * es_fourier(qx, int(G[0]), m, beta)[None, None, :])
+    full = np.zeros((3, G[2], G[1], G[0] // 2 + 1), dtype=np.complex128)
+    full[np.ix_(range(3), qz % G[2], qy % G[1], qx)] = u[:, qz % Ns[2]][:, :, qy % Ns[1]][:, :, :, qx] * D[None]
+    g2 = scipy.fft.irfftn(full, s=(int(G[2]), int(G[1]), int(G[0])), axes=(1, 2, 3), norm="forward", workers=workers)
+    del full
+    g2 = np.ascontiguousarray(g2)
+    L.or_interp(Np, _p(nodes), _p(Ls), _p(G, _ip), m, beta, _p(g2), _p(res))
+    return res
+
+
 def local_cutoffs(params, integration_cutoff=None):
     if integration_cutoff is None:
         rpsi = math.exp(params.Δ - 1) * params.a / 2  # local_integrals.jl:118-119

@@ -225,9 +225,23 @@ def test_tiny_grid_few_tiles(vp, O):
 
 
 def test_lia_segment_fraction(vp, O, cfg2):
-    """lia_segment_fraction = 0.2 with GL(8) near the singularity (host-side local integrals on both sides)."""
+    """lia_segment_fraction = 0.2 with GL(8) near the singularity: local integrals on the device (device point
+    data) against the oracle's host restatement."""
     params = _params(vp, Ns=128, lia_segment_fraction=0.2, quadrature_near_singularity=vp.GaussLegendre(8))
     _check(vp, O, params, cfg2)
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(LIA=False), dict(LIA="only"), dict(longrange=False)])
+def test_local_integrals_device_vs_host(vp, cfg2, variant):
+    """Device local integrals (device point data) equal the host ones (host point data): FP64 1e-13,
+    for every LIA setting (BiotSavart.jl:570-576 adds them unless LIA = Val(:only))."""
+    params = _params(vp, Ns=128, lia_segment_fraction=0.3, quadrature_near_singularity=vp.GaussLegendre(5))
+    _, got_dev = _gpu(vp, params, cfg2, pointdata="device", **variant)
+    _, got_host = _gpu(vp, params, cfg2, pointdata="host", **variant)
+    for k in Q:
+        err = rel_l2(got_dev[k], got_host[k])
+        print(f"  {variant} {k}: device vs host local integrals {err:.2e}", flush=True)
+        assert err < 1e-13
 
 
 # --------------------------------------------------------------------------- #
@@ -328,7 +342,7 @@ def test_device_pointdata_errors(vp, cfg1):
     def call(counts, X=X, T=b.T, zs=zs, ws=ws, g=math.nan):
         counts = np.asarray(counts, dtype=np.int64)
         return L.vpbs_set_filaments(ctx._h, len(counts), P(counts), P(X), P(b.ts), P(np.asarray(T)), P(xoff),
-                                    P(c), P(zs), P(ws), g)
+                                    P(c), P(zs), P(ws), g, 0, None, None)
 
     def msg():
         return L.vpbs_last_error(ctx._h).decode()
@@ -337,7 +351,10 @@ def test_device_pointdata_errors(vp, cfg1):
     assert call(b.Ns, T=np.array([0.0])) == -1 and "period" in msg()
     assert call(b.Ns, zs=None) == -1 and "both" in msg()
     assert call(b.Ns, g=1.5) == -1 and "lia_segment_fraction" in msg()
-    assert L.vpbs_set_filaments(ctx._h, 0, None, None, None, None, None, None, None, None, math.nan) == -1
+    assert L.vpbs_set_filaments(ctx._h, 0, None, None, None, None, None, None, None, None, math.nan, 0, None,
+                                None) == -1
+    assert L.vpbs_set_filaments(ctx._h, 1, P(np.asarray(b.Ns, dtype=np.int64)), P(X), P(b.ts), P(b.T), P(xoff),
+                                P(c), P(zs), P(ws), 0.3, 17, P(zs), P(ws)) == -1 and "near-singularity" in msg()
     Xbad = X.copy()
     Xbad[1, 3] = np.nan
     assert call(b.Ns, X=Xbad) == -1 and "not folded" in msg()
@@ -352,3 +369,133 @@ def test_device_pointdata_errors(vp, cfg1):
     with pytest.raises(vp._lib.VpbsError, match="nquad = 1"):
         ctx.set_filaments(b, _params(vp, quadrature=vp.NoQuadrature()))
     ctx.close()
+
+
+# --------------------------------------------------------------------------- #
+# Long-range field in Fourier space and its device-side consumers
+
+
+def _uhat_case(vp, fs, Ns=64, T=np.float64, **kws):
+    params = _params(vp, T=T, Ns=Ns, **kws)
+    cache = vp.init_cache(params)  # keep_fourier=True by default
+    vs = [np.zeros((len(f), 3)) for f in fs]
+    vp.compute_on_nodes_({"velocity": vs}, cache, fs)
+    return params, cache, vs
+
+
+@pytest.mark.parametrize("variant", ["cfg1", "cfg2", "truncate_spherical", "trefoil_noncubic"])
+def test_uhat_matches_oracle(vp, O, cfg1, cfg2, variant):
+    """Kept unsmoothed velocity v̂ = i k × ω̂/k² on the Fourier grid vs the oracle (FP64, 1e-12)."""
+    if variant == "trefoil_noncubic":
+        R = math.pi / 3
+        S = vp.define_curve(vp.TrefoilKnot(), scale=R, translate=R)
+        fs = [vp.ClosedFilament(S(np.arange(30) / 30), np.zeros(3))]
+        spl = vp.GaussianSplitting.create(Ls=(1.5 * math.pi, 1.5 * math.pi, 2 * math.pi), α=3.0, rcut=1.0,
+                                         Ns=(45, 45, 61))
+        params = vp.ParamsBiotSavart.build(np.float64, Γ=7.4, a=1e-5, splitting=spl)
+        cache = vp.init_cache(params)
+        vp.compute_on_nodes_({"velocity": [np.zeros((30, 3))]}, cache, fs)
+    else:
+        fs = cfg2 if variant == "cfg2" else cfg1
+        kw = {"longrange_truncate_spherical": True} if variant == "truncate_spherical" else {}
+        params, cache, _ = _uhat_case(vp, fs, Ns=128 if variant == "cfg2" else 64, **kw)
+    got = vp.get_longrange_field_fourier(cache)
+    assert got["state"] == "velocity"
+    want = O.longrange_fourier(vp.add_point_charges(fs, params), params, "velocity")
+    err = rel_l2(np.stack(got["field"]), want)
+    print(f"  {variant}: uhat rel L2 vs oracle = {err:.3e}", flush=True)
+    assert err <= TOL64
+    cache.close()
+
+
+def test_uhat_streamfunction_state(vp, O, cfg1):
+    """Streamfunction only: the kept field is ψ̂ = ω̂/k² (longrange.jl:102-113)."""
+    params = _params(vp)
+    cache = vp.init_cache(params)
+    vp.compute_on_nodes_({"streamfunction": [np.zeros((len(f), 3)) for f in cfg1]}, cache, cfg1)
+    got = vp.get_longrange_field_fourier(cache)
+    assert got["state"] == "streamfunction"
+    want = O.longrange_fourier(vp.add_point_charges(cfg1, params), params, "streamfunction")
+    assert rel_l2(np.stack(got["field"]), want) <= TOL64
+    with pytest.raises(RuntimeError, match="not supported"):
+        vp.diagnostics.energy_spectrum(cache)
+    # an evaluation without the long range leaves no field
+    vp.compute_on_nodes_({"velocity": [np.zeros((len(f), 3)) for f in cfg1]}, cache, cfg1, longrange=False)
+    assert cache.ctx.uhat_quantity() is None
+    with pytest.raises(RuntimeError):
+        vp.get_longrange_field_fourier(cache)
+    cache.close()
+
+
+def test_energy_spectrum_device(vp, O, cfg2):
+    """Device energy spectrum vs the oracle's binning of the oracle field (FP64, 1e-12); a finer Δk too."""
+    params, cache, _ = _uhat_case(vp, cfg2, Ns=128)
+    ks, Ek = vp.diagnostics.energy_spectrum(cache)
+    assert len(ks) == 65 and ks[0] == 0
+    u = O.longrange_fourier(vp.add_point_charges(cfg2, params), params, "velocity")
+    want = O.energy_spectrum(u, params.Ls, len(ks), ks[1] - ks[0])
+    err = rel_l2(Ek, want)
+    print(f"  spectrum rel L2 vs oracle = {err:.3e}; E(k=1..4) = {Ek[1:5]}", flush=True)
+    assert err <= TOL64
+    Ek2 = cache.ctx.energy_spectrum(300, 0.37)
+    assert rel_l2(Ek2, O.energy_spectrum(u, params.Ls, 300, 0.37)) <= TOL64
+    cache.close()
+
+
+def test_band_interpolation(vp, O, cfg2):
+    """Band-filtered field at the nodes: vs the oracle's NUFFT restatement (1e-12) and the direct sum (NUFFT tol.)."""
+    params, cache, _ = _uhat_case(vp, cfg2, Ns=128)
+    pd = vp.add_point_charges(cfg2, params)
+    u = O.longrange_fourier(pd, params, "velocity")
+    full = O.band_field(u, params, pd.nodes, 0.0, 1e3, "exact")
+    for kmin, kmax in ((0.0, 4.0), (2.0, 6.0), (10.0, 30.0)):
+        got = cache.ctx.interpolate_band(kmin, kmax)
+        err = rel_l2(got, O.band_field(u, params, pd.nodes, kmin, kmax, "nufft"))
+        errx = np.linalg.norm(got - O.band_field(u, params, pd.nodes, kmin, kmax, "exact")) / np.linalg.norm(full)
+        print(f"  band [{kmin}, {kmax}]: vs oracle NUFFT {err:.2e}, vs direct sum {errx:.2e}", flush=True)
+        assert err <= TOL64 and errx < 1e-6
+    cache.close()
+
+
+def test_uhat_fp32(vp, O, cfg2):
+    """FP32 path: kept field and spectrum against the FP64 oracle within FP32 tolerance."""
+    params, cache, _ = _uhat_case(vp, cfg2, Ns=128, T=np.float32)
+    p64 = _params(vp, Ns=128)
+    u = O.longrange_fourier(vp.add_point_charges(cfg2, p64), p64, "velocity")
+    err = rel_l2(np.stack(vp.get_longrange_field_fourier(cache)["field"]), u)
+    ks, Ek = vp.diagnostics.energy_spectrum(cache)
+    errE = rel_l2(Ek, O.energy_spectrum(u, p64.Ls, len(ks), ks[1] - ks[0]))
+    print(f"  fp32 uhat {err:.2e}, spectrum {errE:.2e}", flush=True)
+    assert err < 2e-6 and errE < 2e-6
+    cache.close()
+
+
+def test_forcing_and_dissipation(vp, cfg2):
+    """FourierBandForcingBS / SmallScaleDissipationBS on the device band field: invariants of the formulas."""
+    params, cache, vs = _uhat_case(vp, cfg2, Ns=128)
+    b = vp.FilamentBatch(cfg2)
+    st = b.evaluate(b.fid, b.local, np.zeros(b.Np), 1)
+    stt = b.evaluate(b.fid, b.local, np.zeros(b.Np), 2)
+    sp = st / np.linalg.norm(st, axis=1)[:, None]
+    F = vp.forcing
+    vf = np.concatenate(F.forcing_velocity(F.FourierBandForcingBS(kmin=1.5, kmax=4.5, α=0.3), cache, cfg2))
+    assert np.abs(np.sum(vf * sp, axis=1)).max() < 1e-12 * np.abs(vf).max()  # ⟂ tangent
+    # α term: vf = α s′ × (band/2)
+    band = cache.ctx.interpolate_band(1.5, 4.5).T
+    assert rel_l2(vf, 0.3 * np.cross(sp, 0.5 * band)) < 1e-14
+    # ε_target mode: rescaled so that the estimated injection rate is ε_target
+    eps = 2e-3
+    vt = np.concatenate(F.forcing_velocity(F.FourierBandForcingBS(kmin=1.5, kmax=4.5, ε_target=eps), cache, cfg2))
+    assert rel_l2(vt, vf * np.sum(vt * vf) / np.sum(vf * vf)) < 1e-12  # parallel to the α forcing
+    # modify_length = false: along the binormal s′ × s″
+    vb = np.concatenate(F.forcing_velocity(F.FourierBandForcingBS(kmin=1.5, kmax=4.5, α=1.0, modify_length=False),
+                                           cache, cfg2))
+    bn = np.cross(sp, np.cross(sp, np.cross(stt, sp)))
+    assert np.linalg.norm(np.cross(vb, bn)) < 1e-10 * np.linalg.norm(vb) * np.linalg.norm(bn)
+    # dissipation: -s′ × (v - v<), with v< the low-pass velocity
+    vd = np.concatenate(F.dissipation_velocity(F.SmallScaleDissipationBS(kdiss=20.0, α=0.5), cache, cfg2, vs))
+    vlow = cache.ctx.interpolate_band(0.0, 20.0).T
+    assert rel_l2(vd, 0.5 * np.cross(np.concatenate(vs) - vlow, sp)) < 1e-14
+    with pytest.raises(ValueError, match="too large"):
+        F.dissipation_velocity(F.SmallScaleDissipationBS(kdiss=70.0, α=0.5), cache, cfg2, vs)
+    cache.close()

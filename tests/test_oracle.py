@@ -276,3 +276,113 @@ def test_lia_segment_fraction_convergence(vp, O):
     assert np.std(vz) / np.mean(vz) < 1e-12
     U = Γ / (4 * math.pi * R) * (math.log(8 * R / a) - Δ)
     assert abs(np.mean(vz) - U) / U < 2e-6
+
+
+# --------------------------------------------------------------------------- #
+# Long-range field in Fourier space and its consumers (uhat, energy spectrum, band field)
+
+
+def _ring_cfg1(vp):
+    fs = vp.tangles.single_ring()
+    spl = vp.GaussianSplitting.create(Ls=TWO_PI, β=3.5, Ns=64)
+    p = vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, splitting=spl)
+    return fs, p, vp.add_point_charges(fs, p)
+
+
+@pytest.mark.parametrize("quantity", ["vorticity", "streamfunction", "velocity"])
+def test_uhat_nufft_vs_exact_sum(vp, O, quantity):
+    """Unsmoothed Fourier field: NUFFT type 1 against the exact sum (NUFFT tolerance, trefoil.jl:402-406)."""
+    fs, p, pd = _ring_cfg1(vp)
+    u = O.longrange_fourier(pd, p, quantity)
+    ue = O.longrange_fourier(pd, p, quantity, method="exact")
+    assert rel_l2(u, ue) < 2e-6
+    # Nyquist planes and the k = 0 mode are zero (longrange.jl:556-583; k = 0 dropped by 1/k²)
+    assert np.all(u[:, 32] == 0) and np.all(u[:, :, 32] == 0) and np.all(u[..., 32] == 0)
+    if quantity != "vorticity":
+        assert np.all(u[:, 0, 0, 0] == 0)
+
+
+def test_uhat_velocity_is_curl_of_streamfunction(vp, O):
+    """v̂ = i k × ψ̂ (longrange.jl:160-172), checked against numpy gradients of the real fields."""
+    fs, p, pd = _ring_cfg1(vp)
+    uv = O.longrange_fourier(pd, p, "velocity")
+    up = O.longrange_fourier(pd, p, "streamfunction")
+    g = 64
+    psi = np.fft.irfftn(up, s=(g, g, g), axes=(1, 2, 3), norm="forward")  # [c][z][y][x]
+    v = np.fft.irfftn(uv, s=(g, g, g), axes=(1, 2, 3), norm="forward")
+    # spectral derivative of psi along each axis, then curl
+    kx, ky, kz = O._wavenumbers((g, g, g), np.array([TWO_PI] * 3))
+    dpsi = {}
+    for ax, k in ((3, kx[None, None, :]), (2, ky[None, :, None]), (1, kz[:, None, None])):
+        dpsi[ax] = np.fft.irfftn(1j * k[None] * up, s=(g, g, g), axes=(1, 2, 3), norm="forward")
+    dx, dy, dz = dpsi[3], dpsi[2], dpsi[1]
+    curl = np.stack([dy[2] - dz[1], dz[0] - dx[2], dx[1] - dy[0]])
+    assert rel_l2(v, curl) < 1e-12
+
+
+def test_energy_spectrum_parseval(vp, O):
+    """Σ E(k) Δk equals the mean kinetic energy ½⟨|v|²⟩ of the real field (numpy irfftn), which pins the
+    Hermitian factor 1/2 on the kx = 0 plane and the 1/Δk normalisation (spectra.jl:151-196)."""
+    fs, p, pd = _ring_cfg1(vp)
+    u = O.longrange_fourier(pd, p, "velocity")
+    E = O.energy_spectrum(u, p.Ls, 60, 1.0)  # 60 bins reach past |k| = √3·31
+    g = np.fft.irfftn(u, s=(64, 64, 64), axes=(1, 2, 3), norm="forward")
+    assert abs(E.sum() - 0.5 * np.mean(np.sum(g ** 2, axis=0))) < 1e-12 * E.sum()
+    # same spectrum from the vorticity (|ω̂|²/k²)
+    w = O.longrange_fourier(pd, p, "vorticity")
+    Ew = O.energy_spectrum(w, p.Ls, 60, 1.0, quantity="vorticity")
+    assert rel_l2(Ew, E) < 1e-12
+
+
+def test_energy_spectrum_single_mode(vp, O):
+    """One planted mode k = (3, -2, 1) (and one at kx = 0): energy in bin round(|k|/Δk) only."""
+    u = np.zeros((3, 32, 32, 17), dtype=complex)
+    u[:, 1, -2 % 32, 3] = [0.5 + 0.25j, -0.1j, 0.3]
+    E = O.energy_spectrum(u, (TWO_PI,) * 3, 20, 1.0)
+    n = int(round(math.sqrt(14)))
+    assert np.count_nonzero(E) == 1 and E[n] == pytest.approx(np.sum(np.abs(u) ** 2))
+    u0 = np.zeros_like(u)
+    u0[:, 2, 1, 0] = [1.0, 0, 0]
+    u0[:, -2 % 32, -1 % 32, 0] = [1.0, 0, 0]  # Hermitian partner
+    E0 = O.energy_spectrum(u0, (TWO_PI,) * 3, 20, 1.0)
+    assert E0[2] == pytest.approx(1.0) and np.count_nonzero(E0) == 1  # ½·(1 + 1)
+
+
+def test_planted_mistake_spectrum_factor(vp, O, monkeypatch):
+    """Dropping the Hermitian factor on the kx = 0 plane breaks the Parseval check."""
+    fs, p, pd = _ring_cfg1(vp)
+    u = O.longrange_fourier(pd, p, "velocity")
+    orig = np.where
+    calls = {"n": 0}
+
+    def where_no_half(cond, a, b):
+        calls["n"] += 1
+        if isinstance(a, float) and a == 0.5 and b == 1.0:
+            return orig(cond, 1.0, 1.0)
+        return orig(cond, a, b)
+
+    monkeypatch.setattr(O.np, "where", where_no_half)
+    E = O.energy_spectrum(u, p.Ls, 60, 1.0)
+    monkeypatch.undo()
+    g = np.fft.irfftn(u, s=(64, 64, 64), axes=(1, 2, 3), norm="forward")
+    assert abs(E.sum() - 0.5 * np.mean(np.sum(g ** 2, axis=0))) > 1e-3 * E.sum()
+
+
+def test_band_field(vp, O):
+    """Band-filtered field at the nodes: NUFFT against the direct sum, and a planted mode against 2·Re(c e^{ik·x})."""
+    fs, p, pd = _ring_cfg1(vp)
+    u = O.longrange_fourier(pd, p, "velocity")
+    full = O.band_field(u, p, pd.nodes, 0.0, 100.0, "exact")
+    for kmin, kmax in ((2.0, 6.0), (0.0, 9.5), (20.0, 31.0)):
+        be = O.band_field(u, p, pd.nodes, kmin, kmax, "exact")
+        bn = O.band_field(u, p, pd.nodes, kmin, kmax, "nufft")
+        # the type-2 NUFFT error is set by the kernel, relative to the whole field (~3e-7 at m = 4, σ = 1.5)
+        assert np.linalg.norm(bn - be) < 1e-6 * np.linalg.norm(full), (kmin, kmax)
+    c = np.array([0.3 - 0.2j, 0.1j, -0.05])
+    u1 = np.zeros_like(u)
+    u1[:, 1, 62, 3] = c  # k = (3, -2, 1)
+    x = pd.nodes
+    ph = 3 * x[0] - 2 * x[1] + 1 * x[2]
+    want = 2 * np.real(c[:, None] * np.exp(1j * ph)[None])
+    assert rel_l2(O.band_field(u1, p, x, 3.0, 4.0, "exact"), want) < 1e-13
+    assert np.all(O.band_field(u1, p, x, 4.0, 9.0, "exact") == 0)  # |k| = √14 < 4 is outside

@@ -25,6 +25,9 @@ LIA_ONLY = 8
 NO_SHORTRANGE = 16
 NO_LONGRANGE = 32
 NEIGHBOUR_STATS = 64
+KEEP_UHAT = 128
+UHAT_VELOCITY = 1
+UHAT_STREAMFUNCTION = 2
 
 EXPORTED = (
     "vpbs_create",
@@ -34,6 +37,9 @@ EXPORTED = (
     "vpbs_set_points_device",
     "vpbs_set_filaments",
     "vpbs_get_points",
+    "vpbs_get_uhat",
+    "vpbs_energy_spectrum",
+    "vpbs_interpolate_band",
     "vpbs_compute",
     "vpbs_get_outputs",
     "vpbs_output_device_ptrs",
@@ -122,10 +128,16 @@ def load() -> ctypes.CDLL:
     L.vpbs_set_points.restype = ctypes.c_int
     L.vpbs_set_points_device.argtypes = pts_sig
     L.vpbs_set_points_device.restype = ctypes.c_int
-    L.vpbs_set_filaments.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, d]
+    L.vpbs_set_filaments.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, d, ctypes.c_int32, vp, vp]
     L.vpbs_set_filaments.restype = ctypes.c_int
     L.vpbs_get_points.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.vpbs_get_points.restype = ctypes.c_int
+    L.vpbs_get_uhat.argtypes = [vp, vp, vp, vp]
+    L.vpbs_get_uhat.restype = ctypes.c_int
+    L.vpbs_energy_spectrum.argtypes = [vp, i64, d, vp]
+    L.vpbs_energy_spectrum.restype = ctypes.c_int
+    L.vpbs_interpolate_band.argtypes = [vp, d, d, vp]
+    L.vpbs_interpolate_band.restype = ctypes.c_int
     L.vpbs_compute.argtypes = [vp, u32, d, vp, vp]
     L.vpbs_compute.restype = ctypes.c_int
     L.vpbs_get_outputs.argtypes = [vp, vp, vp]
@@ -222,7 +234,9 @@ class Context:
 
         Takes the batch's nodes, knots, periods, end-to-end offsets and spline
         coefficients; the quadrature rule and ``lia_segment_fraction`` come from
-        ``params`` (NoQuadrature → straight segments).
+        ``params`` (NoQuadrature → straight segments). With
+        ``lia_segment_fraction`` set, the near-singular local integrals are then
+        evaluated on the device by ``compute`` (``quadrature_near_singularity``).
         """
         from .params import NoQuadrature
 
@@ -238,9 +252,12 @@ class Context:
         else:
             zs, ws = (np.ascontiguousarray(a, dtype=np.float64) for a in quad.rule())
         g = params.lia_segment_fraction
+        zn = wn = None
+        if g is not None:
+            zn, wn = (np.ascontiguousarray(a, dtype=np.float64) for a in params.quadrature_near_singularity.rule())
         rc = self._L.vpbs_set_filaments(
             self._h, len(counts), _ptr(counts), _ptr(X), _ptr(ts), _ptr(T), _ptr(xoff), _ptr(c),
-            _ptr(zs), _ptr(ws), math.nan if g is None else float(g),
+            _ptr(zs), _ptr(ws), math.nan if g is None else float(g), 0 if zn is None else len(zn), _ptr(zn), _ptr(wn),
         )
         self._check(rc, "vpbs_set_filaments")
         self.Np = int(counts.sum())
@@ -266,6 +283,39 @@ class Context:
         rc = self._L.vpbs_compute(self._h, flags, cut, _ptr(v), _ptr(psi))
         self._check(rc, "vpbs_compute")
         return v, psi
+
+    def uhat_quantity(self):
+        """Quantity of the kept Fourier field ("velocity", "streamfunction" or None), without copying it."""
+        q = ctypes.c_int32()
+        self._check(self._L.vpbs_get_uhat(self._h, ctypes.byref(q), None, None), "vpbs_get_uhat")
+        return {UHAT_VELOCITY: "velocity", UHAT_STREAMFUNCTION: "streamfunction"}.get(q.value)
+
+    def get_uhat(self):
+        """Kept Fourier field (vpbs_get_uhat): (quantity, array (3, N2, N1, N0//2+1) complex128).
+
+        quantity is "velocity", "streamfunction" or None (nothing kept).
+        """
+        q = ctypes.c_int32()
+        dims = (ctypes.c_int64 * 3)()
+        self._check(self._L.vpbs_get_uhat(self._h, ctypes.byref(q), dims, None), "vpbs_get_uhat")
+        name = {UHAT_VELOCITY: "velocity", UHAT_STREAMFUNCTION: "streamfunction"}.get(q.value)
+        if name is None:
+            return None, None
+        nkx, n1, n2 = dims
+        buf = np.empty((3, n2, n1, nkx, 2))
+        self._check(self._L.vpbs_get_uhat(self._h, None, None, _ptr(buf)), "vpbs_get_uhat")
+        return name, buf[..., 0] + 1j * buf[..., 1]
+
+    def energy_spectrum(self, nk: int, dk: float) -> np.ndarray:
+        Ek = np.empty(int(nk))
+        self._check(self._L.vpbs_energy_spectrum(self._h, int(nk), float(dk), _ptr(Ek)), "vpbs_energy_spectrum")
+        return Ek
+
+    def interpolate_band(self, kmin: float, kmax: float) -> np.ndarray:
+        out = np.empty((3, self.Np))
+        self._check(self._L.vpbs_interpolate_band(self._h, float(kmin), float(kmax), _ptr(out)),
+                    "vpbs_interpolate_band")
+        return out
 
     def neighbour_stats(self):
         counts = np.empty(self.Np, dtype=np.int64)

@@ -20,8 +20,9 @@ points, charges, derivatives on nodes, ``add_point_charges!``,
 (``pointdata="device"``, ``vpbs_set_filaments``) or on the host and uploaded
 (``pointdata="host"``, ``vpbs_set_points``, the reference's split); the
 short-range and long-range parts run on the GPU through libvpbs; the
-``lia_segment_fraction`` local integrals and the background-vorticity correction
-of ψ for unclosed filaments stay on the host (``BiotSavart.jl:255-286,570-580``).
+``lia_segment_fraction`` local integrals run on the GPU with device point data
+and on the host otherwise; the background-vorticity correction of ψ for
+unclosed filaments stays on the host (``BiotSavart.jl:255-286,570-580``).
 Errors from the library raise ``VpbsError``; invalid arguments raise
 ``ValueError`` like the reference's ``ArgumentError``/``DimensionMismatch``.
 """
@@ -40,6 +41,7 @@ from .pointdata import PointData, add_point_charges
 
 __all__ = [
     "BiotSavartCache",
+    "get_longrange_field_fourier",
     "init_cache",
     "compute_on_nodes_",
     "velocity_on_nodes_",
@@ -56,23 +58,50 @@ class BiotSavartCache:
     ``pointdata``: ``"device"`` builds the point data on the GPU from the
     filament splines, ``"host"`` builds it with :func:`add_point_charges` and
     uploads it (then kept in ``self.pointdata``).
+    ``keep_fourier``: keep the unsmoothed long-range field in Fourier space
+    after each evaluation, as the reference's long-range cache does (read by
+    :func:`get_longrange_field_fourier`, the energy spectrum and the forcing
+    and dissipation terms).
     """
 
-    def __init__(self, params: ParamsBiotSavart, device: int = 0, pointdata: str = "device"):
+    def __init__(self, params: ParamsBiotSavart, device: int = 0, pointdata: str = "device",
+                 keep_fourier: bool = True):
         if pointdata not in ("device", "host"):
             raise ValueError("pointdata must be 'device' or 'host'")
         self.params = params
         self.ctx = _lib.Context(params, device=device)
         self.pointdata_mode = pointdata
+        self.keep_fourier = keep_fourier
         self.pointdata: Optional[PointData] = None
+        self.batch: Optional[FilamentBatch] = None
         self.timings: Dict[str, float] = {}
 
     def close(self):
         self.ctx.close()
 
 
-def init_cache(params: ParamsBiotSavart, device: int = 0, pointdata: str = "device") -> BiotSavartCache:
-    return BiotSavartCache(params, device=device, pointdata=pointdata)
+def init_cache(params: ParamsBiotSavart, device: int = 0, pointdata: str = "device",
+               keep_fourier: bool = True) -> BiotSavartCache:
+    return BiotSavartCache(params, device=device, pointdata=pointdata, keep_fourier=keep_fourier)
+
+
+def get_longrange_field_fourier(cache: BiotSavartCache) -> dict:
+    """Long-range field in Fourier space of the last evaluation (``cache.jl:77-83``).
+
+    Returns ``{"field": (ux, uy, uz), "wavenumbers": (kx, ky, kz), "state": quantity}``
+    with each component of shape ``(N2, N1, N0//2+1)`` (indexed [kz, ky, kx]) and
+    ``quantity`` "velocity" or "streamfunction" (unsmoothed, as the reference's
+    ``LongRangeCacheState`` with ``smoothing_scale = 0``).
+    """
+    q, u = cache.ctx.get_uhat()
+    if q is None:
+        raise RuntimeError("no long-range Fourier field: evaluate with keep_fourier=True and the long range on")
+    Ls = cache.params.Ls
+    Ns = cache.params.Ns
+    kx = 2 * np.pi / Ls[0] * np.arange(Ns[0] // 2 + 1)
+    ky = 2 * np.pi / Ls[1] * np.fft.fftfreq(Ns[1], 1.0 / Ns[1])
+    kz = 2 * np.pi / Ls[2] * np.fft.fftfreq(Ns[2], 1.0 / Ns[2])
+    return {"field": (u[0], u[1], u[2]), "wavenumbers": (kx, ky, kz), "state": q}
 
 
 def background_vorticity_correction(batch: FilamentBatch, params: ParamsBiotSavart) -> np.ndarray:
@@ -115,7 +144,16 @@ def compute_on_nodes_(
     integration_cutoff: Optional[float] = None,
     neighbour_stats: bool = False,
 ):
-    """Compute velocity and/or streamfunction on filament nodes (``BiotSavart.jl:395-404``)."""
+    """Compute velocity and/or streamfunction on filament nodes (``BiotSavart.jl:395-404``).
+
+    ``fs`` is a list of filaments or a :class:`FilamentBatch` built from them
+    (whose spline coefficients are then reused, as the reference's filaments
+    carry theirs between evaluations).
+    """
+    if isinstance(fs, FilamentBatch):
+        batch, fs = fs, fs.fs
+    else:
+        batch = None
     if not fields:
         raise ValueError("fields must contain 'velocity' and/or 'streamfunction'")
     unknown = set(fields) - set(_FIELDS)
@@ -125,7 +163,8 @@ def compute_on_nodes_(
         if len(us) != len(fs) or any(np.shape(u) != (len(f), 3) for u, f in zip(us, fs)):
             raise ValueError(f"wrong dimensions of output vector '{name}'")
     params = cache.params
-    batch = FilamentBatch(fs)
+    if batch is None:
+        batch = FilamentBatch(fs)
     if cache.pointdata_mode == "device":
         cache.pointdata = None
         cache.ctx.set_filaments(batch, params)
@@ -136,11 +175,16 @@ def compute_on_nodes_(
     flags = _flags(fields, LIA, shortrange, longrange)
     if neighbour_stats:
         flags |= _lib.NEIGHBOUR_STATS
+    if cache.keep_fourier and longrange:
+        flags |= _lib.KEEP_UHAT
+    cache.batch = batch
     v, psi = cache.ctx.compute(flags, integration_cutoff)
     cache.timings = cache.ctx.timings()
     if shortrange and LIA != "only":
-        # host-side terms, as in the reference (BiotSavart.jl:570-580)
-        add_local_integrals(v, psi, batch, params)
+        # BiotSavart.jl:570-580; with device point data the local integrals were
+        # added on the device by vpbs_compute
+        if cache.pointdata_mode == "host":
+            add_local_integrals(v, psi, batch, params)
         if psi is not None:
             psi += background_vorticity_correction(batch, params)[:, None]
     for name, arr in (("velocity", v), ("streamfunction", psi)):

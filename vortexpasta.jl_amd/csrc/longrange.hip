@@ -514,6 +514,11 @@ struct FourierArgs {
     double inv4a2;               // 1/(4α²)
     double kmax2_trunc;          // < 0: no spherical truncation
     int do_vel, do_psi;
+    // unsmoothed field kept for the Fourier-space consumers (velocity if do_vel, else streamfunction):
+    // uhat[3][nkx][nky][nkz], kz index jz ≤ kmax2 → kz = jz, else jz - nkz
+    CT *uhat;
+    int keep_uhat, nkz;
+    int64_t ucstride;
 };
 
 template <typename CT>
@@ -532,7 +537,17 @@ __global__ void fourier_kernel(FourierArgs<CT> a, CT *__restrict__ spec, CT *__r
     CT *c1 = spec + 2 * idx;   // complex (re, im) of component 0
     CT *c2 = spec2 + 2 * idx;
     const int64_t cs = 2 * a.cstride;
+    // slot of this mode in the kept field (the retained kz only)
+    CT *cu = nullptr;
+    if (a.keep_uhat && kz >= -a.kmax[2] && kz <= a.kmax[2]) {
+        const int jz = kz >= 0 ? kz : kz + a.nkz;
+        cu = a.uhat + 2 * (((int64_t)kx * a.nky + j) * a.nkz + jz);
+    }
     if (!keep || k2 == 0) {
+        if (cu) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) cu[2 * d * a.ucstride] = cu[2 * d * a.ucstride + 1] = (CT)0;
+        }
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             if (a.do_vel) {
@@ -556,6 +571,28 @@ __global__ void fourier_kernel(FourierArgs<CT> a, CT *__restrict__ spec, CT *__r
         wr[d] = (double)c1[d * cs] * s;
         wi[d] = (double)c1[d * cs + 1] * s;
     }
+    if (cu) {
+        // unsmoothed field: ω̂/k² (one deconvolution, no Ewald factor), then i k × for the velocity
+        const double su = a.pref * D / k2;
+        double ur[3], ui[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            ur[d] = (double)c1[d * cs] * su;
+            ui[d] = (double)c1[d * cs + 1] * su;
+        }
+        if (a.do_vel) {
+            const double ax = KY * ur[2] - KZ * ur[1], bx = KY * ui[2] - KZ * ui[1];
+            const double ay = KZ * ur[0] - KX * ur[2], by = KZ * ui[0] - KX * ui[2];
+            const double az = KX * ur[1] - KY * ur[0], bz = KX * ui[1] - KY * ui[0];
+            ur[0] = -bx, ui[0] = ax, ur[1] = -by, ui[1] = ay, ur[2] = -bz, ui[2] = az;
+        }
+        const int64_t us = 2 * a.ucstride;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            cu[d * us] = (CT)ur[d];
+            cu[d * us + 1] = (CT)ui[d];
+        }
+    }
     if (a.do_psi) {
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -575,6 +612,83 @@ __global__ void fourier_kernel(FourierArgs<CT> a, CT *__restrict__ spec, CT *__r
         c1[2 * cs] = (CT)(-bz);
         c1[2 * cs + 1] = (CT)az;
     }
+}
+
+// ----------------------------------------------------------------------------
+// Consumers of the kept Fourier field.
+
+// z-pencils of the band-filtered field, ready for the inverse transforms: the
+// kept û on kmin² ≤ k² ≤ kmax² times the type-2 deconvolution, zero elsewhere.
+template <typename CT>
+__global__ void band_to_zpencils(FourierArgs<CT> a, double kmin2, double kmax2, CT *__restrict__ zp) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.cstride) return;
+    const int iz = (int)(idx % a.G[2]);
+    const int j = (int)((idx / a.G[2]) % a.nky);
+    const int kx = (int)(idx / ((int64_t)a.G[2] * a.nky));
+    const int ky = j <= a.kmax[1] ? j : j - a.nky;
+    const int kz = iz <= a.G[2] / 2 ? iz : iz - a.G[2];
+    const double KX = a.kscale[0] * kx, KY = a.kscale[1] * ky, KZ = a.kscale[2] * kz;
+    const double k2 = KX * KX + KY * KY + KZ * KZ;
+    const bool in = kz >= -a.kmax[2] && kz <= a.kmax[2] && k2 >= kmin2 && k2 <= kmax2;
+    const int64_t cs = 2 * a.cstride, us = 2 * a.ucstride;
+    CT *o = zp + 2 * idx;
+    if (!in) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) o[d * cs] = o[d * cs + 1] = (CT)0;
+        return;
+    }
+    const double D = a.dx[kx] * a.dy[ky + a.kmax[1]] * a.dz[kz + a.kmax[2]];
+    const int jz = kz >= 0 ? kz : kz + a.nkz;
+    const CT *u = a.uhat + 2 * (((int64_t)kx * a.nky + j) * a.nkz + jz);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        o[d * cs] = (CT)((double)u[d * us] * D);
+        o[d * cs + 1] = (CT)((double)u[d * us + 1] * D);
+    }
+}
+
+// Energy spectrum of the kept field (Diagnostics/spectra.jl:89-196): mode k goes
+// to bin n = trunc(|k|/Δk + 1/2) with weight f·|û|²/Δk (f = 1/2 on the kx = 0
+// plane, 1 elsewhere; from the streamfunction |k × ψ̂|² = k²|ψ̂|²). One partial
+// spectrum per block (LDS bins), summed over blocks in a fixed order.
+template <typename CT>
+__global__ void spectrum_partial(const CT *__restrict__ uhat, int64_t ucs, int nky, int nkz, int K1, int K2,
+                                 double kx_scale, double ky_scale, double kz_scale, double dk_inv, int nk,
+                                 int from_psi, double *__restrict__ partial) {
+    extern __shared__ double bins[];
+    for (int b = threadIdx.x; b < nk; b += blockDim.x) bins[b] = 0.0;
+    __syncthreads();
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < ucs;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int jz = (int)(idx % nkz);
+        const int j = (int)((idx / nkz) % nky);
+        const int kx = (int)(idx / ((int64_t)nkz * nky));
+        const int ky = j <= K1 ? j : j - nky;
+        const int kz = jz <= K2 ? jz : jz - nkz;
+        const double KX = kx_scale * kx, KY = ky_scale * ky, KZ = kz_scale * kz;
+        const double k2 = KX * KX + KY * KY + KZ * KZ;
+        const int n = (int)(sqrt(k2) * dk_inv + 0.5);
+        if (n >= nk) continue;
+        double u2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double re = uhat[2 * (d * ucs + idx)], im = uhat[2 * (d * ucs + idx) + 1];
+            u2 += re * re + im * im;
+        }
+        if (from_psi) u2 *= k2;
+        atomicAdd(&bins[n], (kx == 0 ? 0.5 : 1.0) * u2 * dk_inv);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nk; b += blockDim.x) partial[(int64_t)blockIdx.x * nk + b] = bins[b];
+}
+
+__global__ void spectrum_reduce(const double *__restrict__ partial, int nblocks, int nk, double *__restrict__ Ek) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nk) return;
+    double s = 0.0;
+    for (int k = 0; k < nblocks; k++) s += partial[(int64_t)k * nk + b];
+    Ek[b] = s;
 }
 
 // ----------------------------------------------------------------------------
@@ -712,10 +826,36 @@ double es_fourier(int k, int G, int m, double beta, const std::vector<double> &x
 }
 
 template <typename CT>
-int launch_lr(vpbs_ctx *c, uint32_t flags) {
-    hipStream_t s = c->s_lr;
-    const int64_t Np = c->Np, Nq = c->Nq;
-    const bool vel = flags & VPBS_VELOCITY, psi = flags & VPBS_STREAMFUNCTION;
+FourierArgs<CT> fourier_args(const vpbs_ctx *c) {
+    FourierArgs<CT> fa;
+    for (int d = 0; d < 3; d++) {
+        fa.G[d] = c->G[d];
+        fa.kmax[d] = c->kmax_mode[d];
+        fa.kscale[d] = 2 * M_PI / c->p.Ls[d];
+    }
+    fa.nkx = c->nkx;
+    fa.nky = c->nky;
+    fa.cstride = c->zpen_stride;
+    fa.dx = c->deconv;
+    fa.dy = c->deconv + (c->kmax_mode[0] + 1);
+    fa.dz = fa.dy + (2 * c->kmax_mode[1] + 1);
+    fa.pref = c->p.Gamma / (c->p.Ls[0] * c->p.Ls[1] * c->p.Ls[2]);
+    fa.inv4a2 = 1.0 / (4 * c->p.alpha * c->p.alpha);
+    fa.kmax2_trunc = -1;
+    if (c->p.truncate_spherical) {
+        double km = 1e300;
+        for (int d = 0; d < 3; d++) km = std::fmin(km, 2 * M_PI * ((c->p.Ns[d] - 1) / 2) / c->p.Ls[d]);
+        fa.kmax2_trunc = km * km;  // params.jl:161-166
+    }
+    fa.do_vel = fa.do_psi = 0;
+    fa.uhat = (CT *)c->uhat;
+    fa.keep_uhat = 0;
+    fa.nkz = c->nkz;
+    fa.ucstride = c->uhat_stride;
+    return fa;
+}
+
+LRGeom lr_geom(const vpbs_ctx *c) {
     LRGeom g;
     for (int d = 0; d < 3; d++) {
         g.G[d] = c->G[d];
@@ -725,6 +865,39 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
     g.M = c->p.nufft_m;
     g.tx = c->p.precision == VPBS_F32 ? tile_x<float>() : tile_x<double>();
     g.cstride = c->comp_stride;
+    return g;
+}
+
+// Inverse transforms of z-pencils zp into the real grid (z, y, x passes with the
+// pencil transposes between them).
+template <typename CT>
+int inverse_to_grid(vpbs_ctx *c, Cx<CT> *zp) {
+    hipStream_t s = c->s_lr;
+    const int G0 = c->G[0], G1 = c->G[1], G2 = c->G[2];
+    const int nkx = c->nkx, nky = c->nky, K1 = c->kmax_mode[1];
+    Cx<CT> *rows = (Cx<CT> *)c->grid;
+    Cx<CT> *ypen = (Cx<CT> *)c->ypen;
+    auto fft = [&](int dim, void *buf) -> int {
+        void *io[1] = {buf};
+        if (rocfft_execute(c->plan[dim][1], io, nullptr, c->fft_info) != rocfft_status_success)
+            return fail(c, VPBS_ERR_FFT, "rocfft_execute failed (dimension %d, inverse)", dim);
+        return VPBS_OK;
+    };
+    VPBS_TRY(fft(2, zp));
+    zpencils_to_ypencils<CT><<<dim3(G2 / 32, G1 / 32, 3 * nkx), 256, 0, s>>>(zp, ypen, G1, G2, nky, K1);
+    VPBS_TRY(fft(1, ypen));
+    ypencils_to_pair_rows<CT><<<dim3(G0 / 32, G1 / 32, 3 * G2), 256, 0, s>>>(ypen, rows, G0, G1, G2, nkx);
+    VPBS_TRY(fft(0, rows));
+    VPBS_HIP(c, hipGetLastError());
+    return VPBS_OK;
+}
+
+template <typename CT>
+int launch_lr(vpbs_ctx *c, uint32_t flags) {
+    hipStream_t s = c->s_lr;
+    const int64_t Np = c->Np, Nq = c->Nq;
+    const bool vel = flags & VPBS_VELOCITY, psi = flags & VPBS_STREAMFUNCTION;
+    const LRGeom g = lr_geom(c);
     const CT beta = (CT)c->es_beta;
 
     stage_begin(c, ST_LRSORT, s);
@@ -753,16 +926,7 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
                         dir ? "inverse" : "forward");
         return VPBS_OK;
     };
-    // z-pencils -> ypen -> pair rows -> real grid in place
-    auto inverse = [&](Cx<CT> *zp) -> int {
-        VPBS_TRY(fft(2, 1, zp));
-        zpencils_to_ypencils<CT><<<dim3(G2 / 32, G1 / 32, 3 * nkx), 256, 0, s>>>(zp, ypen, G1, G2, nky, K1);
-        VPBS_TRY(fft(1, 1, ypen));
-        ypencils_to_pair_rows<CT><<<dim3(G0 / 32, G1 / 32, 3 * G2), 256, 0, s>>>(ypen, rows, G0, G1, G2, nkx);
-        VPBS_TRY(fft(0, 1, rows));
-        VPBS_HIP(c, hipGetLastError());
-        return VPBS_OK;
-    };
+    auto inverse = [&](Cx<CT> *zp) -> int { return inverse_to_grid<CT>(c, zp); };
 
     stage_begin(c, ST_FFTF, s);
     Cx<CT> *zpen = (Cx<CT> *)c->zpen;
@@ -774,28 +938,10 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
     stage_end(c, ST_FFTF, s);
     VPBS_HIP(c, hipGetLastError());
 
-    FourierArgs<CT> fa;
-    for (int d = 0; d < 3; d++) {
-        fa.G[d] = c->G[d];
-        fa.kmax[d] = c->kmax_mode[d];
-        fa.kscale[d] = 2 * M_PI / c->p.Ls[d];
-    }
-    fa.nkx = nkx;
-    fa.nky = nky;
-    fa.cstride = c->zpen_stride;
-    fa.dx = c->deconv;
-    fa.dy = c->deconv + (c->kmax_mode[0] + 1);
-    fa.dz = fa.dy + (2 * c->kmax_mode[1] + 1);
-    fa.pref = c->p.Gamma / (c->p.Ls[0] * c->p.Ls[1] * c->p.Ls[2]);
-    fa.inv4a2 = 1.0 / (4 * c->p.alpha * c->p.alpha);
-    fa.kmax2_trunc = -1;
-    if (c->p.truncate_spherical) {
-        double km = 1e300;
-        for (int d = 0; d < 3; d++) km = std::fmin(km, 2 * M_PI * ((c->p.Ns[d] - 1) / 2) / c->p.Ls[d]);
-        fa.kmax2_trunc = km * km;  // params.jl:161-166
-    }
+    FourierArgs<CT> fa = fourier_args<CT>(c);
     fa.do_vel = vel;
     fa.do_psi = psi;
+    fa.keep_uhat = (flags & VPBS_KEEP_UHAT) ? 1 : 0;
     Cx<CT> *zpen2 = psi ? (Cx<CT> *)c->zpen2 : zpen;
     stage_begin(c, ST_FOURIER, s);
     fourier_kernel<CT><<<nblocks(fa.cstride, 256), 256, 0, s>>>(fa, (CT *)zpen, (CT *)zpen2);
@@ -854,6 +1000,8 @@ int longrange_setup(vpbs_ctx *c) {
     c->nky = 2 * c->kmax_mode[1] + 1;
     c->ypen_stride = (int64_t)c->nkx * c->G[2] * c->G[1];
     c->zpen_stride = (int64_t)c->nkx * c->nky * c->G[2];
+    c->nkz = 2 * c->kmax_mode[2] + 1;
+    c->uhat_stride = (int64_t)c->nkx * c->nky * c->nkz;
 
     // deconvolution tables 1/ψ̂(k)
     std::vector<double> gx, gw;
@@ -911,8 +1059,106 @@ int longrange_compute(vpbs_ctx *c, uint32_t flags) {
     const bool psi = flags & VPBS_STREAMFUNCTION;
     const size_t es = c->p.precision == VPBS_F32 ? sizeof(float) : sizeof(double);
     if (psi && !c->zpen2) VPBS_HIP(c, hipMalloc(&c->zpen2, 2 * es * 3 * (size_t)c->zpen_stride));
-    if (c->p.precision == VPBS_F32) return launch_lr<float>(c, flags);
-    return launch_lr<double>(c, flags);
+    const bool keep = flags & VPBS_KEEP_UHAT;
+    if (keep && !c->uhat) VPBS_HIP(c, hipMalloc(&c->uhat, 2 * es * 3 * (size_t)c->uhat_stride));
+    c->uhat_quantity = 0;
+    const int rc = c->p.precision == VPBS_F32 ? launch_lr<float>(c, flags) : launch_lr<double>(c, flags);
+    if (rc == VPBS_OK && keep)
+        c->uhat_quantity = (flags & VPBS_VELOCITY) ? VPBS_UHAT_VELOCITY : VPBS_UHAT_STREAMFUNCTION;
+    return rc;
+}
+
+namespace {
+
+template <typename CT>
+int band_impl(vpbs_ctx *c, double kmin, double kmax, double *v_out) {
+    hipStream_t s = c->s_lr;
+    FourierArgs<CT> fa = fourier_args<CT>(c);
+    band_to_zpencils<CT><<<nblocks(fa.cstride, 256), 256, 0, s>>>(fa, kmin * kmin, kmax * kmax, (CT *)c->zpen);
+    VPBS_HIP(c, hipGetLastError());
+    VPBS_TRY(inverse_to_grid<CT>(c, (Cx<CT> *)c->zpen));
+    launch_interp<CT>(lr_geom(c), c->Np, c->nodes, (CT)c->es_beta, (const CT *)c->grid, c->v_lr, s);
+    VPBS_HIP(c, hipGetLastError());
+    VPBS_HIP(c, hipMemcpyAsync(v_out, c->v_lr, sizeof(double) * 3 * (size_t)c->Np, hipMemcpyDeviceToHost, s));
+    VPBS_HIP(c, hipStreamSynchronize(s));
+    return VPBS_OK;
+}
+
+template <typename CT>
+int spectrum_impl(vpbs_ctx *c, int nk, double dk, double *Ek) {
+    hipStream_t s = c->s_lr;
+    const int nb = 1024, bs = 256;
+    double *partial = nullptr, *out = nullptr;
+    VPBS_HIP(c, hipMallocAsync((void **)&partial, sizeof(double) * ((size_t)nb * nk + nk), s));
+    out = partial + (size_t)nb * nk;
+    spectrum_partial<CT><<<nb, bs, sizeof(double) * nk, s>>>(
+        (const CT *)c->uhat, c->uhat_stride, c->nky, c->nkz, c->kmax_mode[1], c->kmax_mode[2], 2 * M_PI / c->p.Ls[0],
+        2 * M_PI / c->p.Ls[1], 2 * M_PI / c->p.Ls[2], 1.0 / dk, nk, c->uhat_quantity == VPBS_UHAT_STREAMFUNCTION,
+        partial);
+    spectrum_reduce<<<nblocks(nk, 256), 256, 0, s>>>(partial, nb, nk, out);
+    VPBS_HIP(c, hipGetLastError());
+    VPBS_HIP(c, hipMemcpyAsync(Ek, out, sizeof(double) * nk, hipMemcpyDeviceToHost, s));
+    VPBS_HIP(c, hipFreeAsync(partial, s));
+    VPBS_HIP(c, hipStreamSynchronize(s));
+    return VPBS_OK;
+}
+
+template <typename CT>
+int get_uhat_impl(vpbs_ctx *c, double *out) {
+    const int64_t n = 3 * c->uhat_stride;
+    std::vector<CT> h(2 * (size_t)n);
+    VPBS_HIP(c, hipMemcpy(h.data(), c->uhat, sizeof(CT) * 2 * (size_t)n, hipMemcpyDeviceToHost));
+    const int64_t Nkx = c->p.Ns[0] / 2 + 1, N1 = c->p.Ns[1], N2 = c->p.Ns[2];
+    const int64_t K = Nkx * N1 * N2;
+    std::fill(out, out + 2 * 3 * K, 0.0);
+    const int K1 = c->kmax_mode[1], K2 = c->kmax_mode[2];
+    for (int d = 0; d < 3; d++)
+        for (int kx = 0; kx < c->nkx; kx++)
+            for (int j = 0; j < c->nky; j++) {
+                const int ky = j <= K1 ? j : j - c->nky;
+                const int64_t iy = ky >= 0 ? ky : ky + N1;
+                for (int jz = 0; jz < c->nkz; jz++) {
+                    const int kz = jz <= K2 ? jz : jz - c->nkz;
+                    const int64_t iz = kz >= 0 ? kz : kz + N2;
+                    const int64_t src = d * c->uhat_stride + ((int64_t)kx * c->nky + j) * c->nkz + jz;
+                    const int64_t dst = d * K + (iz * N1 + iy) * Nkx + kx;
+                    out[2 * dst] = (double)h[2 * src];
+                    out[2 * dst + 1] = (double)h[2 * src + 1];
+                }
+            }
+    return VPBS_OK;
+}
+
+}  // namespace
+
+int longrange_get_uhat(vpbs_ctx *c, int32_t *quantity, int64_t *dims3, double *out) {
+    if (quantity) *quantity = c->uhat_quantity;
+    if (dims3) {
+        dims3[0] = c->p.Ns[0] / 2 + 1;
+        dims3[1] = c->p.Ns[1];
+        dims3[2] = c->p.Ns[2];
+    }
+    if (!out) return VPBS_OK;
+    if (!c->uhat_quantity)
+        return fail(c, VPBS_ERR_STATE, "no Fourier field kept: run vpbs_compute with VPBS_KEEP_UHAT (and the long range)");
+    return c->p.precision == VPBS_F32 ? get_uhat_impl<float>(c, out) : get_uhat_impl<double>(c, out);
+}
+
+int longrange_spectrum(vpbs_ctx *c, int64_t nk, double dk, double *Ek) {
+    if (c->uhat_quantity != VPBS_UHAT_VELOCITY && c->uhat_quantity != VPBS_UHAT_STREAMFUNCTION)
+        return fail(c, VPBS_ERR_STATE, "no Fourier field kept: run vpbs_compute with VPBS_KEEP_UHAT (and the long range)");
+    if (nk < 1 || nk > 8192) return fail(c, VPBS_ERR_INVALID, "spectrum length must be in [1, 8192] (got %lld)", (long long)nk);
+    if (!(dk > 0) || !std::isfinite(dk)) return fail(c, VPBS_ERR_INVALID, "wavenumber step must be positive");
+    if (!Ek) return fail(c, VPBS_ERR_INVALID, "null output");
+    return c->p.precision == VPBS_F32 ? spectrum_impl<float>(c, (int)nk, dk, Ek) : spectrum_impl<double>(c, (int)nk, dk, Ek);
+}
+
+int longrange_band(vpbs_ctx *c, double kmin, double kmax, double *v_out) {
+    if (!c->uhat_quantity)
+        return fail(c, VPBS_ERR_STATE, "no Fourier field kept: run vpbs_compute with VPBS_KEEP_UHAT (and the long range)");
+    if (!(kmin >= 0) || !(kmax >= kmin)) return fail(c, VPBS_ERR_INVALID, "expected 0 <= kmin <= kmax");
+    if (!v_out) return fail(c, VPBS_ERR_INVALID, "null output");
+    return c->p.precision == VPBS_F32 ? band_impl<float>(c, kmin, kmax, v_out) : band_impl<double>(c, kmin, kmax, v_out);
 }
 
 void longrange_release(vpbs_ctx *c) {
@@ -923,10 +1169,10 @@ void longrange_release(vpbs_ctx *c) {
         }
     if (c->fft_info) rocfft_execution_info_destroy(c->fft_info);
     c->fft_info = nullptr;
-    void *bufs[] = {c->fft_work, c->grid, c->ypen, c->zpen, c->zpen2, c->deconv};
+    void *bufs[] = {c->fft_work, c->grid, c->ypen, c->zpen, c->zpen2, c->deconv, c->uhat};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    c->fft_work = c->grid = c->ypen = c->zpen = c->zpen2 = nullptr;
+    c->fft_work = c->grid = c->ypen = c->zpen = c->zpen2 = c->uhat = nullptr;
     c->deconv = nullptr;
 }
 

@@ -237,6 +237,63 @@ __global__ void point_data_kernel(Fil f, Quad qd, Box box, const double *__restr
     }
 }
 
+// Near-singular local integrals for lia_segment_fraction = γ
+// (add_local_integrals!, local_integrals.jl:209-305): the parts of the two
+// segments next to node i not covered by the LIA term, ζ ∈ [0, 1-γ] of segment
+// i-1 and ζ ∈ [γ, 1] of segment i, integrated with the full Biot–Savart
+// kernel and the near-singularity quadrature rule; FP64, added to the
+// short-range outputs (v += Γ/4π Σ q × r/|r|³, ψ += Γ/4π Σ q/|r|).
+struct LocalQuad {
+    int n;
+    double z[16], w[16];
+    double lo[2], hi[2];
+};
+
+__global__ void local_integrals_kernel(Fil f, LocalQuad lq, Box box, const double *__restrict__ c,
+                                       const double *__restrict__ dco, const double *__restrict__ nodes, double pref,
+                                       double *__restrict__ v, double *__restrict__ psi) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t Np = f.Np;
+    if (i >= Np) return;
+    const int fi = f.fid[i];
+    const int64_t off = f.off[fi];
+    const int N = f.N[fi];
+    const double T = f.T[fi];
+    const int l = (int)(i - off);
+    const double x[3] = {nodes[i], nodes[Np + i], nodes[2 * Np + i]};
+    double av[3] = {0, 0, 0}, ap[3] = {0, 0, 0};
+    for (int side = 0; side < 2; side++) {
+        const int seg = side == 0 ? (l == 0 ? N - 1 : l - 1) : l;
+        const double lo = lq.lo[side], delta = lq.hi[side] - lq.lo[side];
+        const double dt = (knot(f, off, N, T, seg + 1) - knot(f, off, N, T, seg)) * delta;
+        for (int q = 0; q < lq.n; q++) {
+            const double y = lo + delta * lq.z[q];
+            double sv[3], s1[3], r[3];
+            eval_spline<0>(f, fi, seg, y, c, sv);
+            eval_spline<1>(f, fi, seg, y, dco, s1);
+            const double wq = lq.w[q] * dt;
+            for (int d = 0; d < 3; d++) {
+                const double sd = fold(sv[d], box.L[d]);
+                double rd = x[d] - sd;
+                if (rd >= box.Lh[d]) rd -= box.L[d];
+                if (rd < -box.Lh[d]) rd += box.L[d];
+                r[d] = rd;
+                s1[d] *= wq;
+            }
+            const double rinv = 1.0 / sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+            const double rinv3 = rinv * rinv * rinv;
+            av[0] += (s1[1] * r[2] - s1[2] * r[1]) * rinv3;
+            av[1] += (s1[2] * r[0] - s1[0] * r[2]) * rinv3;
+            av[2] += (s1[0] * r[1] - s1[1] * r[0]) * rinv3;
+            for (int d = 0; d < 3; d++) ap[d] += s1[d] * rinv;
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        if (v) v[d * Np + i] += pref * av[d];
+        if (psi) psi[d * Np + i] += pref * ap[d];
+    }
+}
+
 }  // namespace
 
 // Upload the filament description, then derive the point data on the device.
@@ -244,7 +301,8 @@ __global__ void point_data_kernel(Fil f, Quad qd, Box box, const double *__restr
 // done here make every index the kernels form stay inside its array.
 int build_point_data(vpbs_ctx *c, int64_t Nf, const int64_t *counts, const double *X, const double *knots,
                      const double *periods, const double *offsets, const double *coefs, const double *quad_nodes,
-                     const double *quad_weights, double lia_segment_fraction) {
+                     const double *quad_weights, double lia_segment_fraction, int32_t nquad_ns,
+                     const double *quad_ns_nodes, const double *quad_ns_weights) {
     if (Nf < 1) return fail(c, VPBS_ERR_INVALID, "need at least one filament (got Nf = %lld)", (long long)Nf);
     if (!counts || !X || !knots || !periods || !offsets || !coefs)
         return fail(c, VPBS_ERR_INVALID, "null input array");
@@ -256,6 +314,8 @@ int build_point_data(vpbs_ctx *c, int64_t Nf, const int64_t *counts, const doubl
         return fail(c, VPBS_ERR_INVALID, "straight segments (no quadrature rule) need nquad = 1 (got %d)", nq);
     if (!std::isnan(lia_segment_fraction) && !(lia_segment_fraction > 0.0 && lia_segment_fraction <= 1.0))
         return fail(c, VPBS_ERR_INVALID, "lia_segment_fraction must be in (0, 1] (got %g)", lia_segment_fraction);
+    if (nquad_ns < 0 || nquad_ns > 16 || (nquad_ns > 0 && (!quad_ns_nodes || !quad_ns_weights)))
+        return fail(c, VPBS_ERR_INVALID, "near-singularity quadrature: 0..16 points with nodes and weights");
     std::vector<int64_t> off((size_t)Nf);
     std::vector<int32_t> nn((size_t)Nf);
     int64_t Np = 0;
@@ -361,9 +421,46 @@ int build_point_data(vpbs_ctx *c, int64_t Nf, const int64_t *counts, const doubl
     float ms = 0.f;
     VPBS_HIP(c, hipEventElapsedTime(&ms, c->ev_beg[ST_POINTS], c->ev_end[ST_POINTS]));
     c->timings.point_data = ms;
+    // near-singular local integrals, evaluated in vpbs_compute (lia_segment_fraction set)
+    c->local_ns = (!std::isnan(lia_segment_fraction) && nquad_ns > 0) ? nquad_ns : 0;
+    c->local_gamma = lia_segment_fraction;
+    c->local_nf = Nf;
+    for (int q = 0; q < c->local_ns; q++) {
+        c->local_z[q] = quad_ns_nodes[q];
+        c->local_w[q] = quad_ns_weights[q];
+    }
     c->have_points = true;
     c->have_stats = false;
     c->last_vel = c->last_psi = false;
+    return VPBS_OK;
+}
+
+int add_local_integrals(vpbs_ctx *c, uint32_t flags, hipStream_t s) {
+    if (!c->local_ns) return VPBS_OK;
+    const bool vel = flags & VPBS_VELOCITY, psi = flags & VPBS_STREAMFUNCTION;
+    Fil f;
+    f.fid = c->fil_fid;
+    f.off = c->fil_off;
+    f.N = c->fil_n;
+    f.T = c->fil_T;
+    f.Xoff = c->fil_xoff;
+    f.ts = c->fil_ts;
+    f.Np = c->Np;
+    f.Nf = c->local_nf;
+    LocalQuad lq{};
+    lq.n = c->local_ns;
+    for (int q = 0; q < lq.n; q++) {
+        lq.z[q] = c->local_z[q];
+        lq.w[q] = c->local_w[q];
+    }
+    lq.lo[0] = 0.0;  // nonlia_integration_limits (local_integrals.jl:103-104)
+    lq.hi[0] = 1.0 - c->local_gamma;
+    lq.lo[1] = c->local_gamma;
+    lq.hi[1] = 1.0;
+    const double pref = c->p.Gamma / (4 * M_PI);
+    local_integrals_kernel<<<nblocks(c->Np, 128), 128, 0, s>>>(f, lq, c->box, c->fil_c, c->fil_d, c->nodes, pref,
+                                                              vel ? c->v_sr : nullptr, psi ? c->psi_sr : nullptr);
+    VPBS_HIP(c, hipGetLastError());
     return VPBS_OK;
 }
 

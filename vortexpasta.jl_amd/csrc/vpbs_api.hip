@@ -252,6 +252,7 @@ static int set_points_impl(vpbs_ctx *c, int64_t Np, const double *nodes, const i
     c->Np = Np;
     c->Nq = Nq;
     c->timings.point_data = 0.0;
+    c->local_ns = 0;  // no filament description on the device: local integrals stay with the caller
     if (kind == hipMemcpyDeviceToDevice) VPBS_TRY(device_check(c));
     c->have_points = true;
     c->have_stats = false;
@@ -352,10 +353,11 @@ int vpbs_set_points_device(vpbs_ctx *c, int64_t Np, const double *nodes, const i
 
 int vpbs_set_filaments(vpbs_ctx *c, int64_t Nf, const int64_t *counts, const double *X, const double *knots,
                        const double *periods, const double *offsets, const double *coefs, const double *quad_nodes,
-                       const double *quad_weights, double lia_segment_fraction) {
+                       const double *quad_weights, double lia_segment_fraction, int32_t nquad_ns,
+                       const double *quad_ns_nodes, const double *quad_ns_weights) {
     if (!c) return VPBS_ERR_INVALID;
     return build_point_data(c, Nf, counts, X, knots, periods, offsets, coefs, quad_nodes, quad_weights,
-                            lia_segment_fraction);
+                            lia_segment_fraction, nquad_ns, quad_ns_nodes, quad_ns_weights);
 }
 
 int vpbs_get_points(vpbs_ctx *c, double *nodes, int32_t *idx_prev, double *sp, double *spp, double *dlm,
@@ -393,8 +395,15 @@ int vpbs_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff, double 
     const bool with_lr = !(flags & VPBS_NO_LONGRANGE) && !(flags & VPBS_LIA_ONLY);
     const int64_t n3 = 3 * c->Np;
 
-    if (with_lr) VPBS_TRY(longrange_compute(c, flags));
-    if (with_sr) VPBS_TRY(shortrange_compute(c, flags, integration_cutoff));
+    if (with_lr)
+        VPBS_TRY(longrange_compute(c, flags));
+    else
+        c->uhat_quantity = 0;
+    if (with_sr) {
+        VPBS_TRY(shortrange_compute(c, flags, integration_cutoff));
+        // BiotSavart.jl:570-576: with the short range, unless LIA = Val(:only)
+        if (!(flags & VPBS_LIA_ONLY)) VPBS_TRY(add_local_integrals(c, flags, c->s_sr));
+    }
     if (with_lr) VPBS_HIP(c, hipEventRecord(c->ev_lr_done, c->s_lr));
 
     // merge on the short-range stream
@@ -474,6 +483,25 @@ int vpbs_neighbour_stats(vpbs_ctx *c, int64_t *counts, uint64_t *hashes) {
     if (counts) VPBS_HIP(c, hipMemcpy(counts, c->stat_counts, sizeof(int64_t) * c->Np, hipMemcpyDeviceToHost));
     if (hashes) VPBS_HIP(c, hipMemcpy(hashes, c->stat_hashes, sizeof(uint64_t) * c->Np, hipMemcpyDeviceToHost));
     return VPBS_OK;
+}
+
+int vpbs_get_uhat(vpbs_ctx *c, int32_t *quantity, int64_t *dims3, double *out) {
+    if (!c) return VPBS_ERR_INVALID;
+    VPBS_HIP(c, hipSetDevice(c->device));
+    return longrange_get_uhat(c, quantity, dims3, out);
+}
+
+int vpbs_energy_spectrum(vpbs_ctx *c, int64_t nk, double dk, double *Ek) {
+    if (!c) return VPBS_ERR_INVALID;
+    VPBS_HIP(c, hipSetDevice(c->device));
+    return longrange_spectrum(c, nk, dk, Ek);
+}
+
+int vpbs_interpolate_band(vpbs_ctx *c, double kmin, double kmax, double *v_out) {
+    if (!c) return VPBS_ERR_INVALID;
+    if (!c->have_points) return fail(c, VPBS_ERR_STATE, "no point data: call vpbs_set_points or vpbs_set_filaments");
+    VPBS_HIP(c, hipSetDevice(c->device));
+    return longrange_band(c, kmin, kmax, v_out);
 }
 
 int vpbs_get_timings(vpbs_ctx *c, vpbs_timings *t) {

@@ -69,6 +69,11 @@ struct vpbs_ctx {
     int32_t *fil_fid = nullptr, *fil_n = nullptr;
     int64_t *fil_off = nullptr;
     double *fil_T = nullptr, *fil_xoff = nullptr;
+    // near-singular local integrals (lia_segment_fraction), set by vpbs_set_filaments
+    int local_ns = 0;          // near-singularity quadrature points (0: not evaluated on the device)
+    double local_gamma = 0;
+    int64_t local_nf = 0;
+    double local_z[16]{}, local_w[16]{};
 
     // outputs (device, FP64, 3*Np each)
     double *v_sr = nullptr, *psi_sr = nullptr, *v_lr = nullptr, *psi_lr = nullptr, *v_out = nullptr,
@@ -103,6 +108,11 @@ struct vpbs_ctx {
     int32_t nkx = 0, nky = 0;
     int64_t ypen_stride = 0, zpen_stride = 0;  // complex values per component
     void *ypen = nullptr, *zpen = nullptr, *zpen2 = nullptr;
+    // unsmoothed field kept by VPBS_KEEP_UHAT: [3][nkx][nky][nkz] complex (compute type)
+    int32_t nkz = 0;
+    int64_t uhat_stride = 0;   // complex values per component
+    void *uhat = nullptr;
+    int uhat_quantity = 0;     // 0: none, VPBS_UHAT_VELOCITY or VPBS_UHAT_STREAMFUNCTION
     int32_t *lr_key = nullptr, *lr_start = nullptr, *lr_perm = nullptr;
     uint8_t *lr_active = nullptr;  // per tile: 1 if any neighbour tile holds a point
     int32_t *lr_i0 = nullptr;   // tile-sorted points: first stencil index per direction, 3*Nq
@@ -160,6 +170,9 @@ int shortrange_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff);
 // longrange.hip
 int longrange_setup(vpbs_ctx *c);
 int longrange_compute(vpbs_ctx *c, uint32_t flags);
+int longrange_get_uhat(vpbs_ctx *c, int32_t *quantity, int64_t *dims3, double *out);
+int longrange_spectrum(vpbs_ctx *c, int64_t nk, double dk, double *Ek);
+int longrange_band(vpbs_ctx *c, double kmin, double kmax, double *v_out);
 void longrange_release(vpbs_ctx *c);
 int oversampled_size(int N, double sigma);
 double es_beta(int m, double sigma);
@@ -167,7 +180,9 @@ double es_beta(int m, double sigma);
 // pointdata.hip
 int build_point_data(vpbs_ctx *c, int64_t Nf, const int64_t *counts, const double *X, const double *knots,
                      const double *periods, const double *offsets, const double *coefs, const double *quad_nodes,
-                     const double *quad_weights, double lia_segment_fraction);
+                     const double *quad_weights, double lia_segment_fraction, int32_t nquad_ns,
+                     const double *quad_ns_nodes, const double *quad_ns_weights);
+int add_local_integrals(vpbs_ctx *c, uint32_t flags, hipStream_t s);
 
 // vpbs_api.hip
 int ensure_capacity(vpbs_ctx *c, int64_t Np, int64_t Nq);
