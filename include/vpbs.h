@@ -166,6 +166,23 @@ int vpbs_set_filaments(vpbs_ctx *ctx, int64_t Nf, const int64_t *counts, const d
                        const double *quad_weights, double lia_segment_fraction, int32_t nquad_ns,
                        const double *quad_ns_nodes, const double *quad_ns_weights);
 
+/*
+ * Advance the filaments uploaded by vpbs_set_filaments by nsteps steps of the
+ * SSPRK33 scheme with their Biot–Savart velocity (Timestepping.step!,
+ * Timestepping.jl:1038-1076; SSPRK33.jl:17-39; advect.jl:20-72), entirely on
+ * the device: each stage builds the point data, evaluates the velocity
+ * (VPBS_VELOCITY, LIA on, short and long range) and moves the nodes; within a
+ * step the knots are kept and only the spline coefficients are solved again;
+ * after the step the knots are recomputed from the new nodes (chordal). No
+ * forcing, reconnections or adaptive time step. The point data of the final
+ * state is left in the context.
+ */
+int vpbs_advance(vpbs_ctx *ctx, double dt, int32_t nsteps);
+
+/* Filament state on the device (after vpbs_set_filaments / vpbs_advance): nodes X[3*Np]
+ * (not folded), knots[Np], periods[Nf], spline coefficients coefs[3*Np]; any may be NULL. */
+int vpbs_get_filaments(vpbs_ctx *ctx, double *X, double *knots, double *periods, double *coefs);
+
 /* Copy the device point data (from vpbs_set_points* or vpbs_set_filaments) to host arrays; any may be NULL. */
 int vpbs_get_points(vpbs_ctx *ctx, double *nodes, int32_t *idx_prev, double *sp, double *spp, double *dlm,
                     double *dlp, double *points, double *charges);

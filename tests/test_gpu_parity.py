@@ -499,3 +499,81 @@ def test_forcing_and_dissipation(vp, cfg2):
     with pytest.raises(ValueError, match="too large"):
         F.dissipation_velocity(F.SmallScaleDissipationBS(kdiss=70.0, α=0.5), cache, cfg2, vs)
     cache.close()
+
+
+# --------------------------------------------------------------------------- #
+# Device-resident time stepping (vpbs_advance)
+
+
+def _oracle_velocity(vp, O, params):
+    def velocity(b):
+        pd = vp.add_point_charges(b.fs, params, batch=b)
+        return O.compute(pd, params, ("velocity",))["velocity"].T
+    return velocity
+
+
+# Time steps below the Kelvin-wave stability limit at the node spacing δ: the
+# fastest mode, k = π/δ, oscillates at ω ≈ Γ k²/(4π) ln(1/(ka)) (≈ 7e3 for the
+# 64-node ring, ≈ 1.3e4 for rings with δ = 2π/256); with ω·Δt ≫ 1 rounding
+# differences between two equal implementations grow exponentially.
+
+
+def test_device_stepper_vs_oracle(vp, O, cfg1):
+    """Two SSPRK33 steps of the 64-node ring on the device vs the host stepper driven by the FP64 oracle (1e-10)."""
+    params = _params(vp)
+    dt = 1e-4
+    b0 = vp.FilamentBatch(cfg1)
+    want = b0
+    for _ in range(2):
+        want = vp.timestepping.ssprk33_step(want, dt, _oracle_velocity(vp, O, params))
+    cache = vp.init_cache(params)
+    st = vp.timestepping.DeviceStepper(cache, b0)
+    st.advance(dt, 2)
+    got = cache.ctx.get_filaments()
+    err = np.abs(got["X"] - want.X).max() / np.abs(want.X - b0.X).max()
+    print(f"  displacement error rel. to the displacement: {err:.2e}", flush=True)
+    assert err < 1e-10
+    assert np.allclose(got["knots"], want.ts, rtol=0, atol=1e-13)
+    assert np.allclose(got["coefs"], want.c, rtol=0, atol=1e-12)
+    cache.close()
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+def test_device_stepper_vs_host_stepper(vp, cfg2, T):
+    """Three steps of 16 rings: device-resident stepping vs host stepping with the same library velocity."""
+    fs = cfg2[:16]
+    params = _params(vp, T=T, Ns=128)
+    dt = 5e-5
+    hcache = vp.init_cache(params, pointdata="host", keep_fourier=False)
+    b = vp.FilamentBatch(fs)
+    for _ in range(3):
+        b = vp.timestepping.ssprk33_step(b, dt, vp.timestepping.cache_velocity(hcache))
+    dcache = vp.init_cache(params)
+    st = vp.timestepping.DeviceStepper(dcache, vp.FilamentBatch(fs))
+    st.advance(dt, 3)
+    X = st.nodes()
+    X0 = np.concatenate([f.nodes for f in fs])
+    err = np.abs(X - b.X).max() / np.abs(b.X - X0).max()
+    print(f"  {T.__name__}: device vs host stepper, error rel. to displacement {err:.2e}", flush=True)
+    assert err < (1e-10 if T is np.float64 else 1e-6)
+    # the final point data is that of the final state
+    pd = dcache.ctx.get_points()
+    assert np.abs(pd["nodes"] - vp.filaments.fold_coordinates_periodic(X.T.ravel(), 2 * math.pi).reshape(3, -1)).max() < 1e-15
+    hcache.close()
+    dcache.close()
+
+
+def test_device_stepper_errors(vp, cfg1):
+    params = _params(vp)
+    ctx = vp._lib.Context(params)
+    with pytest.raises(vp._lib.VpbsError, match="vpbs_set_filaments"):
+        ctx.advance(0.01, 1)
+    ctx.set_points(vp.add_point_charges(cfg1, params))
+    with pytest.raises(vp._lib.VpbsError, match="vpbs_set_filaments"):
+        ctx.advance(0.01, 1)
+    ctx.set_filaments(vp.FilamentBatch(cfg1), params)
+    with pytest.raises(vp._lib.VpbsError, match="positive"):
+        ctx.advance(-1.0, 1)
+    ctx.advance(0.01, 0)  # no steps: state unchanged
+    assert np.array_equal(ctx.get_filaments()["X"], vp.FilamentBatch(cfg1).X)
+    ctx.close()

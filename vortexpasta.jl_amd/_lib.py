@@ -37,6 +37,8 @@ EXPORTED = (
     "vpbs_set_points_device",
     "vpbs_set_filaments",
     "vpbs_get_points",
+    "vpbs_advance",
+    "vpbs_get_filaments",
     "vpbs_get_uhat",
     "vpbs_energy_spectrum",
     "vpbs_interpolate_band",
@@ -132,6 +134,10 @@ def load() -> ctypes.CDLL:
     L.vpbs_set_filaments.restype = ctypes.c_int
     L.vpbs_get_points.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.vpbs_get_points.restype = ctypes.c_int
+    L.vpbs_advance.argtypes = [vp, d, ctypes.c_int32]
+    L.vpbs_advance.restype = ctypes.c_int
+    L.vpbs_get_filaments.argtypes = [vp, vp, vp, vp, vp]
+    L.vpbs_get_filaments.restype = ctypes.c_int
     L.vpbs_get_uhat.argtypes = [vp, vp, vp, vp]
     L.vpbs_get_uhat.restype = ctypes.c_int
     L.vpbs_energy_spectrum.argtypes = [vp, i64, d, vp]
@@ -261,6 +267,7 @@ class Context:
         )
         self._check(rc, "vpbs_set_filaments")
         self.Np = int(counts.sum())
+        self.Nf = len(counts)
 
     def get_points(self):
         """Device point data as a dict of host arrays (vpbs_get_points)."""
@@ -283,6 +290,19 @@ class Context:
         rc = self._L.vpbs_compute(self._h, flags, cut, _ptr(v), _ptr(psi))
         self._check(rc, "vpbs_compute")
         return v, psi
+
+    def advance(self, dt: float, nsteps: int = 1):
+        """SSPRK33 steps on the device-resident filaments (vpbs_advance)."""
+        self._check(self._L.vpbs_advance(self._h, float(dt), int(nsteps)), "vpbs_advance")
+
+    def get_filaments(self):
+        """Device filament state: dict with X (Np, 3), knots (Np,), periods (Nf,), coefs (Np, 3)."""
+        X = np.empty((3, self.Np))
+        c = np.empty((3, self.Np))
+        ts = np.empty(self.Np)
+        T = np.empty(self.Nf)
+        self._check(self._L.vpbs_get_filaments(self._h, _ptr(X), _ptr(ts), _ptr(T), _ptr(c)), "vpbs_get_filaments")
+        return {"X": X.T.copy(), "knots": ts, "periods": T, "coefs": c.T.copy()}
 
     def uhat_quantity(self):
         """Quantity of the kept Fourier field ("velocity", "streamfunction" or None), without copying it."""

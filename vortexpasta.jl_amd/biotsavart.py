@@ -201,5 +201,6 @@ def velocity_on_nodes_(vs: List[np.ndarray], cache: BiotSavartCache, fs, **kws):
 
 
 def velocity_on_nodes(cache: BiotSavartCache, fs, **kws) -> List[np.ndarray]:
-    vs = [np.empty((len(f), 3)) for f in fs]
+    lens = fs.Ns if isinstance(fs, FilamentBatch) else [len(f) for f in fs]
+    vs = [np.empty((int(n), 3)) for n in lens]
     return velocity_on_nodes_(vs, cache, fs, **kws)

@@ -294,7 +294,224 @@ __global__ void local_integrals_kernel(Fil f, LocalQuad lq, Box box, const doubl
     }
 }
 
+// ----------------------------------------------------------------------------
+// Time stepping on the device (Timestepping: SSPRK33.jl:17-39, advect.jl:20-72).
+
+// Chordal knots of each filament: t_0 = 0, t_{i+1} = t_i + |X_{i+1} - X_i|, and the
+// period T including the closing segment to X_0 + offset (one thread per filament).
+__global__ void chordal_knots(Fil f, const double *__restrict__ X, double *__restrict__ ts, double *__restrict__ T) {
+    const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (fi >= f.Nf) return;
+    const int64_t off = f.off[fi];
+    const int N = f.N[fi];
+    const int64_t Np = f.Np;
+    double acc = 0.0;
+    ts[off] = 0.0;
+    for (int i = 0; i < N; i++) {
+        const int64_t a = off + i, b = (i == N - 1) ? off : a + 1;
+        double d2 = 0.0;
+        for (int d = 0; d < 3; d++) {
+            const double xb = X[d * Np + b] + (i == N - 1 ? f.Xoff[d * f.Nf + fi] : 0.0);
+            const double dd = xb - X[d * Np + a];
+            d2 = d2 + dd * dd;
+        }
+        acc = acc + sqrt(d2);
+        if (i < N - 1) ts[a + 1] = acc;
+    }
+    T[fi] = acc;
+}
+
+// Coefficients c of the periodic quintic interpolant of Y(t_i) = X_i - (t_i/T)·offset
+// (spline.jl:163-193), one thread per filament. Row i of the collocation system
+// holds B_{i-5+r,6}(t_i), r = 0..4, at columns (i-5+r) mod N; with unknowns
+// shifted by 3 it is a cyclic 5-band matrix with the largest value on the
+// diagonal. The band part is eliminated without pivoting (B-spline collocation
+// matrices are totally positive) and the periodic corners are added back through
+// the Woodbury identity with U = [e_0, e_1, e_{N-2}, e_{N-1}].
+// buf: 12 arrays of Np doubles: 5 band diagonals, then 7 right-hand sides
+// (3 components of Y, 4 columns of U).
+__global__ void spline_solve(Fil f, const double *__restrict__ X, double *__restrict__ buf, double *__restrict__ c) {
+    const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (fi >= f.Nf) return;
+    const int64_t off = f.off[fi], Np = f.Np;
+    const int N = f.N[fi];
+    const double T = f.T[fi];
+    double *band[5], *rhs[7];
+    for (int r = 0; r < 5; r++) band[r] = buf + r * Np + off;
+    for (int k = 0; k < 7; k++) rhs[k] = buf + (5 + k) * Np + off;
+    double c0a = 0, c0b = 0, c1 = 0, c2 = 0, c3a = 0, c3b = 0;  // periodic corners
+    for (int i = 0; i < N; i++) {
+        const double ti = f.ts[off + i];
+        double B[6];
+        basis<6>(f, off, N, T, i, ti, B);
+        for (int r = 0; r < 5; r++) band[r][i] = B[r];
+        if (i == 0) {
+            c0a = B[0], c0b = B[1];
+            band[0][i] = band[1][i] = 0.0;
+        }
+        if (i == 1) {
+            c1 = B[0];
+            band[0][i] = 0.0;
+        }
+        if (i == N - 2) {
+            c2 = B[4];
+            band[4][i] = 0.0;
+        }
+        if (i == N - 1) {
+            c3a = B[3], c3b = B[4];
+            band[3][i] = band[4][i] = 0.0;
+        }
+        for (int d = 0; d < 3; d++) rhs[d][i] = X[d * Np + off + i] - (ti / T) * f.Xoff[d * f.Nf + fi];
+        rhs[3][i] = i == 0 ? 1.0 : 0.0;
+        rhs[4][i] = i == 1 ? 1.0 : 0.0;
+        rhs[5][i] = i == N - 2 ? 1.0 : 0.0;
+        rhs[6][i] = i == N - 1 ? 1.0 : 0.0;
+    }
+    // forward elimination (row i: band[r][i] at column i - 2 + r)
+    for (int k = 0; k < N; k++) {
+        const double piv = band[2][k];
+        for (int sft = 1; sft <= 2 && k + sft < N; sft++) {
+            const int i = k + sft;
+            const double l = band[2 - sft][i] / piv;
+            band[2 - sft][i] = 0.0;
+            band[3 - sft][i] -= l * band[3][k];
+            if (k + 2 < N) band[4 - sft][i] -= l * band[4][k];
+            for (int q = 0; q < 7; q++) rhs[q][i] -= l * rhs[q][k];
+        }
+    }
+    // back substitution, in place in rhs
+    for (int k = N - 1; k >= 0; k--) {
+        for (int q = 0; q < 7; q++) {
+            double v = rhs[q][k];
+            if (k + 1 < N) v -= band[3][k] * rhs[q][k + 1];
+            if (k + 2 < N) v -= band[4][k] * rhs[q][k + 2];
+            rhs[q][k] = v / band[2][k];
+        }
+    }
+    // Woodbury: x = z - Z (I + Vᵀ Z)⁻¹ Vᵀ z, Vᵀ rows = the corner entries
+    auto vt = [&](int a, const double *x) -> double {
+        switch (a) {
+            case 0: return c0a * x[N - 2] + c0b * x[N - 1];
+            case 1: return c1 * x[N - 1];
+            case 2: return c2 * x[0];
+            default: return c3a * x[0] + c3b * x[1];
+        }
+    };
+    double M[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) M[a][b] = (a == b ? 1.0 : 0.0) + vt(a, rhs[3 + b]);
+    double g[4][3];
+    for (int a = 0; a < 4; a++)
+        for (int d = 0; d < 3; d++) g[a][d] = vt(a, rhs[d]);
+    // solve M w = g (4×4, partial pivoting)
+    int perm[4] = {0, 1, 2, 3};
+    for (int k = 0; k < 4; k++) {
+        int p = k;
+        for (int r = k + 1; r < 4; r++)
+            if (fabs(M[perm[r]][k]) > fabs(M[perm[p]][k])) p = r;
+        const int tmp = perm[k];
+        perm[k] = perm[p];
+        perm[p] = tmp;
+        for (int r = k + 1; r < 4; r++) {
+            const double l = M[perm[r]][k] / M[perm[k]][k];
+            for (int b = k; b < 4; b++) M[perm[r]][b] -= l * M[perm[k]][b];
+            for (int d = 0; d < 3; d++) g[perm[r]][d] -= l * g[perm[k]][d];
+        }
+    }
+    double w[4][3];
+    for (int k = 3; k >= 0; k--)
+        for (int d = 0; d < 3; d++) {
+            double v = g[perm[k]][d];
+            for (int b = k + 1; b < 4; b++) v -= M[perm[k]][b] * w[b][d];
+            w[k][d] = v / M[perm[k]][k];
+        }
+    // unknown m = (j + 3) mod N holds coefficient j
+    for (int j = 0; j < N; j++) {
+        int m = j + 3;
+        if (m >= N) m -= N;
+        for (int d = 0; d < 3; d++) {
+            double v = rhs[d][m];
+            for (int b = 0; b < 4; b++) v -= rhs[3 + b][m] * w[b][d];
+            c[d * Np + off + j] = v;
+        }
+    }
+}
+
+// out = x0 + a·v over 3·Np values
+__global__ void axpy_nodes(int64_t n, const double *__restrict__ x0, double a, const double *__restrict__ v,
+                           double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = x0[i] + a * v[i];
+}
+
+// SSPRK33 velocity combinations: mode 0: vs = v1 + v2; mode 1: vs = (vs + 4·v3)/6
+__global__ void rk_combine(int64_t n, int mode, double *__restrict__ vs, const double *__restrict__ v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    vs[i] = mode == 0 ? vs[i] + v[i] : (vs[i] + 4.0 * v[i]) / 6.0;
+}
+
 }  // namespace
+
+Fil filament_view(const vpbs_ctx *c) {
+    Fil f;
+    f.fid = c->fil_fid;
+    f.off = c->fil_off;
+    f.N = c->fil_n;
+    f.T = c->fil_T;
+    f.Xoff = c->fil_xoff;
+    f.ts = c->fil_ts;
+    f.Np = c->Np;
+    f.Nf = c->pd_nf;
+    return f;
+}
+
+// Point data from the filament state on the device (nodes, knots, periods,
+// offsets, spline coefficients), with the quadrature set by vpbs_set_filaments.
+int point_data_from_state(vpbs_ctx *c) {
+    hipStream_t s = c->s_sr;
+    const Fil f = filament_view(c);
+    const int64_t Np = c->Np;
+    Quad qd{};
+    qd.n = c->p.nquad;
+    qd.straight = c->pd_straight;
+    qd.gamma = std::isnan(c->pd_gamma) ? 1.0 : c->pd_gamma;
+    for (int q = 0; q < qd.n && !qd.straight; q++) {
+        qd.z[q] = c->pd_z[q];
+        qd.w[q] = c->pd_w[q];
+    }
+    if (std::isnan(c->pd_gamma)) {
+        qd.lim_m[0] = 0.0;
+        qd.lim_m[1] = 1.0;
+        qd.lim_p[0] = 0.0;
+        qd.lim_p[1] = 1.0;
+    } else {  // lia_integration_limits (local_integrals.jl:100-101)
+        const double g = c->pd_gamma;
+        qd.lim_m[0] = 1.0 - g;
+        qd.lim_m[1] = 1.0;
+        qd.lim_p[0] = 0.0;
+        qd.lim_p[1] = g;
+    }
+    c->have_points = false;
+    stage_begin(c, ST_POINTS, s);
+    const unsigned nb = nblocks(Np, 256);
+    derivative_coefs<<<nb, 256, 0, s>>>(f, c->fil_c, c->fil_d);
+    second_derivative_coefs<<<nb, 256, 0, s>>>(f, c->fil_d, c->fil_e);
+    point_data_kernel<<<nblocks(Np, 128), 128, 0, s>>>(f, qd, c->box, c->fil_x, c->fil_c, c->fil_d, c->fil_e,
+                                                       c->nodes, c->idx_prev, c->sp, c->spp, c->dlm, c->dlp,
+                                                       c->points, c->charges);
+    VPBS_HIP(c, hipGetLastError());
+    stage_end(c, ST_POINTS, s);
+    VPBS_HIP(c, hipStreamSynchronize(s));
+    VPBS_TRY(device_check(c));
+    float ms = 0.f;
+    VPBS_HIP(c, hipEventElapsedTime(&ms, c->ev_beg[ST_POINTS], c->ev_end[ST_POINTS]));
+    c->timings.point_data = ms;
+    c->have_points = true;
+    c->have_stats = false;
+    c->last_vel = c->last_psi = false;
+    return VPBS_OK;
+}
 
 // Upload the filament description, then derive the point data on the device.
 // Argument meaning as in vpbs_set_filaments (include/vpbs.h); the host checks
@@ -375,53 +592,14 @@ int build_point_data(vpbs_ctx *c, int64_t Nf, const int64_t *counts, const doubl
     VPBS_HIP(c, hipMemcpyAsync(c->fil_T, periods, sizeof(double) * (size_t)Nf, hipMemcpyHostToDevice, s));
     VPBS_HIP(c, hipMemcpyAsync(c->fil_xoff, offsets, sizeof(double) * 3 * (size_t)Nf, hipMemcpyHostToDevice, s));
 
-    Fil f;
-    f.fid = c->fil_fid;
-    f.off = c->fil_off;
-    f.N = c->fil_n;
-    f.T = c->fil_T;
-    f.Xoff = c->fil_xoff;
-    f.ts = c->fil_ts;
-    f.Np = Np;
-    f.Nf = Nf;
-    Quad qd{};
-    qd.n = nq;
-    qd.straight = straight;
-    qd.gamma = std::isnan(lia_segment_fraction) ? 1.0 : lia_segment_fraction;
+    // quadrature and local-integral configuration, kept for rebuilds from the device state
+    c->pd_nf = Nf;
+    c->pd_straight = straight;
+    c->pd_gamma = lia_segment_fraction;
     for (int q = 0; q < nq && !straight; q++) {
-        qd.z[q] = quad_nodes[q];
-        qd.w[q] = quad_weights[q];
+        c->pd_z[q] = quad_nodes[q];
+        c->pd_w[q] = quad_weights[q];
     }
-    if (std::isnan(lia_segment_fraction)) {
-        qd.lim_m[0] = 0.0;
-        qd.lim_m[1] = 1.0;
-        qd.lim_p[0] = 0.0;
-        qd.lim_p[1] = 1.0;
-    } else {  // lia_integration_limits (local_integrals.jl:100-101)
-        const double g = lia_segment_fraction;
-        qd.lim_m[0] = 1.0 - g;
-        qd.lim_m[1] = 1.0;
-        qd.lim_p[0] = 0.0;
-        qd.lim_p[1] = g;
-    }
-    stage_begin(c, ST_POINTS, s);
-    const unsigned nb = nblocks(Np, 256);
-    node_filament<<<nb, 256, 0, s>>>(c->fil_off, Nf, Np, c->fil_fid);
-    derivative_coefs<<<nb, 256, 0, s>>>(f, c->fil_c, c->fil_d);
-    second_derivative_coefs<<<nb, 256, 0, s>>>(f, c->fil_d, c->fil_e);
-    point_data_kernel<<<nblocks(Np, 128), 128, 0, s>>>(f, qd, c->box, c->fil_x, c->fil_c, c->fil_d, c->fil_e,
-                                                       c->nodes, c->idx_prev, c->sp, c->spp, c->dlm, c->dlp,
-                                                       c->points, c->charges);
-    VPBS_HIP(c, hipGetLastError());
-    stage_end(c, ST_POINTS, s);
-    VPBS_HIP(c, hipStreamSynchronize(s));
-    c->Np = Np;
-    c->Nq = Nq;
-    VPBS_TRY(device_check(c));
-    float ms = 0.f;
-    VPBS_HIP(c, hipEventElapsedTime(&ms, c->ev_beg[ST_POINTS], c->ev_end[ST_POINTS]));
-    c->timings.point_data = ms;
-    // near-singular local integrals, evaluated in vpbs_compute (lia_segment_fraction set)
     c->local_ns = (!std::isnan(lia_segment_fraction) && nquad_ns > 0) ? nquad_ns : 0;
     c->local_gamma = lia_segment_fraction;
     c->local_nf = Nf;
@@ -429,24 +607,18 @@ int build_point_data(vpbs_ctx *c, int64_t Nf, const int64_t *counts, const doubl
         c->local_z[q] = quad_ns_nodes[q];
         c->local_w[q] = quad_ns_weights[q];
     }
-    c->have_points = true;
-    c->have_stats = false;
-    c->last_vel = c->last_psi = false;
-    return VPBS_OK;
+    node_filament<<<nblocks(Np, 256), 256, 0, s>>>(c->fil_off, Nf, Np, c->fil_fid);
+    VPBS_HIP(c, hipGetLastError());
+    c->Np = Np;
+    c->Nq = Nq;
+    c->have_filaments = true;
+    return point_data_from_state(c);
 }
 
 int add_local_integrals(vpbs_ctx *c, uint32_t flags, hipStream_t s) {
     if (!c->local_ns) return VPBS_OK;
     const bool vel = flags & VPBS_VELOCITY, psi = flags & VPBS_STREAMFUNCTION;
-    Fil f;
-    f.fid = c->fil_fid;
-    f.off = c->fil_off;
-    f.N = c->fil_n;
-    f.T = c->fil_T;
-    f.Xoff = c->fil_xoff;
-    f.ts = c->fil_ts;
-    f.Np = c->Np;
-    f.Nf = c->local_nf;
+    const Fil f = filament_view(c);
     LocalQuad lq{};
     lq.n = c->local_ns;
     for (int q = 0; q < lq.n; q++) {
@@ -462,6 +634,56 @@ int add_local_integrals(vpbs_ctx *c, uint32_t flags, hipStream_t s) {
                                                               vel ? c->v_sr : nullptr, psi ? c->psi_sr : nullptr);
     VPBS_HIP(c, hipGetLastError());
     return VPBS_OK;
+}
+
+// SSPRK33 steps of the Biot–Savart velocity on the device-resident filaments
+// (Timestepping.jl:1038-1076 without forcing, reconnections or adaptivity).
+int advance_filaments(vpbs_ctx *c, double dt, int nsteps) {
+    const int64_t Np = c->Np, n3 = 3 * Np;
+    if (n3 > c->cap_step) {
+        if (c->step_buf) VPBS_HIP(c, hipFree(c->step_buf));
+        c->step_buf = nullptr;
+        VPBS_HIP(c, hipMalloc((void **)&c->step_buf, sizeof(double) * 3 * (size_t)n3));
+        c->cap_step = n3;
+    }
+    if (12 * Np > c->cap_solve) {
+        if (c->solve_buf) VPBS_HIP(c, hipFree(c->solve_buf));
+        c->solve_buf = nullptr;
+        VPBS_HIP(c, hipMalloc((void **)&c->solve_buf, sizeof(double) * 12 * (size_t)Np));
+        c->cap_solve = 12 * Np;
+    }
+    double *X0 = c->step_buf, *V1 = X0 + n3, *Vs = V1 + n3;
+    hipStream_t s = c->s_sr;
+    const Fil f = filament_view(c);
+    const unsigned nbn = nblocks(n3, 256), nbf = nblocks(c->pd_nf, 64);
+    auto velocity = [&]() -> int {  // node velocities into c->v_out from the current state
+        VPBS_TRY(point_data_from_state(c));
+        return vpbs_compute(c, VPBS_VELOCITY, __builtin_nan(""), nullptr, nullptr);
+    };
+    auto stage = [&](double a, const double *v) -> int {  // X = X0 + a·v, coefficients with the same knots
+        axpy_nodes<<<nbn, 256, 0, s>>>(n3, X0, a, v, c->fil_x);
+        spline_solve<<<nbf, 64, 0, s>>>(f, c->fil_x, c->solve_buf, c->fil_c);
+        VPBS_HIP(c, hipGetLastError());
+        return VPBS_OK;
+    };
+    for (int step = 0; step < nsteps; step++) {
+        VPBS_TRY(velocity());
+        VPBS_HIP(c, hipMemcpyAsync(X0, c->fil_x, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+        VPBS_HIP(c, hipMemcpyAsync(V1, c->v_out, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+        VPBS_TRY(stage(dt, V1));
+        VPBS_TRY(velocity());
+        VPBS_HIP(c, hipMemcpyAsync(Vs, V1, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+        rk_combine<<<nbn, 256, 0, s>>>(n3, 0, Vs, c->v_out);
+        VPBS_TRY(stage(dt / 4, Vs));
+        VPBS_TRY(velocity());
+        rk_combine<<<nbn, 256, 0, s>>>(n3, 1, Vs, c->v_out);
+        // final advection: new knots (chordal), then coefficients
+        axpy_nodes<<<nbn, 256, 0, s>>>(n3, X0, dt, Vs, c->fil_x);
+        chordal_knots<<<nbf, 64, 0, s>>>(f, c->fil_x, c->fil_ts, c->fil_T);
+        spline_solve<<<nbf, 64, 0, s>>>(f, c->fil_x, c->solve_buf, c->fil_c);
+        VPBS_HIP(c, hipGetLastError());
+    }
+    return point_data_from_state(c);  // point data of the final state
 }
 
 }  // namespace vpbs

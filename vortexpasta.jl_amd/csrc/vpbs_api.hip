@@ -112,7 +112,7 @@ static void release(vpbs_ctx *c) {
                     c->stat_hashes, c->lr_key,  c->lr_start, c->lr_active, c->lr_perm,   c->lr_i0,      c->lr_w,
                     c->sort_sr.rank, c->sort_sr.scan_tmp, c->sort_tg.rank, c->sort_tg.scan_tmp,
                     c->sort_lr.rank, c->sort_lr.scan_tmp, c->fil_x, c->fil_ts, c->fil_c, c->fil_d, c->fil_e,
-                    c->fil_fid, c->fil_n, c->fil_off, c->fil_T, c->fil_xoff};
+                    c->fil_fid, c->fil_n, c->fil_off, c->fil_T, c->fil_xoff, c->step_buf, c->solve_buf};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int st = 0; st < ST_COUNT; st++) {
@@ -253,6 +253,7 @@ static int set_points_impl(vpbs_ctx *c, int64_t Np, const double *nodes, const i
     c->Nq = Nq;
     c->timings.point_data = 0.0;
     c->local_ns = 0;  // no filament description on the device: local integrals stay with the caller
+    c->have_filaments = false;
     if (kind == hipMemcpyDeviceToDevice) VPBS_TRY(device_check(c));
     c->have_points = true;
     c->have_stats = false;
@@ -482,6 +483,27 @@ int vpbs_neighbour_stats(vpbs_ctx *c, int64_t *counts, uint64_t *hashes) {
     VPBS_HIP(c, hipSetDevice(c->device));
     if (counts) VPBS_HIP(c, hipMemcpy(counts, c->stat_counts, sizeof(int64_t) * c->Np, hipMemcpyDeviceToHost));
     if (hashes) VPBS_HIP(c, hipMemcpy(hashes, c->stat_hashes, sizeof(uint64_t) * c->Np, hipMemcpyDeviceToHost));
+    return VPBS_OK;
+}
+
+int vpbs_advance(vpbs_ctx *c, double dt, int32_t nsteps) {
+    if (!c) return VPBS_ERR_INVALID;
+    if (!c->have_filaments) return fail(c, VPBS_ERR_STATE, "no filaments on the device: call vpbs_set_filaments first");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(c, VPBS_ERR_INVALID, "time step must be positive and finite");
+    if (nsteps < 0) return fail(c, VPBS_ERR_INVALID, "number of steps must be >= 0");
+    VPBS_HIP(c, hipSetDevice(c->device));
+    return advance_filaments(c, dt, nsteps);
+}
+
+int vpbs_get_filaments(vpbs_ctx *c, double *X, double *knots, double *periods, double *coefs) {
+    if (!c) return VPBS_ERR_INVALID;
+    if (!c->have_filaments) return fail(c, VPBS_ERR_STATE, "no filaments on the device: call vpbs_set_filaments first");
+    VPBS_HIP(c, hipSetDevice(c->device));
+    const size_t d3 = sizeof(double) * 3 * (size_t)c->Np;
+    if (X) VPBS_HIP(c, hipMemcpy(X, c->fil_x, d3, hipMemcpyDeviceToHost));
+    if (coefs) VPBS_HIP(c, hipMemcpy(coefs, c->fil_c, d3, hipMemcpyDeviceToHost));
+    if (knots) VPBS_HIP(c, hipMemcpy(knots, c->fil_ts, sizeof(double) * (size_t)c->Np, hipMemcpyDeviceToHost));
+    if (periods) VPBS_HIP(c, hipMemcpy(periods, c->fil_T, sizeof(double) * (size_t)c->pd_nf, hipMemcpyDeviceToHost));
     return VPBS_OK;
 }
 

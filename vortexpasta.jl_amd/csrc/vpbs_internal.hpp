@@ -69,6 +69,15 @@ struct vpbs_ctx {
     int32_t *fil_fid = nullptr, *fil_n = nullptr;
     int64_t *fil_off = nullptr;
     double *fil_T = nullptr, *fil_xoff = nullptr;
+    bool have_filaments = false;   // filament state on the device (vpbs_set_filaments)
+    int64_t pd_nf = 0;             // filaments of that state
+    bool pd_straight = false;      // NoQuadrature
+    double pd_gamma = 0;           // lia_segment_fraction (NaN: none)
+    double pd_z[16]{}, pd_w[16]{}; // quadrature rule on [0, 1]
+    double *step_buf = nullptr;    // time stepping: X0, V1, Vs (3·Np each)
+    int64_t cap_step = 0;
+    double *solve_buf = nullptr;   // spline solve: 5 band + 7 right-hand sides per node
+    int64_t cap_solve = 0;
     // near-singular local integrals (lia_segment_fraction), set by vpbs_set_filaments
     int local_ns = 0;          // near-singularity quadrature points (0: not evaluated on the device)
     double local_gamma = 0;
@@ -183,6 +192,8 @@ int build_point_data(vpbs_ctx *c, int64_t Nf, const int64_t *counts, const doubl
                      const double *quad_weights, double lia_segment_fraction, int32_t nquad_ns,
                      const double *quad_ns_nodes, const double *quad_ns_weights);
 int add_local_integrals(vpbs_ctx *c, uint32_t flags, hipStream_t s);
+int point_data_from_state(vpbs_ctx *c);
+int advance_filaments(vpbs_ctx *c, double dt, int nsteps);
 
 // vpbs_api.hip
 int ensure_capacity(vpbs_ctx *c, int64_t Np, int64_t Nq);

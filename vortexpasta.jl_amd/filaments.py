@@ -210,7 +210,9 @@ class FilamentBatch:
     and offsets ``Xoff``.
     """
 
-    def __init__(self, fs: Sequence[ClosedFilament]):
+    def __init__(self, fs: Sequence[ClosedFilament], knots=None):
+        """``knots``: optional ``(ts, T)`` to keep instead of recomputing chordal knots
+        (the reference keeps the knots within Runge–Kutta stages, ``advect.jl:47-61``)."""
         if len(fs) == 0:
             raise ValueError("empty filament list")
         self.fs = list(fs)
@@ -224,8 +226,19 @@ class FilamentBatch:
         fid = np.repeat(np.arange(self.Nf), Ns)
         self.fid = fid
         self.local = np.arange(self.Np) - self.offs[fid]
-        self._compute_knots()
+        if knots is None:
+            self._compute_knots()
+        else:
+            self.ts = np.array(knots[0], dtype=np.float64)
+            self.T = np.array(knots[1], dtype=np.float64)
+            if self.ts.shape != (self.Np,) or self.T.shape != (self.Nf,):
+                raise ValueError("knots do not match the filaments")
         self._solve_coefficients()
+
+    def with_nodes(self, X: np.ndarray, keep_knots: bool) -> "FilamentBatch":
+        """Same filaments with node positions X (Np, 3); knots kept or recomputed (chordal)."""
+        fs = [ClosedFilament(X[a : a + n], f.offset) for f, a, n in zip(self.fs, self.offs, self.Ns)]
+        return FilamentBatch(fs, knots=(self.ts, self.T) if keep_knots else None)
 
     # per-node filament properties
     def _fN(self, fid):
