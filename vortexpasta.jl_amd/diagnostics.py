@@ -1,9 +1,12 @@
-"""Energy spectrum of the long-range field (device-side ``Diagnostics`` consumer).
+"""Diagnostics consuming the Biot–Savart outputs.
 
-Mirrors ``Diagnostics.init_spectrum`` / ``energy_spectrum`` / ``energy_spectrum!``
-(``src/Diagnostics/spectra.jl:49-115``): the spectrum is computed on the GPU from
-the Fourier field the last evaluation kept (``vpbs_energy_spectrum``), without
-copying that field to the host.
+* Energy spectrum of the long-range field, mirroring ``Diagnostics.init_spectrum`` /
+  ``energy_spectrum`` / ``energy_spectrum!`` (``src/Diagnostics/spectra.jl:49-115``):
+  computed on the GPU from the Fourier field the last evaluation kept
+  (``vpbs_energy_spectrum``), without copying that field to the host.
+* Kinetic energy from streamfunction values on nodes,
+  ``kinetic_energy_from_streamfunction`` without a quadrature rule
+  (``src/Diagnostics/energy.jl:104-119``): ``E = Γ/(2V) Σ_i ψ_i·s′_i (t_{i+1} − t_{i−1})/2``.
 """
 
 from __future__ import annotations
@@ -14,7 +17,7 @@ import numpy as np
 
 from .biotsavart import BiotSavartCache
 
-__all__ = ["init_spectrum", "energy_spectrum", "energy_spectrum_"]
+__all__ = ["init_spectrum", "energy_spectrum", "energy_spectrum_", "kinetic_energy_from_streamfunction"]
 
 
 def init_spectrum(cache: BiotSavartCache) -> Tuple[np.ndarray, np.ndarray]:
@@ -42,3 +45,26 @@ def energy_spectrum_(Ek: np.ndarray, ks: np.ndarray, cache: BiotSavartCache):
 def energy_spectrum(cache: BiotSavartCache) -> Tuple[np.ndarray, np.ndarray]:
     ks, Ek = init_spectrum(cache)
     return energy_spectrum_(Ek, ks, cache)
+
+
+def kinetic_energy_from_streamfunction(batch, psi: np.ndarray, Γ: float, Ls=None, sp: np.ndarray = None) -> float:
+    """Kinetic energy per unit mass from streamfunction values on nodes (``energy.jl:104-119``).
+
+    ``batch``: the filaments (a ``FilamentBatch``); ``psi``: (3, Np) or (Np, 3)
+    streamfunction on the nodes in batch order; ``Ls``: periods (``None`` or an
+    infinite period gives V = 1, ``energy.jl:70-75``); ``sp``: s′ on the nodes,
+    (3, Np), evaluated from the splines when not given. The trapezoidal weight of
+    node i is (t_{i+1} − t_{i−1})/2 with the filament's knots.
+    """
+    psi = np.asarray(psi, dtype=np.float64)
+    if psi.shape == (batch.Np, 3):
+        psi = psi.T
+    if psi.shape != (3, batch.Np):
+        raise ValueError(f"psi must have shape (3, {batch.Np}), got {psi.shape}")
+    if sp is None:
+        sp = batch.evaluate(batch.fid, batch.local, np.zeros(batch.Np), deriv=1).T
+    dt = (batch.knot(batch.fid, batch.local + 1) - batch.knot(batch.fid, batch.local - 1)) / 2
+    V = 1.0
+    if Ls is not None and all(np.isfinite(L) for L in Ls):
+        V = float(np.prod(Ls))
+    return Γ / (2 * V) * float(np.sum(np.einsum("dn,dn->n", psi, sp) * dt))
