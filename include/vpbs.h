@@ -39,6 +39,10 @@ extern "C" {
 #define VPBS_F64 0 /* plain FP64 path */
 #define VPBS_F32 1 /* FP32 bulk, FP64 node differencing / LIA / local self terms */
 
+/* transforms of the long-range part (vpbs_params.fft) */
+#define VPBS_FFT_FUSED 0  /* hand-written FFTs fused with the pencil reorderings and the Fourier-space kernel */
+#define VPBS_FFT_ROCFFT 1 /* rocFFT 1D transforms with separate reordering and Fourier-space kernels */
+
 /* vpbs_compute flags */
 #define VPBS_VELOCITY 1u
 #define VPBS_STREAMFUNCTION 2u
@@ -76,7 +80,8 @@ typedef struct vpbs_params {
     int32_t avoid_explicit_erf; /* params.jl:289-298 */
     int32_t truncate_spherical; /* params.jl:243-246 */
     int32_t device;        /* HIP device ordinal */
-    int32_t reserved;
+    int32_t fft;           /* VPBS_FFT_FUSED (default) or VPBS_FFT_ROCFFT; grids whose sizes the
+                              fused transforms do not cover (> 6144 FP32 / 3072 FP64 per axis) use rocFFT */
 } vpbs_params;
 
 /* Per-stage GPU times of the last vpbs_compute, in milliseconds (HIP events). */
@@ -88,9 +93,9 @@ typedef struct vpbs_timings {
     double self;         /* self-interaction removal */
     double lr_sort;      /* NUFFT tile sort of quadrature points */
     double spread;       /* type-1 spreading (all 3 components) */
-    double fft_forward;  /* rocFFT r2c, 3 components */
-    double fourier;      /* fused deconvolution / Green / curl / Ewald kernel */
-    double fft_backward; /* rocFFT c2r, all components */
+    double fft_forward;  /* x and y transforms (fused: with the pencil reorderings), 3 components */
+    double fourier;      /* Fourier-space kernel (fused: with the forward and inverse z transforms) */
+    double fft_backward; /* inverse y and x transforms of the velocity */
     double interp;       /* type-2 interpolation */
     double merge;        /* SR + LR sum */
     int64_t pair_candidates; /* unused (reserved) */

@@ -577,3 +577,56 @@ def test_device_stepper_errors(vp, cfg1):
     ctx.advance(0.01, 0)  # no steps: state unchanged
     assert np.array_equal(ctx.get_filaments()["X"], vp.FilamentBatch(cfg1).X)
     ctx.close()
+
+
+# ---------------------------------------------------------------------------
+# Transform paths: the fused hand-written FFTs (default, every test above) and rocFFT.
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+def test_fft_paths_agree(vp, O, cfg2, T):
+    """Velocity and streamfunction from the fused transforms and from rocFFT on cfg2; FP64 also vs the oracle."""
+    outs = {}
+    for fft in ("fused", "rocfft"):
+        params = _params(vp, T=T, Ns=128, backend_long=vp.HIPLongRangeBackend(fft=fft))
+        cache, outs[fft] = _gpu(vp, params, cfg2, longrange=True, shortrange=False, LIA=False)
+        cache.close()
+    tol = 1e-12 if T is np.float64 else 2e-6
+    for k in Q:
+        err = rel_l2(outs["fused"][k], outs["rocfft"][k])
+        print(f"  {T.__name__} {k}: fused vs rocFFT {err:.2e}", flush=True)
+        assert err <= tol
+    if T is np.float64:
+        params = _params(vp, Ns=128)
+        want = O.compute(vp.add_point_charges(cfg2, params), params, Q, LIA=False, shortrange=False)
+        for k in Q:
+            assert rel_l2(outs["fused"][k], want[k]) <= TOL64
+
+
+@pytest.mark.parametrize("variant", ["cfg1", "trefoil_noncubic", "tiny"])
+def test_rocfft_path(vp, O, cfg1, variant):
+    """The rocFFT transform path against the oracle (FP64, 1e-12)."""
+    bl = vp.HIPLongRangeBackend(fft="rocfft")
+    if variant == "cfg1":
+        _check(vp, O, _params(vp, backend_long=bl), cfg1)
+    elif variant == "tiny":
+        fs = vp.tangles.random_rings(3, 0.8, 1.2, TWO_PI / 24, seed=2)
+        spl = vp.GaussianSplitting.create(Ls=TWO_PI, α=1.0, rcut=2.0, Ns=10)
+        _check(vp, O, vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, splitting=spl, backend_long=bl), fs)
+    else:
+        R = math.pi / 3
+        S = vp.define_curve(vp.TrefoilKnot(), scale=R, translate=R)
+        fs = [vp.ClosedFilament(S(np.arange(30) / 30), np.zeros(3))]
+        spl = vp.GaussianSplitting.create(Ls=(1.5 * math.pi, 1.5 * math.pi, 2 * math.pi), α=3.0, rcut=1.0,
+                                         Ns=(45, 45, 61))
+        _check(vp, O, vp.ParamsBiotSavart.build(np.float64, Γ=7.4, a=1e-5, splitting=spl, backend_long=bl), fs)
+
+
+def test_rocfft_path_uhat_and_band(vp, O, cfg2):
+    """rocFFT path: kept field and band interpolation vs the oracle (same layout of the kept field)."""
+    params, cache, _ = _uhat_case(vp, cfg2, Ns=128, backend_long=vp.HIPLongRangeBackend(fft="rocfft"))
+    pd = vp.add_point_charges(cfg2, params)
+    u = O.longrange_fourier(pd, params, "velocity")
+    assert rel_l2(np.stack(vp.get_longrange_field_fourier(cache)["field"]), u) <= TOL64
+    got = cache.ctx.interpolate_band(2.0, 6.0)
+    assert rel_l2(got, O.band_field(u, params, pd.nodes, 2.0, 6.0, "nufft")) <= TOL64
+    cache.close()

@@ -26,6 +26,7 @@ NO_SHORTRANGE = 16
 NO_LONGRANGE = 32
 NEIGHBOUR_STATS = 64
 KEEP_UHAT = 128
+FFT_FUSED, FFT_ROCFFT = 0, 1  # vpbs_params.fft
 UHAT_VELOCITY = 1
 UHAT_STREAMFUNCTION = 2
 
@@ -69,7 +70,7 @@ class VpbsParams(ctypes.Structure):
         ("avoid_explicit_erf", ctypes.c_int32),
         ("truncate_spherical", ctypes.c_int32),
         ("device", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("fft", ctypes.c_int32),
     ]
 
 
@@ -180,6 +181,7 @@ def make_params(params, device: int = 0) -> VpbsParams:
     p.avoid_explicit_erf = int(params.avoid_explicit_erf)
     p.truncate_spherical = int(params.longrange_truncate_spherical)
     p.device = device
+    p.fft = FFT_ROCFFT if params.backend_long.fft == "rocfft" else FFT_FUSED
     return p
 
 

@@ -515,7 +515,7 @@ struct FourierArgs {
     double kmax2_trunc;          // < 0: no spherical truncation
     int do_vel, do_psi;
     // unsmoothed field kept for the Fourier-space consumers (velocity if do_vel, else streamfunction):
-    // uhat[3][nkx][nky][nkz], kz index jz ≤ kmax2 → kz = jz, else jz - nkz
+    // uhat[3][nky][nkz][nkx], kz index jz ≤ kmax2 → kz = jz, else jz - nkz
     CT *uhat;
     int keep_uhat, nkz;
     int64_t ucstride;
@@ -541,7 +541,7 @@ __global__ void fourier_kernel(FourierArgs<CT> a, CT *__restrict__ spec, CT *__r
     CT *cu = nullptr;
     if (a.keep_uhat && kz >= -a.kmax[2] && kz <= a.kmax[2]) {
         const int jz = kz >= 0 ? kz : kz + a.nkz;
-        cu = a.uhat + 2 * (((int64_t)kx * a.nky + j) * a.nkz + jz);
+        cu = a.uhat + 2 * (((int64_t)j * a.nkz + jz) * a.nkx + kx);
     }
     if (!keep || k2 == 0) {
         if (cu) {
@@ -640,7 +640,7 @@ __global__ void band_to_zpencils(FourierArgs<CT> a, double kmin2, double kmax2, 
     }
     const double D = a.dx[kx] * a.dy[ky + a.kmax[1]] * a.dz[kz + a.kmax[2]];
     const int jz = kz >= 0 ? kz : kz + a.nkz;
-    const CT *u = a.uhat + 2 * (((int64_t)kx * a.nky + j) * a.nkz + jz);
+    const CT *u = a.uhat + 2 * (((int64_t)j * a.nkz + jz) * a.nkx + kx);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         o[d * cs] = (CT)((double)u[d * us] * D);
@@ -653,7 +653,7 @@ __global__ void band_to_zpencils(FourierArgs<CT> a, double kmin2, double kmax2, 
 // plane, 1 elsewhere; from the streamfunction |k × ψ̂|² = k²|ψ̂|²). One partial
 // spectrum per block (LDS bins), summed over blocks in a fixed order.
 template <typename CT>
-__global__ void spectrum_partial(const CT *__restrict__ uhat, int64_t ucs, int nky, int nkz, int K1, int K2,
+__global__ void spectrum_partial(const CT *__restrict__ uhat, int64_t ucs, int nkx, int nkz, int K1, int K2,
                                  double kx_scale, double ky_scale, double kz_scale, double dk_inv, int nk,
                                  int from_psi, double *__restrict__ partial) {
     extern __shared__ double bins[];
@@ -661,10 +661,10 @@ __global__ void spectrum_partial(const CT *__restrict__ uhat, int64_t ucs, int n
     __syncthreads();
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < ucs;
          idx += (int64_t)gridDim.x * blockDim.x) {
-        const int jz = (int)(idx % nkz);
-        const int j = (int)((idx / nkz) % nky);
-        const int kx = (int)(idx / ((int64_t)nkz * nky));
-        const int ky = j <= K1 ? j : j - nky;
+        const int kx = (int)(idx % nkx);
+        const int jz = (int)((idx / nkx) % nkz);
+        const int j = (int)(idx / ((int64_t)nkx * nkz));
+        const int ky = j <= K1 ? j : j - (2 * K1 + 1);
         const int kz = jz <= K2 ? jz : jz - nkz;
         const double KX = kx_scale * kx, KY = ky_scale * ky, KZ = kz_scale * kz;
         const double k2 = KX * KX + KY * KY + KZ * KZ;
@@ -872,6 +872,7 @@ LRGeom lr_geom(const vpbs_ctx *c) {
 // pencil transposes between them).
 template <typename CT>
 int inverse_to_grid(vpbs_ctx *c, Cx<CT> *zp) {
+    if (c->fft_fused) return spectral_inverse(c, zp);
     hipStream_t s = c->s_lr;
     const int G0 = c->G[0], G1 = c->G[1], G2 = c->G[2];
     const int nkx = c->nkx, nky = c->nky, K1 = c->kmax_mode[1];
@@ -914,6 +915,27 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
     launch_spread<CT>(g, nb_spread, Nq, c->lr_start, c->lr_active, c->lr_i0, (const CT *)c->lr_w, grid, s);
     stage_end(c, ST_SPREAD, s);
     VPBS_HIP(c, hipGetLastError());
+
+    if (c->fft_fused) {
+        stage_begin(c, ST_FFTF, s);
+        VPBS_TRY(spectral_forward(c));
+        stage_end(c, ST_FFTF, s);
+        stage_begin(c, ST_FOURIER, s);
+        VPBS_TRY(spectral_fourier(c, flags));
+        stage_end(c, ST_FOURIER, s);
+        stage_begin(c, ST_FFTB, s);
+        if (vel) VPBS_TRY(spectral_inverse(c, c->zpen));
+        stage_end(c, ST_FFTB, s);
+        stage_begin(c, ST_INTERP, s);
+        if (vel) launch_interp<CT>(g, Np, c->nodes, beta, grid, c->v_lr, s);
+        if (psi) {
+            VPBS_TRY(spectral_inverse(c, c->zpen2));
+            launch_interp<CT>(g, Np, c->nodes, beta, grid, c->psi_lr, s);
+        }
+        stage_end(c, ST_INTERP, s);
+        VPBS_HIP(c, hipGetLastError());
+        return VPBS_OK;
+    }
 
     const int G0 = c->G[0], G1 = c->G[1], G2 = c->G[2];
     const int nkx = c->nkx, nky = c->nky, K1 = c->kmax_mode[1];
@@ -1019,6 +1041,13 @@ int longrange_setup(vpbs_ctx *c) {
     VPBS_HIP(c, hipMalloc(&c->ypen, 2 * es * 3 * (size_t)c->ypen_stride));
     VPBS_HIP(c, hipMalloc(&c->zpen, 2 * es * 3 * (size_t)c->zpen_stride));
 
+    c->fft_fused = c->p.fft == VPBS_FFT_FUSED && spectral_supported(c);
+    if (c->fft_fused) {
+        VPBS_TRY(spectral_setup(c));
+        c->plans_ready = true;
+        return VPBS_OK;
+    }
+
     // rocFFT: in-place batched contiguous c2c along each direction
     const rocfft_precision prec = c->p.precision == VPBS_F32 ? rocfft_precision_single : rocfft_precision_double;
     const size_t len[3] = {(size_t)c->G[0], (size_t)c->G[1], (size_t)c->G[2]};
@@ -1073,10 +1102,14 @@ namespace {
 template <typename CT>
 int band_impl(vpbs_ctx *c, double kmin, double kmax, double *v_out) {
     hipStream_t s = c->s_lr;
-    FourierArgs<CT> fa = fourier_args<CT>(c);
-    band_to_zpencils<CT><<<nblocks(fa.cstride, 256), 256, 0, s>>>(fa, kmin * kmin, kmax * kmax, (CT *)c->zpen);
-    VPBS_HIP(c, hipGetLastError());
-    VPBS_TRY(inverse_to_grid<CT>(c, (Cx<CT> *)c->zpen));
+    if (c->fft_fused) {
+        VPBS_TRY(spectral_band(c, kmin * kmin, kmax * kmax));
+    } else {
+        FourierArgs<CT> fa = fourier_args<CT>(c);
+        band_to_zpencils<CT><<<nblocks(fa.cstride, 256), 256, 0, s>>>(fa, kmin * kmin, kmax * kmax, (CT *)c->zpen);
+        VPBS_HIP(c, hipGetLastError());
+        VPBS_TRY(inverse_to_grid<CT>(c, (Cx<CT> *)c->zpen));
+    }
     launch_interp<CT>(lr_geom(c), c->Np, c->nodes, (CT)c->es_beta, (const CT *)c->grid, c->v_lr, s);
     VPBS_HIP(c, hipGetLastError());
     VPBS_HIP(c, hipMemcpyAsync(v_out, c->v_lr, sizeof(double) * 3 * (size_t)c->Np, hipMemcpyDeviceToHost, s));
@@ -1092,7 +1125,7 @@ int spectrum_impl(vpbs_ctx *c, int nk, double dk, double *Ek) {
     VPBS_HIP(c, hipMallocAsync((void **)&partial, sizeof(double) * ((size_t)nb * nk + nk), s));
     out = partial + (size_t)nb * nk;
     spectrum_partial<CT><<<nb, bs, sizeof(double) * nk, s>>>(
-        (const CT *)c->uhat, c->uhat_stride, c->nky, c->nkz, c->kmax_mode[1], c->kmax_mode[2], 2 * M_PI / c->p.Ls[0],
+        (const CT *)c->uhat, c->uhat_stride, c->nkx, c->nkz, c->kmax_mode[1], c->kmax_mode[2], 2 * M_PI / c->p.Ls[0],
         2 * M_PI / c->p.Ls[1], 2 * M_PI / c->p.Ls[2], 1.0 / dk, nk, c->uhat_quantity == VPBS_UHAT_STREAMFUNCTION,
         partial);
     spectrum_reduce<<<nblocks(nk, 256), 256, 0, s>>>(partial, nb, nk, out);
@@ -1120,7 +1153,7 @@ int get_uhat_impl(vpbs_ctx *c, double *out) {
                 for (int jz = 0; jz < c->nkz; jz++) {
                     const int kz = jz <= K2 ? jz : jz - c->nkz;
                     const int64_t iz = kz >= 0 ? kz : kz + N2;
-                    const int64_t src = d * c->uhat_stride + ((int64_t)kx * c->nky + j) * c->nkz + jz;
+                    const int64_t src = d * c->uhat_stride + ((int64_t)j * c->nkz + jz) * c->nkx + kx;
                     const int64_t dst = d * K + (iz * N1 + iy) * Nkx + kx;
                     out[2 * dst] = (double)h[2 * src];
                     out[2 * dst + 1] = (double)h[2 * src + 1];
@@ -1162,6 +1195,7 @@ int longrange_band(vpbs_ctx *c, double kmin, double kmax, double *v_out) {
 }
 
 void longrange_release(vpbs_ctx *c) {
+    spectral_release(c);
     for (auto &pd : c->plan)
         for (auto &p : pd) {
             if (p) rocfft_plan_destroy(p);

@@ -1,0 +1,897 @@
+// Spectral part of the long-range NUFFT on gfx950: hand-written FFTs fused with
+// the pencil reorderings and with the Fourier-space kernel.
+//
+// The real spreading grid holds pairs of y-rows as one complex row (pair p:
+// value (x, y, z) at ((z·G1/2 + y/2)·G0 + x)·2 + (y & 1)), so one complex
+// transform along x serves two real rows. Five kernels, each a single pass over
+// HBM with the transform done in LDS:
+//
+//   spec_x_forward  pair rows → FFT_x → A_k = (Z_k + conj Z_-k)/2, B_k = (Z_k - conj Z_-k)/2i
+//                   for the retained kx ∈ [0, nkx) → x1[c][z][y][kx]
+//   spec_y_forward  x1 tiles [y][kx-block] → FFT_y → retained ky → x2[c][jy][z][kx]
+//   spec_z_fourier  x2 tiles [z][kx-block] of the 3 components → FFT_z → deconvolution,
+//                   Γ/V, Green function 1/k², Ewald factor e^{-k²/4α²}, curl i k ×
+//                   (velocity) and/or nothing (streamfunction) on the retained modes,
+//                   optionally the unsmoothed field into uhat → inverse FFT_z → x2 (velocity)
+//                   and x2b (streamfunction), in z space
+//   spec_y_inverse  x2 → zero-padded ky rows → inverse FFT_y → x1
+//   spec_x_inverse  x1 → pair rows Z_k = A_k + i B_k (Hermitian half rebuilt, DC real)
+//                   → inverse FFT_x → real grid
+//
+// These restate the transforms of the reference's NUFFT back end
+// (longrange.jl:488-495 type 1, 360-378 type 2, with the Fourier-space operators
+// of longrange.jl:102-156 and the Ewald factor of cache_common.jl:64-76,
+// restricted to the modes kept by longrange.jl:556-583).
+//
+// Transforms: Stockham autosort (decimation in time) over mixed radices
+// {16, 8, 4, 2, 3, 5} (every grid size is 32·2^a·3^b·5^c), done on a tile of B
+// columns in LDS, S[row·B + b]: consecutive lanes take consecutive columns, so
+// LDS accesses run along rows. Each pass reads its R inputs per butterfly into
+// registers, synchronises, applies twiddles from an LDS table e^{-2πin/L}, does
+// the R-point DFT and writes the outputs back in place. Tiles hold at most 6144
+// (FP32) or 3072 (FP64) complex values: B = that / L, a power of two.
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "vpbs_internal.hpp"
+
+namespace vpbs {
+namespace {
+
+// threads per block of the x and y kernels (NTP) and of the z/Fourier kernels (NTZ)
+constexpr int NTP = 512;
+constexpr int NTZ = 512;
+
+template <typename CT>
+constexpr int spec_tile() {
+    return sizeof(CT) == 4 ? 6144 : 3072;
+}
+
+template <typename CT>
+struct C2 {
+    CT re, im;
+};
+
+template <typename CT>
+__device__ __forceinline__ C2<CT> cadd(C2<CT> a, C2<CT> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+template <typename CT>
+__device__ __forceinline__ C2<CT> csub(C2<CT> a, C2<CT> b) {
+    return {a.re - b.re, a.im - b.im};
+}
+template <typename CT>
+__device__ __forceinline__ C2<CT> cmul(C2<CT> a, C2<CT> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// multiply by S·i (S = ±1)
+template <int S, typename CT>
+__device__ __forceinline__ C2<CT> mul_si(C2<CT> a) {
+    return S > 0 ? C2<CT>{-a.im, a.re} : C2<CT>{a.im, -a.re};
+}
+template <typename CT>
+__device__ __forceinline__ C2<CT> cscale(C2<CT> a, CT s) {
+    return {a.re * s, a.im * s};
+}
+
+// Dynamic LDS of every kernel here: the tile(s), then the twiddle table. The
+// transform routines address it through this symbol with element offsets, so
+// that their accesses compile to LDS instructions also out of line.
+extern __shared__ __align__(16) unsigned char spec_smem[];
+
+template <typename CT>
+__device__ __forceinline__ C2<CT> *lds(int off) {
+    return reinterpret_cast<C2<CT> *>(spec_smem) + off;
+}
+
+struct AxisPlan {
+    int L;
+    int npass;
+    int radix[6];
+};
+
+// ---------------------------------------------------------------------------
+// Small DFTs in registers, X_k = Σ_n x_n e^{S·2πi nk/R} (S = -1 forward, +1 inverse).
+
+template <int S, typename CT>
+__device__ __forceinline__ void dft2(C2<CT> (&v)[2]) {
+    const C2<CT> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <int S, typename CT>
+__device__ __forceinline__ void dft3(C2<CT> (&v)[3]) {
+    constexpr double h = 0.86602540378443864676;  // √3/2
+    const C2<CT> t1 = cadd(v[1], v[2]);
+    const C2<CT> t2 = csub(v[0], cscale(t1, (CT)0.5));
+    const C2<CT> t3 = mul_si<S>(cscale(csub(v[1], v[2]), (CT)h));
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(t2, t3);
+    v[2] = csub(t2, t3);
+}
+
+template <int S, typename CT>
+__device__ __forceinline__ void dft4(C2<CT> (&v)[4]) {
+    const C2<CT> a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    const C2<CT> c = cadd(v[1], v[3]), d = mul_si<S>(csub(v[1], v[3]));
+    v[0] = cadd(a, c);
+    v[2] = csub(a, c);
+    v[1] = cadd(b, d);
+    v[3] = csub(b, d);
+}
+
+template <int S, typename CT>
+__device__ __forceinline__ void dft5(C2<CT> (&v)[5]) {
+    constexpr double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;  // cos 2π/5, cos 4π/5
+    constexpr double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;   // sin 2π/5, sin 4π/5
+    const C2<CT> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    const C2<CT> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    const C2<CT> x0 = v[0];
+    const C2<CT> p1 = cadd(x0, cadd(cscale(a1, (CT)c1), cscale(a2, (CT)c2)));
+    const C2<CT> p2 = cadd(x0, cadd(cscale(a1, (CT)c2), cscale(a2, (CT)c1)));
+    const C2<CT> q1 = mul_si<S>(cadd(cscale(b1, (CT)s1), cscale(b2, (CT)s2)));
+    const C2<CT> q2 = mul_si<S>(csub(cscale(b1, (CT)s2), cscale(b2, (CT)s1)));
+    v[0] = cadd(x0, cadd(a1, a2));
+    v[1] = cadd(p1, q1);
+    v[4] = csub(p1, q1);
+    v[2] = cadd(p2, q2);
+    v[3] = csub(p2, q2);
+}
+
+template <int S, typename CT>
+__device__ __forceinline__ void dft8(C2<CT> (&v)[8]) {
+    constexpr double r = 0.70710678118654752440;  // √½
+    C2<CT> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft4<S>(e);
+    dft4<S>(o);
+    // ω8^k = e^{S·iπk/4}
+    const C2<CT> t0 = o[0];
+    const C2<CT> t1 = cscale(cadd(o[1], mul_si<S>(o[1])), (CT)r);
+    const C2<CT> t2 = mul_si<S>(o[2]);
+    const C2<CT> t3 = cscale(csub(mul_si<S>(o[3]), o[3]), (CT)r);
+    v[0] = cadd(e[0], t0);
+    v[4] = csub(e[0], t0);
+    v[1] = cadd(e[1], t1);
+    v[5] = csub(e[1], t1);
+    v[2] = cadd(e[2], t2);
+    v[6] = csub(e[2], t2);
+    v[3] = cadd(e[3], t3);
+    v[7] = csub(e[3], t3);
+}
+
+template <int S, typename CT>
+__device__ __forceinline__ void dft16(C2<CT> (&v)[16]) {
+    // cos(πk/8), sin(πk/8), k = 0..7
+    constexpr double cs[8] = {1.0,
+                              0.92387953251128675613,
+                              0.70710678118654752440,
+                              0.38268343236508977173,
+                              0.0,
+                              -0.38268343236508977173,
+                              -0.70710678118654752440,
+                              -0.92387953251128675613};
+    constexpr double sn[8] = {0.0,
+                              0.38268343236508977173,
+                              0.70710678118654752440,
+                              0.92387953251128675613,
+                              1.0,
+                              0.92387953251128675613,
+                              0.70710678118654752440,
+                              0.38268343236508977173};
+    C2<CT> e[8], o[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        e[k] = v[2 * k];
+        o[k] = v[2 * k + 1];
+    }
+    dft8<S>(e);
+    dft8<S>(o);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const C2<CT> w{(CT)cs[k], (CT)(S * sn[k])};
+        const C2<CT> t = k == 0 ? o[0] : cmul(o[k], w);
+        v[k] = cadd(e[k], t);
+        v[k + 8] = csub(e[k], t);
+    }
+}
+
+template <int R, int S, typename CT>
+__device__ __forceinline__ void dft(C2<CT> (&v)[R]) {
+    if constexpr (R == 2) dft2<S>(v);
+    if constexpr (R == 3) dft3<S>(v);
+    if constexpr (R == 4) dft4<S>(v);
+    if constexpr (R == 5) dft5<S>(v);
+    if constexpr (R == 8) dft8<S>(v);
+    if constexpr (R == 16) dft16<S>(v);
+}
+
+// One Stockham pass of radix R on the LDS tile S (L rows × 2^lgB columns),
+// Ns = product of the radices of the previous passes; tw[n] = e^{-2πin/L}.
+// Kept out of line: inlined into the radix switch, the passes' live ranges
+// merge and the kernels need ~50% more registers.
+template <typename CT, int R, int S, int NT>
+__device__ __noinline__ void stockham_pass(int s_off, int lgB, int L, int Ns, int tw_off) {
+    C2<CT> *Sm = lds<CT>(s_off);
+    const C2<CT> *tw = lds<CT>(tw_off);
+    constexpr int ITEMS = (spec_tile<CT>() / R + NT - 1) / NT;
+    const int nb = L / R;
+    const int nwork = nb << lgB;
+    const int B1 = (1 << lgB) - 1;
+    const int twstride = L / (Ns * R);
+    C2<CT> v[ITEMS][R];
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        const int w = threadIdx.x + NT * i;
+        if (w < nwork) {
+            const int b = w & B1, j = w >> lgB;
+#pragma unroll
+            for (int k = 0; k < R; k++) v[i][k] = Sm[((j + k * nb) << lgB) + b];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        const int w = threadIdx.x + NT * i;
+        if (w < nwork) {
+            const int b = w & B1, j = w >> lgB;
+            const int q = j / Ns, jm = j - q * Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int k = 1; k < R; k++) {
+                    C2<CT> t = tw[k * jm * twstride];
+                    if (S > 0) t.im = -t.im;
+                    v[i][k] = cmul(v[i][k], t);
+                }
+            }
+            dft<R, S>(v[i]);
+            const int base = q * Ns * R + jm;
+#pragma unroll
+            for (int k = 0; k < R; k++) Sm[((base + k * Ns) << lgB) + b] = v[i][k];
+        }
+    }
+    __syncthreads();
+}
+
+// Whole transform of every column of the tile (natural order in and out).
+// s_off, tw_off: element offsets of the tile and of the twiddle table in spec_smem.
+template <typename CT, int S, int NT>
+__device__ void lds_fft(int s_off, int lgB, const AxisPlan &ax, int tw_off) {
+    int Ns = 1;
+    for (int p = 0; p < ax.npass; p++) {
+        const int R = ax.radix[p];
+        switch (R) {
+            case 16: stockham_pass<CT, 16, S, NT>(s_off, lgB, ax.L, Ns, tw_off); break;
+            case 8: stockham_pass<CT, 8, S, NT>(s_off, lgB, ax.L, Ns, tw_off); break;
+            case 4: stockham_pass<CT, 4, S, NT>(s_off, lgB, ax.L, Ns, tw_off); break;
+            case 2: stockham_pass<CT, 2, S, NT>(s_off, lgB, ax.L, Ns, tw_off); break;
+            case 3: stockham_pass<CT, 3, S, NT>(s_off, lgB, ax.L, Ns, tw_off); break;
+            default: stockham_pass<CT, 5, S, NT>(s_off, lgB, ax.L, Ns, tw_off); break;
+        }
+        Ns *= R;
+    }
+}
+
+// Blocks are dealt to the 8 XCDs round-robin; remap so that consecutive tiles
+// (which share cache lines) run on the same XCD. The grid is padded to a
+// multiple of 8 blocks; tiles ≥ n are idle.
+__device__ __forceinline__ int64_t xcd_remap(int64_t n) {
+    const int64_t per = ((int64_t)gridDim.x + 7) / 8;
+    return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
+template <typename CT, int NT>
+__device__ __forceinline__ void load_twiddles(C2<CT> *twl, const C2<CT> *__restrict__ tw, int L) {
+    for (int n = threadIdx.x; n < L; n += NT) twl[n] = tw[n];
+}
+
+// Tile element e = threadIdx.x + NT·i (i < EMAX) ↔ (row e >> lgB, column e & (B-1)):
+// gathers all of a thread's global loads before its LDS stores, so that they are in
+// flight together. get(row, col, ok) returns the value (ok: e inside the tile).
+template <typename CT, int NT, typename F>
+__device__ __forceinline__ void tile_load(C2<CT> *Sm, int n, int lgB, F get) {
+    constexpr int EMAX = spec_tile<CT>() / NT;
+    C2<CT> t[EMAX];
+#pragma unroll
+    for (int i = 0; i < EMAX; i++) {
+        const int e = threadIdx.x + NT * i;
+        if (e < n) t[i] = get(e >> lgB, e & ((1 << lgB) - 1));
+    }
+#pragma unroll
+    for (int i = 0; i < EMAX; i++) {
+        const int e = threadIdx.x + NT * i;
+        if (e < n) Sm[e] = t[i];
+    }
+}
+
+template <typename CT, int NT, typename F>
+__device__ __forceinline__ void tile_store(const C2<CT> *Sm, int n, int lgB, F put) {
+    constexpr int EMAX = spec_tile<CT>() / NT;
+#pragma unroll
+    for (int i = 0; i < EMAX; i++) {
+        const int e = threadIdx.x + NT * i;
+        if (e < n) put(e >> lgB, e & ((1 << lgB) - 1), Sm[e]);
+    }
+}
+
+struct SpecGeom {
+    int G[3];
+    int nkx, nky, nkz, K1, K2;
+    int lgB[3];       // log2 of the tile width of the x, y and z kernels
+    AxisPlan ax[3];
+};
+
+// ---------------------------------------------------------------------------
+// Forward x: tile = 2^lgB consecutive pair rows r (r = (c·G2 + z)·G1/2 + p).
+// The two real rows y = 2p, 2p + 1 of pair row r are rows 2r, 2r + 1 of x1.
+template <typename CT>
+__global__ __launch_bounds__(NTP) void spec_x_forward(SpecGeom g, const C2<CT> *__restrict__ rows,
+                                                      C2<CT> *__restrict__ x1, const C2<CT> *__restrict__ tw) {
+    const int L = g.G[0], lgB = g.lgB[0], B = 1 << lgB;
+    C2<CT> *Sm = lds<CT>(0);
+    const int tw_off = L << lgB;
+    C2<CT> *twl = lds<CT>(tw_off);
+    const int64_t nrows = 3 * (int64_t)g.G[2] * (g.G[1] >> 1);
+    const int64_t ntiles = nrows >> lgB;
+    const int64_t tile = xcd_remap(ntiles);
+    if (tile >= ntiles) return;
+    const int64_t r0 = tile << lgB;
+    const C2<CT> *src = rows + r0 * L;
+    // S[x·B + b] = row b, column x
+    tile_load<CT, NTP>(Sm, L << lgB, lgB, [&](int x, int b) { return src[(int64_t)b * L + x]; });
+    load_twiddles<CT, NTP>(twl, tw, L);
+    __syncthreads();
+    lds_fft<CT, -1, NTP>(0, lgB, g.ax[0], tw_off);
+    const int nkx = g.nkx;
+    C2<CT> *dst = x1 + 2 * r0 * nkx;
+    for (int b = 0; b < B; b++) {
+        for (int k = threadIdx.x; k < nkx; k += NTP) {
+            const C2<CT> zk = Sm[(k << lgB) + b];
+            const C2<CT> zm = Sm[((k == 0 ? 0 : L - k) << lgB) + b];
+            dst[(2 * b) * nkx + k] = C2<CT>{(zk.re + zm.re) * (CT)0.5, (zk.im - zm.im) * (CT)0.5};
+            dst[(2 * b + 1) * nkx + k] = C2<CT>{(zk.im + zm.im) * (CT)0.5, (zm.re - zk.re) * (CT)0.5};
+        }
+    }
+}
+
+// Forward y: tile = (c, z, kx-block); x1[c][z][y][kx] → x2[c][jy][z][kx] for the
+// retained ky = jy (jy ≤ K1) or jy - nky.
+template <typename CT>
+__global__ __launch_bounds__(NTP) void spec_y_forward(SpecGeom g, const C2<CT> *__restrict__ x1,
+                                                      C2<CT> *__restrict__ x2, const C2<CT> *__restrict__ tw) {
+    const int L = g.G[1], lgB = g.lgB[1], B = 1 << lgB;
+    C2<CT> *Sm = lds<CT>(0);
+    const int tw_off = L << lgB;
+    C2<CT> *twl = lds<CT>(tw_off);
+    const int nkx = g.nkx, nkb = (nkx + B - 1) >> lgB;
+    const int64_t ntiles = 3 * (int64_t)g.G[2] * nkb;
+    const int64_t tile = xcd_remap(ntiles);
+    if (tile >= ntiles) return;
+    const int kb = (int)(tile % nkb);
+    const int64_t cz = tile / nkb;  // c·G2 + z
+    const int c = (int)(cz / g.G[2]), z = (int)(cz - (int64_t)c * g.G[2]);
+    const int kx0 = kb << lgB;
+    const C2<CT> *src = x1 + cz * L * nkx + kx0;
+    tile_load<CT, NTP>(Sm, L << lgB, lgB, [&](int y, int b) {
+        return kx0 + b < nkx ? src[(int64_t)y * nkx + b] : C2<CT>{(CT)0, (CT)0};
+    });
+    load_twiddles<CT, NTP>(twl, tw, L);
+    __syncthreads();
+    lds_fft<CT, -1, NTP>(0, lgB, g.ax[1], tw_off);
+    const int nky = g.nky, K1 = g.K1;
+    C2<CT> *dst = x2 + ((int64_t)c * nky * g.G[2] + z) * nkx + kx0;
+    const int64_t jstride = (int64_t)g.G[2] * nkx;
+    constexpr int EMAX = spec_tile<CT>() / NTP;
+#pragma unroll
+    for (int i = 0; i < EMAX; i++) {
+        const int e = threadIdx.x + NTP * i;
+        const int b = e & (B - 1), jy = e >> lgB;
+        if (jy < nky && kx0 + b < nkx) {
+            const int iy = jy <= K1 ? jy : L - nky + jy;
+            dst[jy * jstride + b] = Sm[(iy << lgB) + b];
+        }
+    }
+}
+
+struct SpecFourier {
+    double kscale[3];
+    double pref;          // Γ/V
+    double kmax2_trunc;   // < 0: no spherical truncation
+    const double *dx, *dy, *dz;  // 1/ψ̂ (deconvolution), dy/dz indexed by k + K
+    const double *ex, *ey, *ez;  // (1/ψ̂)² e^{-k_d²/4α²}, same indexing
+    int do_vel, do_psi, keep_uhat;
+    void *uhat;                  // [3][nky][nkz][nkx]
+    int64_t ucstride;            // complex values per component
+};
+
+// z passes fused with the Fourier-space kernel: tile = (jy, kx-block), the three
+// components in three LDS tiles. x2 holds z rows on input; on output, x2 the
+// velocity and x2b the streamfunction (each when requested), in z space, ready
+// for the inverse y, x passes. With both outputs, the velocity spectrum makes a
+// round trip through x2 while the tiles transform the streamfunction.
+template <typename CT>
+__global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f, C2<CT> *__restrict__ x2,
+                                                      C2<CT> *__restrict__ x2b, const C2<CT> *__restrict__ tw) {
+    constexpr int EMAX = spec_tile<CT>() / NTZ;
+    const int L = g.G[2], lgB = g.lgB[2], B = 1 << lgB;
+    const int n = L << lgB;
+    C2<CT> *T[3] = {lds<CT>(0), lds<CT>(n), lds<CT>(2 * n)};
+    const int tw_off = 3 * n;
+    C2<CT> *twl = lds<CT>(tw_off);
+    const int nkx = g.nkx, nky = g.nky, nkb = (nkx + B - 1) >> lgB;
+    const int64_t ntiles = (int64_t)nky * nkb;
+    const int64_t tile = xcd_remap(ntiles);
+    if (tile >= ntiles) return;
+    const int kb = (int)(tile % nkb), jy = (int)(tile / nkb);
+    const int kx0 = kb << lgB;
+    const int64_t cs = (int64_t)nky * L * nkx;  // complex values per component of x2
+    const int64_t off = (int64_t)jy * L * nkx + kx0;
+    load_twiddles<CT, NTZ>(twl, tw, L);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const C2<CT> *src = x2 + c * cs + off;
+        tile_load<CT, NTZ>(T[c], n, lgB, [&](int z, int b) {
+            return kx0 + b < nkx ? src[(int64_t)z * nkx + b] : C2<CT>{(CT)0, (CT)0};
+        });
+    }
+    __syncthreads();
+    for (int c = 0; c < 3; c++) lds_fft<CT, -1, NTZ>(c * n, lgB, g.ax[2], tw_off);
+    // Fourier-space operators on the retained modes
+    const int K1 = g.K1, K2 = g.K2, nkz = g.nkz;
+    const int ky = jy <= K1 ? jy : jy - nky;
+    const double KY = f.kscale[1] * ky;
+    const bool both = f.do_vel && f.do_psi;
+    for (int i = 0; i < EMAX; i++) {
+        const int e = threadIdx.x + NTZ * i;
+        if (e >= n) break;
+        const int b = e & (B - 1), iz = e >> lgB;
+        const int kx = kx0 + b;
+        const int kz = iz <= (L >> 1) ? iz : iz - L;
+        const bool kz_kept = kz >= -K2 && kz <= K2;
+        const double KX = f.kscale[0] * kx, KZ = f.kscale[2] * kz;
+        const double k2 = KX * KX + KY * KY + KZ * KZ;
+        bool keep = kx < nkx && kz_kept && k2 != 0;
+        if (f.kmax2_trunc >= 0 && k2 > f.kmax2_trunc) keep = false;
+        C2<CT> *u = nullptr;
+        if (f.keep_uhat && kx < nkx && kz_kept) {
+            const int jz = kz >= 0 ? kz : kz + nkz;
+            u = (C2<CT> *)f.uhat + ((int64_t)jy * nkz + jz) * nkx + kx;
+        }
+        const C2<CT> zero{(CT)0, (CT)0};
+        if (!keep) {
+            if (u) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) u[d * f.ucstride] = zero;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) T[d][e] = zero;
+            if (both && kx < nkx) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) x2[d * cs + off + (int64_t)iz * nkx + b] = zero;
+            }
+            continue;
+        }
+        double wr[3], wi[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const C2<CT> w = T[d][e];
+            wr[d] = (double)w.re;
+            wi[d] = (double)w.im;
+        }
+        const double inv_k2 = 1.0 / k2;
+        if (u) {
+            // unsmoothed field ω̂/k² (one deconvolution, no Ewald factor), then i k × for the velocity
+            const double su = f.pref * f.dx[kx] * f.dy[ky + K1] * f.dz[kz + K2] * inv_k2;
+            double ur[3], ui[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) ur[d] = wr[d] * su, ui[d] = wi[d] * su;
+            if (f.do_vel) {
+                const double ax = KY * ur[2] - KZ * ur[1], bx = KY * ui[2] - KZ * ui[1];
+                const double ay = KZ * ur[0] - KX * ur[2], by = KZ * ui[0] - KX * ui[2];
+                const double az = KX * ur[1] - KY * ur[0], bz = KX * ui[1] - KY * ui[0];
+                ur[0] = -bx, ui[0] = ax, ur[1] = -by, ui[1] = ay, ur[2] = -bz, ui[2] = az;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) u[d * f.ucstride] = C2<CT>{(CT)ur[d], (CT)ui[d]};
+        }
+        // ω̂ = Γ/V · B̂/ψ̂; ψ̂_out = ω̂·e^{-k²/4α²}/(ψ̂ k²) (deconvolution of type 1 and type 2)
+        const double s = f.pref * f.ex[kx] * f.ey[ky + K1] * f.ez[kz + K2] * inv_k2;
+#pragma unroll
+        for (int d = 0; d < 3; d++) wr[d] *= s, wi[d] *= s;
+        if (f.do_psi) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) T[d][e] = C2<CT>{(CT)wr[d], (CT)wi[d]};
+        }
+        if (f.do_vel) {
+            // v̂ = i k × ψ̂: (i k × w)_x = i (KY w_z - KZ w_y), i (a + ib) = -b + ia
+            const double ax = KY * wr[2] - KZ * wr[1], bx = KY * wi[2] - KZ * wi[1];
+            const double ay = KZ * wr[0] - KX * wr[2], by = KZ * wi[0] - KX * wi[2];
+            const double az = KX * wr[1] - KY * wr[0], bz = KX * wi[1] - KY * wi[0];
+            const C2<CT> v[3] = {{(CT)(-bx), (CT)ax}, {(CT)(-by), (CT)ay}, {(CT)(-bz), (CT)az}};
+            if (both) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) x2[d * cs + off + (int64_t)iz * nkx + b] = v[d];
+            } else {
+#pragma unroll
+                for (int d = 0; d < 3; d++) T[d][e] = v[d];
+            }
+        }
+    }
+    __syncthreads();
+    C2<CT> *out = f.do_psi ? x2b : x2;
+    for (int c = 0; c < 3; c++) lds_fft<CT, 1, NTZ>(c * n, lgB, g.ax[2], tw_off);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        C2<CT> *dst = out + c * cs + off;
+        tile_store<CT, NTZ>(T[c], n, lgB, [&](int z, int b, C2<CT> v) {
+            if (kx0 + b < nkx) dst[(int64_t)z * nkx + b] = v;
+        });
+    }
+    if (both) {
+        // this block's writes to x2 above are visible to its own loads after the barrier
+        __threadfence_block();
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const C2<CT> *src = x2 + c * cs + off;
+            tile_load<CT, NTZ>(T[c], n, lgB, [&](int z, int b) {
+                return kx0 + b < nkx ? src[(int64_t)z * nkx + b] : C2<CT>{(CT)0, (CT)0};
+            });
+        }
+        __syncthreads();
+        for (int c = 0; c < 3; c++) lds_fft<CT, 1, NTZ>(c * n, lgB, g.ax[2], tw_off);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            C2<CT> *dst = x2 + c * cs + off;
+            tile_store<CT, NTZ>(T[c], n, lgB, [&](int z, int b, C2<CT> v) {
+                if (kx0 + b < nkx) dst[(int64_t)z * nkx + b] = v;
+            });
+        }
+    }
+}
+
+// Band-filtered kept field in z space: the kept û on kmin² ≤ k² ≤ kmax² times the
+// type-2 deconvolution, zero elsewhere, inverse FFT_z → x2. Tile = (jy, kx-block).
+template <typename CT>
+__global__ __launch_bounds__(NTZ) void spec_z_band(SpecGeom g, SpecFourier f, double kmin2, double kmax2,
+                                                   C2<CT> *__restrict__ x2, const C2<CT> *__restrict__ tw) {
+    const int L = g.G[2], lgB = g.lgB[2], B = 1 << lgB;
+    C2<CT> *Sm = lds<CT>(0);
+    const int tw_off = L << lgB;
+    C2<CT> *twl = lds<CT>(tw_off);
+    const int nkx = g.nkx, nky = g.nky, nkz = g.nkz, nkb = (nkx + B - 1) >> lgB;
+    const int64_t ntiles = (int64_t)nky * nkb;
+    const int64_t tile = xcd_remap(ntiles);
+    if (tile >= ntiles) return;
+    const int kb = (int)(tile % nkb), jy = (int)(tile / nkb);
+    const int kx0 = kb << lgB;
+    const int n = L << lgB;
+    const int64_t cs = (int64_t)nky * L * nkx;
+    const int64_t off = (int64_t)jy * L * nkx + kx0;
+    const int K1 = g.K1, K2 = g.K2;
+    const int ky = jy <= K1 ? jy : jy - nky;
+    const double KY = f.kscale[1] * ky;
+    load_twiddles<CT, NTZ>(twl, tw, L);
+    const C2<CT> *uh = (const C2<CT> *)f.uhat;
+    for (int c = 0; c < 3; c++) {
+        tile_load<CT, NTZ>(Sm, n, lgB, [&](int iz, int b) {
+            const int kx = kx0 + b;
+            const int kz = iz <= (L >> 1) ? iz : iz - L;
+            C2<CT> v{(CT)0, (CT)0};
+            if (kx < nkx && kz >= -K2 && kz <= K2) {
+                const double KX = f.kscale[0] * kx, KZ = f.kscale[2] * kz;
+                const double k2 = KX * KX + KY * KY + KZ * KZ;
+                if (k2 >= kmin2 && k2 <= kmax2) {
+                    const double D = f.dx[kx] * f.dy[ky + K1] * f.dz[kz + K2];
+                    const int jz = kz >= 0 ? kz : kz + nkz;
+                    const C2<CT> u = uh[c * f.ucstride + ((int64_t)jy * nkz + jz) * nkx + kx];
+                    v = C2<CT>{(CT)((double)u.re * D), (CT)((double)u.im * D)};
+                }
+            }
+            return v;
+        });
+        __syncthreads();
+        lds_fft<CT, 1, NTZ>(0, lgB, g.ax[2], tw_off);
+        C2<CT> *dst = x2 + c * cs + off;
+        tile_store<CT, NTZ>(Sm, n, lgB, [&](int z, int b, C2<CT> v) {
+            if (kx0 + b < nkx) dst[(int64_t)z * nkx + b] = v;
+        });
+        __syncthreads();
+    }
+}
+
+// Inverse y: tile = (c, z, kx-block); x2[c][jy][z][kx] on the retained ky (zero
+// elsewhere) → inverse FFT_y → x1[c][z][y][kx].
+template <typename CT>
+__global__ __launch_bounds__(NTP) void spec_y_inverse(SpecGeom g, const C2<CT> *__restrict__ x2,
+                                                      C2<CT> *__restrict__ x1, const C2<CT> *__restrict__ tw) {
+    const int L = g.G[1], lgB = g.lgB[1], B = 1 << lgB;
+    C2<CT> *Sm = lds<CT>(0);
+    const int tw_off = L << lgB;
+    C2<CT> *twl = lds<CT>(tw_off);
+    const int nkx = g.nkx, nky = g.nky, K1 = g.K1, nkb = (nkx + B - 1) >> lgB;
+    const int64_t ntiles = 3 * (int64_t)g.G[2] * nkb;
+    const int64_t tile = xcd_remap(ntiles);
+    if (tile >= ntiles) return;
+    const int kb = (int)(tile % nkb);
+    const int64_t cz = tile / nkb;
+    const int c = (int)(cz / g.G[2]), z = (int)(cz - (int64_t)c * g.G[2]);
+    const int kx0 = kb << lgB;
+    const C2<CT> *src = x2 + ((int64_t)c * nky * g.G[2] + z) * nkx + kx0;
+    const int64_t jstride = (int64_t)g.G[2] * nkx;
+    tile_load<CT, NTP>(Sm, L << lgB, lgB, [&](int iy, int b) {
+        const int ky = iy <= (L >> 1) ? iy : iy - L;
+        C2<CT> v{(CT)0, (CT)0};
+        if (ky >= -K1 && ky <= K1 && kx0 + b < nkx) v = src[(ky >= 0 ? ky : ky + nky) * jstride + b];
+        return v;
+    });
+    load_twiddles<CT, NTP>(twl, tw, L);
+    __syncthreads();
+    lds_fft<CT, 1, NTP>(0, lgB, g.ax[1], tw_off);
+    C2<CT> *dst = x1 + cz * L * nkx + kx0;
+    tile_store<CT, NTP>(Sm, L << lgB, lgB, [&](int y, int b, C2<CT> v) {
+        if (kx0 + b < nkx) dst[(int64_t)y * nkx + b] = v;
+    });
+}
+
+// Inverse x: tile = 2^lgB pair rows; Z_k = A_k + i B_k for k < nkx (A, B from rows
+// 2r, 2r + 1 of x1), their conjugates at G0 - k (Hermitian symmetry of real rows),
+// zero otherwise, DC keeps only its real parts; inverse FFT_x → pair rows.
+template <typename CT>
+__global__ __launch_bounds__(NTP) void spec_x_inverse(SpecGeom g, const C2<CT> *__restrict__ x1,
+                                                      C2<CT> *__restrict__ rows, const C2<CT> *__restrict__ tw) {
+    const int L = g.G[0], lgB = g.lgB[0], B = 1 << lgB;
+    C2<CT> *Sm = lds<CT>(0);
+    const int tw_off = L << lgB;
+    C2<CT> *twl = lds<CT>(tw_off);
+    const int64_t nrows = 3 * (int64_t)g.G[2] * (g.G[1] >> 1);
+    const int64_t ntiles = nrows >> lgB;
+    const int64_t tile = xcd_remap(ntiles);
+    if (tile >= ntiles) return;
+    const int64_t r0 = tile << lgB;
+    const int nkx = g.nkx;
+    const C2<CT> *src = x1 + 2 * r0 * nkx;
+    tile_load<CT, NTP>(Sm, L << lgB, lgB, [&](int k, int b) {
+        const C2<CT> *ra = src + (2 * b) * nkx, *rb = ra + nkx;
+        C2<CT> zk{(CT)0, (CT)0};
+        if (k < nkx) {
+            C2<CT> A = ra[k], Bv = rb[k];
+            if (k == 0) A.im = Bv.im = (CT)0;
+            zk = C2<CT>{A.re - Bv.im, A.im + Bv.re};
+        } else if (k > L - nkx) {
+            const C2<CT> A = ra[L - k], Bv = rb[L - k];  // conj(A) + i conj(B)
+            zk = C2<CT>{A.re + Bv.im, Bv.re - A.im};
+        }
+        return zk;
+    });
+    load_twiddles<CT, NTP>(twl, tw, L);
+    __syncthreads();
+    lds_fft<CT, 1, NTP>(0, lgB, g.ax[0], tw_off);
+    C2<CT> *dst = rows + r0 * L;
+    tile_store<CT, NTP>(Sm, L << lgB, lgB, [&](int x, int b, C2<CT> v) { dst[(int64_t)b * L + x] = v; });
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+
+AxisPlan plan_axis(int L) {
+    AxisPlan a{};
+    a.L = L;
+    int n = L, np = 0;
+    while (n % 16 == 0 && np < 6) a.radix[np++] = 16, n /= 16;
+    if (n % 8 == 0 && np < 6) a.radix[np++] = 8, n /= 8;
+    if (n % 4 == 0 && np < 6) a.radix[np++] = 4, n /= 4;
+    if (n % 2 == 0 && np < 6) a.radix[np++] = 2, n /= 2;
+    while (n % 3 == 0 && np < 6) a.radix[np++] = 3, n /= 3;
+    while (n % 5 == 0 && np < 6) a.radix[np++] = 5, n /= 5;
+    a.npass = n == 1 ? np : -1;
+    return a;
+}
+
+template <typename CT>
+SpecGeom spec_geom(const vpbs_ctx *c) {
+    SpecGeom g{};
+    for (int d = 0; d < 3; d++) {
+        g.G[d] = c->G[d];
+        g.ax[d] = plan_axis(c->G[d]);
+        int lg = 0;
+        while ((c->G[d] << (lg + 1)) <= spec_tile<CT>()) lg++;
+        if (d == 2)
+            while (lg > 0 && sizeof(C2<CT>) * (3 * ((size_t)c->G[d] << lg) + c->G[d]) > c->spec_lds_max) lg--;
+        g.lgB[d] = lg;
+    }
+    g.nkx = c->nkx;
+    g.nky = c->nky;
+    g.nkz = c->nkz;
+    g.K1 = c->kmax_mode[1];
+    g.K2 = c->kmax_mode[2];
+    return g;
+}
+
+template <typename CT>
+size_t spec_lds(const SpecGeom &g, int d, int ntiles = 1) {
+    return sizeof(C2<CT>) * (ntiles * ((size_t)g.G[d] << g.lgB[d]) + g.G[d]);
+}
+
+unsigned grid8(int64_t ntiles) { return (unsigned)(8 * ((ntiles + 7) / 8)); }
+
+template <typename CT>
+const C2<CT> *twiddles(const vpbs_ctx *c, int d) {
+    const C2<CT> *t = (const C2<CT> *)c->spec_tw;
+    for (int k = 0; k < d; k++) t += c->G[k];
+    return t;
+}
+
+template <typename CT>
+SpecFourier spec_fourier_args(const vpbs_ctx *c) {
+    SpecFourier f{};
+    for (int d = 0; d < 3; d++) f.kscale[d] = 2 * M_PI / c->p.Ls[d];
+    f.pref = c->p.Gamma / (c->p.Ls[0] * c->p.Ls[1] * c->p.Ls[2]);
+    f.kmax2_trunc = -1;
+    if (c->p.truncate_spherical) {
+        double km = 1e300;
+        for (int d = 0; d < 3; d++) km = std::fmin(km, 2 * M_PI * ((c->p.Ns[d] - 1) / 2) / c->p.Ls[d]);
+        f.kmax2_trunc = km * km;  // params.jl:161-166
+    }
+    const int n0 = c->kmax_mode[0] + 1, n1 = 2 * c->kmax_mode[1] + 1;
+    f.dx = c->deconv;
+    f.dy = f.dx + n0;
+    f.dz = f.dy + n1;
+    f.ex = c->spec_fac;
+    f.ey = f.ex + n0;
+    f.ez = f.ey + n1;
+    f.uhat = c->uhat;
+    f.ucstride = c->uhat_stride;
+    return f;
+}
+
+template <typename CT>
+int forward_impl(vpbs_ctx *c) {
+    hipStream_t s = c->s_lr;
+    const SpecGeom g = spec_geom<CT>(c);
+    const int64_t nrows = 3 * (int64_t)c->G[2] * (c->G[1] / 2);
+    const int64_t nkb = (c->nkx + (1 << g.lgB[1]) - 1) >> g.lgB[1];
+    spec_x_forward<CT><<<grid8(nrows >> g.lgB[0]), NTP, spec_lds<CT>(g, 0), s>>>(
+        g, (const C2<CT> *)c->grid, (C2<CT> *)c->ypen, twiddles<CT>(c, 0));
+    spec_y_forward<CT><<<grid8(3 * (int64_t)c->G[2] * nkb), NTP, spec_lds<CT>(g, 1), s>>>(
+        g, (const C2<CT> *)c->ypen, (C2<CT> *)c->zpen, twiddles<CT>(c, 1));
+    VPBS_HIP(c, hipGetLastError());
+    return VPBS_OK;
+}
+
+template <typename CT>
+int fourier_impl(vpbs_ctx *c, uint32_t flags) {
+    hipStream_t s = c->s_lr;
+    const SpecGeom g = spec_geom<CT>(c);
+    SpecFourier f = spec_fourier_args<CT>(c);
+    f.do_vel = (flags & VPBS_VELOCITY) ? 1 : 0;
+    f.do_psi = (flags & VPBS_STREAMFUNCTION) ? 1 : 0;
+    f.keep_uhat = (flags & VPBS_KEEP_UHAT) ? 1 : 0;
+    const int64_t nkb = (c->nkx + (1 << g.lgB[2]) - 1) >> g.lgB[2];
+    C2<CT> *x2b = f.do_psi ? (C2<CT> *)c->zpen2 : nullptr;
+    spec_z_fourier<CT><<<grid8((int64_t)c->nky * nkb), NTZ, spec_lds<CT>(g, 2, 3), s>>>(
+        g, f, (C2<CT> *)c->zpen, x2b, twiddles<CT>(c, 2));
+    VPBS_HIP(c, hipGetLastError());
+    return VPBS_OK;
+}
+
+template <typename CT>
+int inverse_impl(vpbs_ctx *c, void *zp) {
+    hipStream_t s = c->s_lr;
+    const SpecGeom g = spec_geom<CT>(c);
+    const int64_t nrows = 3 * (int64_t)c->G[2] * (c->G[1] / 2);
+    const int64_t nkb = (c->nkx + (1 << g.lgB[1]) - 1) >> g.lgB[1];
+    spec_y_inverse<CT><<<grid8(3 * (int64_t)c->G[2] * nkb), NTP, spec_lds<CT>(g, 1), s>>>(
+        g, (const C2<CT> *)zp, (C2<CT> *)c->ypen, twiddles<CT>(c, 1));
+    spec_x_inverse<CT><<<grid8(nrows >> g.lgB[0]), NTP, spec_lds<CT>(g, 0), s>>>(
+        g, (const C2<CT> *)c->ypen, (C2<CT> *)c->grid, twiddles<CT>(c, 0));
+    VPBS_HIP(c, hipGetLastError());
+    return VPBS_OK;
+}
+
+template <typename CT>
+int band_impl(vpbs_ctx *c, double kmin2, double kmax2) {
+    hipStream_t s = c->s_lr;
+    const SpecGeom g = spec_geom<CT>(c);
+    const SpecFourier f = spec_fourier_args<CT>(c);
+    const int64_t nkb = (c->nkx + (1 << g.lgB[2]) - 1) >> g.lgB[2];
+    spec_z_band<CT><<<grid8((int64_t)c->nky * nkb), NTZ, spec_lds<CT>(g, 2), s>>>(
+        g, f, kmin2, kmax2, (C2<CT> *)c->zpen, twiddles<CT>(c, 2));
+    VPBS_HIP(c, hipGetLastError());
+    return inverse_impl<CT>(c, c->zpen);
+}
+
+template <typename CT>
+int setup_impl(vpbs_ctx *c) {
+    // twiddles e^{-2πin/G_d}, n ∈ [0, G_d), one table per axis
+    std::vector<C2<CT>> tw;
+    for (int d = 0; d < 3; d++) {
+        const int L = c->G[d];
+        for (int n = 0; n < L; n++) {
+            const long double a = -2.0L * 3.14159265358979323846264338327950288L * n / L;
+            tw.push_back(C2<CT>{(CT)std::cos(a), (CT)std::sin(a)});
+        }
+    }
+    VPBS_HIP(c, hipMalloc(&c->spec_tw, sizeof(C2<CT>) * tw.size()));
+    VPBS_HIP(c, hipMemcpy(c->spec_tw, tw.data(), sizeof(C2<CT>) * tw.size(), hipMemcpyHostToDevice));
+    int lds_dev = 0;
+    VPBS_HIP(c, hipDeviceGetAttribute(&lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    c->spec_lds_max = std::min((size_t)lds_dev, (size_t)160 * 1024);
+    const SpecGeom g = spec_geom<CT>(c);
+    const int64_t nrows = 3 * (int64_t)c->G[2] * (c->G[1] / 2);
+    if (nrows % ((int64_t)1 << g.lgB[0]))
+        return fail(c, VPBS_ERR_INVALID, "fused transforms: %lld pair rows do not split into tiles of %d",
+                    (long long)nrows, 1 << g.lgB[0]);
+    const size_t lds_p = std::max(spec_lds<CT>(g, 0), spec_lds<CT>(g, 1)), lds_z = spec_lds<CT>(g, 2, 3);
+    if (std::max(lds_p, lds_z) > (size_t)lds_dev)
+        return fail(c, VPBS_ERR_INVALID, "fused transforms need %zu B of LDS per block, the device has %d",
+                    std::max(lds_p, lds_z), lds_dev);
+    auto attr = [&](const void *k, size_t lds) {
+        return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    };
+    VPBS_HIP(c, attr((const void *)spec_x_forward<CT>, lds_p));
+    VPBS_HIP(c, attr((const void *)spec_y_forward<CT>, lds_p));
+    VPBS_HIP(c, attr((const void *)spec_z_fourier<CT>, lds_z));
+    VPBS_HIP(c, attr((const void *)spec_z_band<CT>, lds_z));
+    VPBS_HIP(c, attr((const void *)spec_y_inverse<CT>, lds_p));
+    VPBS_HIP(c, attr((const void *)spec_x_inverse<CT>, lds_p));
+    return VPBS_OK;
+}
+
+}  // namespace
+
+bool spectral_supported(const vpbs_ctx *c) {
+    const int tile = c->p.precision == VPBS_F32 ? spec_tile<float>() : spec_tile<double>();
+    for (int d = 0; d < 3; d++) {
+        const AxisPlan a = plan_axis(c->G[d]);
+        if (a.npass < 0 || c->G[d] > tile) return false;
+    }
+    return true;
+}
+
+int spectral_setup(vpbs_ctx *c) {
+    // separable factors (1/ψ̂)² e^{-k_d²/4α²} of the Fourier-space kernel
+    const double inv4a2 = 1.0 / (4 * c->p.alpha * c->p.alpha);
+    std::vector<double> hd((size_t)(c->kmax_mode[0] + 1) + (2 * c->kmax_mode[1] + 1) + (2 * c->kmax_mode[2] + 1));
+    VPBS_HIP(c, hipMemcpy(hd.data(), c->deconv, sizeof(double) * hd.size(), hipMemcpyDeviceToHost));
+    std::vector<double> fac(hd.size());
+    size_t i = 0;
+    for (int d = 0; d < 3; d++) {
+        const int K = c->kmax_mode[d];
+        const double ks = 2 * M_PI / c->p.Ls[d];
+        for (int k = d == 0 ? 0 : -K; k <= K; k++, i++) {
+            const double kk = ks * k;
+            fac[i] = hd[i] * hd[i] * std::exp(-kk * kk * inv4a2);
+        }
+    }
+    VPBS_HIP(c, hipMalloc((void **)&c->spec_fac, sizeof(double) * fac.size()));
+    VPBS_HIP(c, hipMemcpy(c->spec_fac, fac.data(), sizeof(double) * fac.size(), hipMemcpyHostToDevice));
+    return c->p.precision == VPBS_F32 ? setup_impl<float>(c) : setup_impl<double>(c);
+}
+
+int spectral_forward(vpbs_ctx *c) {
+    return c->p.precision == VPBS_F32 ? forward_impl<float>(c) : forward_impl<double>(c);
+}
+
+int spectral_fourier(vpbs_ctx *c, uint32_t flags) {
+    return c->p.precision == VPBS_F32 ? fourier_impl<float>(c, flags) : fourier_impl<double>(c, flags);
+}
+
+int spectral_inverse(vpbs_ctx *c, void *zp) {
+    return c->p.precision == VPBS_F32 ? inverse_impl<float>(c, zp) : inverse_impl<double>(c, zp);
+}
+
+int spectral_band(vpbs_ctx *c, double kmin2, double kmax2) {
+    return c->p.precision == VPBS_F32 ? band_impl<float>(c, kmin2, kmax2) : band_impl<double>(c, kmin2, kmax2);
+}
+
+void spectral_release(vpbs_ctx *c) {
+    if (c->spec_tw) (void)hipFree(c->spec_tw);
+    if (c->spec_fac) (void)hipFree(c->spec_fac);
+    c->spec_tw = nullptr;
+    c->spec_fac = nullptr;
+}
+
+}  // namespace vpbs

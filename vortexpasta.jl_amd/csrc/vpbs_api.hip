@@ -81,6 +81,10 @@ static int validate(const vpbs_params *p, std::string &msg) {
         msg = "nufft_sigma must be > 1";
         return VPBS_ERR_INVALID;
     }
+    if (p->fft != VPBS_FFT_FUSED && p->fft != VPBS_FFT_ROCFFT) {
+        msg = "fft must be VPBS_FFT_FUSED (0) or VPBS_FFT_ROCFFT (1)";
+        return VPBS_ERR_INVALID;
+    }
     const double Lmin = std::fmin(p->Ls[0], std::fmin(p->Ls[1], p->Ls[2]));
     const int M = p->nsubdiv;
     const double rmax = ((double)M / (2 * M + 1)) * Lmin;  // CellLists.jl:115

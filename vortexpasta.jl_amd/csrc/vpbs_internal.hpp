@@ -109,15 +109,18 @@ struct vpbs_ctx {
     int64_t ntiles = 0;
     // Real grid, 3 components, rows of y = 2p and y = 2p+1 interleaved as one
     // complex row (pair p): value (x, y, z) at ((z·G1/2 + y/2)·G0 + x)·2 + (y & 1).
-    // Spreading writes it, the x-pass transforms it in place, interpolation reads it.
+    // Spreading writes it, the inverse x transform rewrites it, interpolation reads it.
     void *grid = nullptr;
-    // Pencils of retained modes (complex): y-pencils [3][nkx][G2][G1] and
-    // z-pencils [3][nkx][nky][G2] (vorticity, then velocity, in place); the
-    // streamfunction gets its own z-pencils (allocated on demand).
+    // Pencils of retained modes (complex), same sizes in both transform paths:
+    //   fused:  ypen = x1 [3][G2][G1][nkx], zpen = x2 [3][nky][G2][nkx] (z space on both
+    //           sides of the z transforms: vorticity in, velocity out);
+    //   rocFFT: ypen [3][nkx][G2][G1], zpen [3][nkx][nky][G2] (vorticity, then velocity, in place).
+    // The streamfunction gets its own zpen2 (allocated on demand).
     int32_t nkx = 0, nky = 0;
     int64_t ypen_stride = 0, zpen_stride = 0;  // complex values per component
     void *ypen = nullptr, *zpen = nullptr, *zpen2 = nullptr;
-    // unsmoothed field kept by VPBS_KEEP_UHAT: [3][nkx][nky][nkz] complex (compute type)
+    // unsmoothed field kept by VPBS_KEEP_UHAT: [3][nky][nkz][nkx] complex (compute type),
+    // ky = jy (jy ≤ K1) or jy - nky, kz likewise with K2 and nkz
     int32_t nkz = 0;
     int64_t uhat_stride = 0;   // complex values per component
     void *uhat = nullptr;
@@ -135,6 +138,12 @@ struct vpbs_ctx {
     void *fft_work = nullptr;
     size_t fft_work_size = 0;
     bool plans_ready = false;
+    // fused transforms (spectral.hip): twiddle tables e^{-2πin/G_d} (compute type) and the
+    // separable factors (1/ψ̂)² e^{-k_d²/4α²} of the Fourier-space kernel (FP64)
+    bool fft_fused = false;
+    size_t spec_lds_max = 0;    // LDS per block available to the fused kernels
+    void *spec_tw = nullptr;
+    double *spec_fac = nullptr;
 };
 
 namespace vpbs {
@@ -180,6 +189,17 @@ int shortrange_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff);
 int longrange_setup(vpbs_ctx *c);
 int longrange_compute(vpbs_ctx *c, uint32_t flags);
 int longrange_get_uhat(vpbs_ctx *c, int32_t *quantity, int64_t *dims3, double *out);
+
+// fused spectral pipeline (spectral.hip): grid → x1 → x2 (forward x, y), the z
+// transforms with the Fourier-space kernel on x2 (velocity) / zpen2 (streamfunction),
+// and x2 → x1 → grid (inverse y, x); x1 = ypen, x2 = zpen.
+bool spectral_supported(const vpbs_ctx *c);
+int spectral_setup(vpbs_ctx *c);
+int spectral_forward(vpbs_ctx *c);
+int spectral_fourier(vpbs_ctx *c, uint32_t flags);
+int spectral_inverse(vpbs_ctx *c, void *zp);
+int spectral_band(vpbs_ctx *c, double kmin2, double kmax2);
+void spectral_release(vpbs_ctx *c);
 int longrange_spectrum(vpbs_ctx *c, int64_t nk, double dk, double *Ek);
 int longrange_band(vpbs_ctx *c, double kmin, double kmax, double *v_out);
 void longrange_release(vpbs_ctx *c);

@@ -220,16 +220,25 @@ class HIPShortRangeBackend:
 
 @dataclass(frozen=True)
 class HIPLongRangeBackend:
-    """Long-range part on the GPU: hand-written NUFFT (ES kernel) + rocFFT.
+    """Long-range part on the GPU: hand-written NUFFT (ES kernel) and transforms.
 
     Replaces ``NonuniformFFTsBackend(ROCBackend(); σ, m)``. ``σ`` is the
     oversampling factor and ``m`` the kernel half-support, as in the reference
     (``nonuniformffts.jl:107-120``); the defaults give ~6-digit accuracy.
+    ``fft`` selects the transforms: ``"fused"`` (hand-written FFTs fused with the
+    pencil reorderings and the Fourier-space kernel; grids up to 6144 (FP32) or
+    3072 (FP64) per axis, larger ones use rocFFT) or ``"rocfft"`` (rocFFT 1D
+    transforms with separate reordering and Fourier-space kernels).
     """
 
     σ: float = 1.5
     m: int = 4
     device: int = 0
+    fft: str = "fused"
+
+    def __post_init__(self):
+        if self.fft not in ("fused", "rocfft"):
+            raise ValueError(f"fft must be 'fused' or 'rocfft', got {self.fft!r}")
 
     def has_real_to_complex(self) -> bool:
         return True
