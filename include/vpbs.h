@@ -80,8 +80,8 @@ typedef struct vpbs_params {
     int32_t avoid_explicit_erf; /* params.jl:289-298 */
     int32_t truncate_spherical; /* params.jl:243-246 */
     int32_t device;        /* HIP device ordinal */
-    int32_t fft;           /* VPBS_FFT_FUSED (default) or VPBS_FFT_ROCFFT; grids whose sizes the
-                              fused transforms do not cover (> 6144 FP32 / 3072 FP64 per axis) use rocFFT */
+    int32_t fft;           /* VPBS_FFT_FUSED (default) or VPBS_FFT_ROCFFT; grids the fused transforms
+                              do not cover (x, y > 6144 FP32 / 3072 FP64, z > 3072 / 1536) use rocFFT */
 } vpbs_params;
 
 /* Per-stage GPU times of the last vpbs_compute, in milliseconds (HIP events). */

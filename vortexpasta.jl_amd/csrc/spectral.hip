@@ -33,6 +33,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "vpbs_internal.hpp"
@@ -42,12 +43,19 @@ namespace {
 
 // threads per block of the x and y kernels (NTP) and of the z/Fourier kernels (NTZ)
 constexpr int NTP = 512;
-constexpr int NTZ = 512;
+constexpr int NTZ = 256;
 
+// complex values per tile: x and y kernels (one tile), z kernels (three tiles)
 template <typename CT>
 constexpr int spec_tile() {
     return sizeof(CT) == 4 ? 6144 : 3072;
 }
+template <typename CT>
+constexpr int spec_tile_z() {
+    return sizeof(CT) == 4 ? 3072 : 1536;
+}
+// LDS budget of the z kernels: two blocks per CU
+constexpr size_t SPEC_LDS_Z = 80 * 1024;
 
 template <typename CT>
 struct C2 {
@@ -88,9 +96,36 @@ __device__ __forceinline__ C2<CT> *lds(int off) {
 
 struct AxisPlan {
     int L;
-    int npass;
-    int radix[6];
+    int npass;  // -1: L has a prime factor above 5
+    int radix[8];
 };
+
+// Radices of a transform length: 16s first, then one of 8/4/2, then 3s and 5s.
+__host__ __device__ constexpr AxisPlan plan_axis(int L) {
+    AxisPlan a{L, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
+    int n = L;
+    while (n % 16 == 0 && a.npass < 8) a.radix[a.npass++] = 16, n /= 16;
+    if (n % 8 == 0 && a.npass < 8) a.radix[a.npass++] = 8, n /= 8;
+    if (n % 4 == 0 && a.npass < 8) a.radix[a.npass++] = 4, n /= 4;
+    if (n % 2 == 0 && a.npass < 8) a.radix[a.npass++] = 2, n /= 2;
+    while (n % 3 == 0 && a.npass < 8) a.radix[a.npass++] = 3, n /= 3;
+    while (n % 5 == 0 && a.npass < 8) a.radix[a.npass++] = 5, n /= 5;
+    if (n != 1) a.npass = -1;
+    return a;
+}
+
+// Tile width 2^lgb of the x/y kernels (KIND_P: one tile of ≤ spec_tile values) and
+// of the z kernels (KIND_Z: three tiles of ≤ spec_tile_z values within SPEC_LDS_Z).
+enum { KIND_P = 0, KIND_Z = 1 };
+template <typename CT>
+__host__ __device__ constexpr int tile_lgb(int L, int kind) {
+    const int tile = kind == KIND_Z ? spec_tile_z<CT>() : spec_tile<CT>();
+    int lg = 0;
+    while ((L << (lg + 1)) <= tile) lg++;
+    if (kind == KIND_Z)
+        while (lg > 0 && sizeof(CT) * 2 * (3 * ((size_t)L << lg) + (size_t)L) > SPEC_LDS_Z) lg--;
+    return lg;
+}
 
 // ---------------------------------------------------------------------------
 // Small DFTs in registers, X_k = Σ_n x_n e^{S·2πi nk/R} (S = -1 forward, +1 inverse).
@@ -216,7 +251,8 @@ template <typename CT, int R, int S, int NT>
 __device__ __noinline__ void stockham_pass(int s_off, int lgB, int L, int Ns, int tw_off) {
     C2<CT> *Sm = lds<CT>(s_off);
     const C2<CT> *tw = lds<CT>(tw_off);
-    constexpr int ITEMS = (spec_tile<CT>() / R + NT - 1) / NT;
+    constexpr int TILE = NT == NTZ ? spec_tile_z<CT>() : spec_tile<CT>();
+    constexpr int ITEMS = (TILE / R + NT - 1) / NT;
     const int nb = L / R;
     const int nwork = nb << lgB;
     const int B1 = (1 << lgB) - 1;
@@ -274,6 +310,70 @@ __device__ void lds_fft(int s_off, int lgB, const AxisPlan &ax, int tw_off) {
     }
 }
 
+// The same pass with the length, tile width and stride known at compile time (the
+// grid sizes of the common configurations): strides fold into immediate offsets
+// and the index arithmetic into shifts and constant multiplies.
+template <typename CT, int R, int S, int NT, int L, int LGB, int NS>
+__device__ __forceinline__ void stockham_static(int s_off, int tw_off) {
+    constexpr int NB = L / R, NWORK = NB << LGB, B1 = (1 << LGB) - 1, TWS = L / (NS * R);
+    constexpr int ITEMS = (NWORK + NT - 1) / NT;
+    C2<CT> *Sm = lds<CT>(s_off);
+    const C2<CT> *tw = lds<CT>(tw_off);
+    C2<CT> v[ITEMS][R];
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        const int w = threadIdx.x + NT * i;
+        if (NWORK % NT == 0 || w < NWORK) {
+            const int b = w & B1, j = w >> LGB;
+#pragma unroll
+            for (int k = 0; k < R; k++) v[i][k] = Sm[((j + k * NB) << LGB) + b];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        const int w = threadIdx.x + NT * i;
+        if (NWORK % NT == 0 || w < NWORK) {
+            const int b = w & B1, j = w >> LGB;
+            const int q = j / NS, jm = j - q * NS;
+            if constexpr (NS > 1) {
+                const int t1 = jm * TWS;
+#pragma unroll
+                for (int k = 1; k < R; k++) {
+                    C2<CT> t = tw[k * t1];
+                    if (S > 0) t.im = -t.im;
+                    v[i][k] = cmul(v[i][k], t);
+                }
+            }
+            dft<R, S>(v[i]);
+            const int base = q * NS * R + jm;
+#pragma unroll
+            for (int k = 0; k < R; k++) Sm[((base + k * NS) << LGB) + b] = v[i][k];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename CT, int S, int NT, int L, int LGB, int P = 0, int NS = 1>
+__device__ __forceinline__ void fft_static(int s_off, int tw_off) {
+    constexpr AxisPlan rd = plan_axis(L);
+    if constexpr (P < rd.npass) {
+        constexpr int R = rd.radix[P];
+        stockham_static<CT, R, S, NT, L, LGB, NS>(s_off, tw_off);
+        fft_static<CT, S, NT, L, LGB, P + 1, NS * R>(s_off, tw_off);
+    }
+}
+
+// Transform of every column of a tile: compile-time length LS, or LS = 0 for the
+// run-time plan ax with tile width 2^lgB.
+template <typename CT, int S, int NT, int LS, int KIND>
+__device__ __forceinline__ void fft_tile(int s_off, int lgB, const AxisPlan &ax, int tw_off) {
+    if constexpr (LS > 0)
+        fft_static<CT, S, NT, LS, tile_lgb<CT>(LS, KIND)>(s_off, tw_off);
+    else
+        lds_fft<CT, S, NT>(s_off, lgB, ax, tw_off);
+}
+
 // Blocks are dealt to the 8 XCDs round-robin; remap so that consecutive tiles
 // (which share cache lines) run on the same XCD. The grid is padded to a
 // multiple of 8 blocks; tiles ≥ n are idle.
@@ -292,7 +392,7 @@ __device__ __forceinline__ void load_twiddles(C2<CT> *twl, const C2<CT> *__restr
 // flight together. get(row, col, ok) returns the value (ok: e inside the tile).
 template <typename CT, int NT, typename F>
 __device__ __forceinline__ void tile_load(C2<CT> *Sm, int n, int lgB, F get) {
-    constexpr int EMAX = spec_tile<CT>() / NT;
+    constexpr int EMAX = (NT == NTZ ? spec_tile_z<CT>() : spec_tile<CT>()) / NT;
     C2<CT> t[EMAX];
 #pragma unroll
     for (int i = 0; i < EMAX; i++) {
@@ -308,7 +408,7 @@ __device__ __forceinline__ void tile_load(C2<CT> *Sm, int n, int lgB, F get) {
 
 template <typename CT, int NT, typename F>
 __device__ __forceinline__ void tile_store(const C2<CT> *Sm, int n, int lgB, F put) {
-    constexpr int EMAX = spec_tile<CT>() / NT;
+    constexpr int EMAX = (NT == NTZ ? spec_tile_z<CT>() : spec_tile<CT>()) / NT;
 #pragma unroll
     for (int i = 0; i < EMAX; i++) {
         const int e = threadIdx.x + NT * i;
@@ -326,10 +426,12 @@ struct SpecGeom {
 // ---------------------------------------------------------------------------
 // Forward x: tile = 2^lgB consecutive pair rows r (r = (c·G2 + z)·G1/2 + p).
 // The two real rows y = 2p, 2p + 1 of pair row r are rows 2r, 2r + 1 of x1.
-template <typename CT>
+template <typename CT, int LS>
 __global__ __launch_bounds__(NTP) void spec_x_forward(SpecGeom g, const C2<CT> *__restrict__ rows,
                                                       C2<CT> *__restrict__ x1, const C2<CT> *__restrict__ tw) {
-    const int L = g.G[0], lgB = g.lgB[0], B = 1 << lgB;
+    const int L = LS ? LS : g.G[0];
+    const int lgB = LS ? tile_lgb<CT>(LS, KIND_P) : g.lgB[0];
+    const int B = 1 << lgB;
     C2<CT> *Sm = lds<CT>(0);
     const int tw_off = L << lgB;
     C2<CT> *twl = lds<CT>(tw_off);
@@ -343,25 +445,26 @@ __global__ __launch_bounds__(NTP) void spec_x_forward(SpecGeom g, const C2<CT> *
     tile_load<CT, NTP>(Sm, L << lgB, lgB, [&](int x, int b) { return src[(int64_t)b * L + x]; });
     load_twiddles<CT, NTP>(twl, tw, L);
     __syncthreads();
-    lds_fft<CT, -1, NTP>(0, lgB, g.ax[0], tw_off);
+    fft_tile<CT, -1, NTP, LS, KIND_P>(0, lgB, g.ax[0], tw_off);
     const int nkx = g.nkx;
     C2<CT> *dst = x1 + 2 * r0 * nkx;
-    for (int b = 0; b < B; b++) {
-        for (int k = threadIdx.x; k < nkx; k += NTP) {
-            const C2<CT> zk = Sm[(k << lgB) + b];
-            const C2<CT> zm = Sm[((k == 0 ? 0 : L - k) << lgB) + b];
-            dst[(2 * b) * nkx + k] = C2<CT>{(zk.re + zm.re) * (CT)0.5, (zk.im - zm.im) * (CT)0.5};
-            dst[(2 * b + 1) * nkx + k] = C2<CT>{(zk.im + zm.im) * (CT)0.5, (zm.re - zk.re) * (CT)0.5};
-        }
+    for (int e = threadIdx.x; e < (nkx << lgB); e += NTP) {
+        const int b = e / nkx, k = e - b * nkx;
+        const C2<CT> zk = Sm[(k << lgB) + b];
+        const C2<CT> zm = Sm[((k == 0 ? 0 : L - k) << lgB) + b];
+        dst[(2 * b) * nkx + k] = C2<CT>{(zk.re + zm.re) * (CT)0.5, (zk.im - zm.im) * (CT)0.5};
+        dst[(2 * b + 1) * nkx + k] = C2<CT>{(zk.im + zm.im) * (CT)0.5, (zm.re - zk.re) * (CT)0.5};
     }
 }
 
 // Forward y: tile = (c, z, kx-block); x1[c][z][y][kx] → x2[c][jy][z][kx] for the
 // retained ky = jy (jy ≤ K1) or jy - nky.
-template <typename CT>
+template <typename CT, int LS>
 __global__ __launch_bounds__(NTP) void spec_y_forward(SpecGeom g, const C2<CT> *__restrict__ x1,
                                                       C2<CT> *__restrict__ x2, const C2<CT> *__restrict__ tw) {
-    const int L = g.G[1], lgB = g.lgB[1], B = 1 << lgB;
+    const int L = LS ? LS : g.G[1];
+    const int lgB = LS ? tile_lgb<CT>(LS, KIND_P) : g.lgB[1];
+    const int B = 1 << lgB;
     C2<CT> *Sm = lds<CT>(0);
     const int tw_off = L << lgB;
     C2<CT> *twl = lds<CT>(tw_off);
@@ -379,7 +482,7 @@ __global__ __launch_bounds__(NTP) void spec_y_forward(SpecGeom g, const C2<CT> *
     });
     load_twiddles<CT, NTP>(twl, tw, L);
     __syncthreads();
-    lds_fft<CT, -1, NTP>(0, lgB, g.ax[1], tw_off);
+    fft_tile<CT, -1, NTP, LS, KIND_P>(0, lgB, g.ax[1], tw_off);
     const int nky = g.nky, K1 = g.K1;
     C2<CT> *dst = x2 + ((int64_t)c * nky * g.G[2] + z) * nkx + kx0;
     const int64_t jstride = (int64_t)g.G[2] * nkx;
@@ -411,11 +514,13 @@ struct SpecFourier {
 // velocity and x2b the streamfunction (each when requested), in z space, ready
 // for the inverse y, x passes. With both outputs, the velocity spectrum makes a
 // round trip through x2 while the tiles transform the streamfunction.
-template <typename CT>
+template <typename CT, int LS>
 __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f, C2<CT> *__restrict__ x2,
                                                       C2<CT> *__restrict__ x2b, const C2<CT> *__restrict__ tw) {
-    constexpr int EMAX = spec_tile<CT>() / NTZ;
-    const int L = g.G[2], lgB = g.lgB[2], B = 1 << lgB;
+    constexpr int EMAX = spec_tile_z<CT>() / NTZ;
+    const int L = LS ? LS : g.G[2];
+    const int lgB = LS ? tile_lgb<CT>(LS, KIND_Z) : g.lgB[2];
+    const int B = 1 << lgB;
     const int n = L << lgB;
     C2<CT> *T[3] = {lds<CT>(0), lds<CT>(n), lds<CT>(2 * n)};
     const int tw_off = 3 * n;
@@ -437,7 +542,7 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
         });
     }
     __syncthreads();
-    for (int c = 0; c < 3; c++) lds_fft<CT, -1, NTZ>(c * n, lgB, g.ax[2], tw_off);
+    for (int c = 0; c < 3; c++) fft_tile<CT, -1, NTZ, LS, KIND_Z>(c * n, lgB, g.ax[2], tw_off);
     // Fourier-space operators on the retained modes
     const int K1 = g.K1, K2 = g.K2, nkz = g.nkz;
     const int ky = jy <= K1 ? jy : jy - nky;
@@ -521,7 +626,7 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
     }
     __syncthreads();
     C2<CT> *out = f.do_psi ? x2b : x2;
-    for (int c = 0; c < 3; c++) lds_fft<CT, 1, NTZ>(c * n, lgB, g.ax[2], tw_off);
+    for (int c = 0; c < 3; c++) fft_tile<CT, 1, NTZ, LS, KIND_Z>(c * n, lgB, g.ax[2], tw_off);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         C2<CT> *dst = out + c * cs + off;
@@ -541,7 +646,7 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
             });
         }
         __syncthreads();
-        for (int c = 0; c < 3; c++) lds_fft<CT, 1, NTZ>(c * n, lgB, g.ax[2], tw_off);
+        for (int c = 0; c < 3; c++) fft_tile<CT, 1, NTZ, LS, KIND_Z>(c * n, lgB, g.ax[2], tw_off);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             C2<CT> *dst = x2 + c * cs + off;
@@ -554,10 +659,12 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
 
 // Band-filtered kept field in z space: the kept û on kmin² ≤ k² ≤ kmax² times the
 // type-2 deconvolution, zero elsewhere, inverse FFT_z → x2. Tile = (jy, kx-block).
-template <typename CT>
+template <typename CT, int LS>
 __global__ __launch_bounds__(NTZ) void spec_z_band(SpecGeom g, SpecFourier f, double kmin2, double kmax2,
                                                    C2<CT> *__restrict__ x2, const C2<CT> *__restrict__ tw) {
-    const int L = g.G[2], lgB = g.lgB[2], B = 1 << lgB;
+    const int L = LS ? LS : g.G[2];
+    const int lgB = LS ? tile_lgb<CT>(LS, KIND_Z) : g.lgB[2];
+    const int B = 1 << lgB;
     C2<CT> *Sm = lds<CT>(0);
     const int tw_off = L << lgB;
     C2<CT> *twl = lds<CT>(tw_off);
@@ -593,7 +700,7 @@ __global__ __launch_bounds__(NTZ) void spec_z_band(SpecGeom g, SpecFourier f, do
             return v;
         });
         __syncthreads();
-        lds_fft<CT, 1, NTZ>(0, lgB, g.ax[2], tw_off);
+        fft_tile<CT, 1, NTZ, LS, KIND_Z>(0, lgB, g.ax[2], tw_off);
         C2<CT> *dst = x2 + c * cs + off;
         tile_store<CT, NTZ>(Sm, n, lgB, [&](int z, int b, C2<CT> v) {
             if (kx0 + b < nkx) dst[(int64_t)z * nkx + b] = v;
@@ -604,10 +711,12 @@ __global__ __launch_bounds__(NTZ) void spec_z_band(SpecGeom g, SpecFourier f, do
 
 // Inverse y: tile = (c, z, kx-block); x2[c][jy][z][kx] on the retained ky (zero
 // elsewhere) → inverse FFT_y → x1[c][z][y][kx].
-template <typename CT>
+template <typename CT, int LS>
 __global__ __launch_bounds__(NTP) void spec_y_inverse(SpecGeom g, const C2<CT> *__restrict__ x2,
                                                       C2<CT> *__restrict__ x1, const C2<CT> *__restrict__ tw) {
-    const int L = g.G[1], lgB = g.lgB[1], B = 1 << lgB;
+    const int L = LS ? LS : g.G[1];
+    const int lgB = LS ? tile_lgb<CT>(LS, KIND_P) : g.lgB[1];
+    const int B = 1 << lgB;
     C2<CT> *Sm = lds<CT>(0);
     const int tw_off = L << lgB;
     C2<CT> *twl = lds<CT>(tw_off);
@@ -629,7 +738,7 @@ __global__ __launch_bounds__(NTP) void spec_y_inverse(SpecGeom g, const C2<CT> *
     });
     load_twiddles<CT, NTP>(twl, tw, L);
     __syncthreads();
-    lds_fft<CT, 1, NTP>(0, lgB, g.ax[1], tw_off);
+    fft_tile<CT, 1, NTP, LS, KIND_P>(0, lgB, g.ax[1], tw_off);
     C2<CT> *dst = x1 + cz * L * nkx + kx0;
     tile_store<CT, NTP>(Sm, L << lgB, lgB, [&](int y, int b, C2<CT> v) {
         if (kx0 + b < nkx) dst[(int64_t)y * nkx + b] = v;
@@ -639,10 +748,12 @@ __global__ __launch_bounds__(NTP) void spec_y_inverse(SpecGeom g, const C2<CT> *
 // Inverse x: tile = 2^lgB pair rows; Z_k = A_k + i B_k for k < nkx (A, B from rows
 // 2r, 2r + 1 of x1), their conjugates at G0 - k (Hermitian symmetry of real rows),
 // zero otherwise, DC keeps only its real parts; inverse FFT_x → pair rows.
-template <typename CT>
+template <typename CT, int LS>
 __global__ __launch_bounds__(NTP) void spec_x_inverse(SpecGeom g, const C2<CT> *__restrict__ x1,
                                                       C2<CT> *__restrict__ rows, const C2<CT> *__restrict__ tw) {
-    const int L = g.G[0], lgB = g.lgB[0], B = 1 << lgB;
+    const int L = LS ? LS : g.G[0];
+    const int lgB = LS ? tile_lgb<CT>(LS, KIND_P) : g.lgB[0];
+    const int B = 1 << lgB;
     C2<CT> *Sm = lds<CT>(0);
     const int tw_off = L << lgB;
     C2<CT> *twl = lds<CT>(tw_off);
@@ -653,22 +764,37 @@ __global__ __launch_bounds__(NTP) void spec_x_inverse(SpecGeom g, const C2<CT> *
     const int64_t r0 = tile << lgB;
     const int nkx = g.nkx;
     const C2<CT> *src = x1 + 2 * r0 * nkx;
-    tile_load<CT, NTP>(Sm, L << lgB, lgB, [&](int k, int b) {
-        const C2<CT> *ra = src + (2 * b) * nkx, *rb = ra + nkx;
-        C2<CT> zk{(CT)0, (CT)0};
-        if (k < nkx) {
-            C2<CT> A = ra[k], Bv = rb[k];
-            if (k == 0) A.im = Bv.im = (CT)0;
-            zk = C2<CT>{A.re - Bv.im, A.im + Bv.re};
-        } else if (k > L - nkx) {
-            const C2<CT> A = ra[L - k], Bv = rb[L - k];  // conj(A) + i conj(B)
-            zk = C2<CT>{A.re + Bv.im, Bv.re - A.im};
+    // each (A_k, B_k) is read once and gives both Z_k and Z_{G0-k}
+    {
+        constexpr int EMAX = spec_tile<CT>() / NTP;
+        const int nin = nkx << lgB;
+        C2<CT> a[EMAX], bb[EMAX];
+#pragma unroll
+        for (int i = 0; i < EMAX; i++) {
+            const int e = threadIdx.x + NTP * i;
+            if (e < nin) {
+                const int k = e >> lgB, b = e & (B - 1);
+                a[i] = src[(2 * b) * nkx + k];
+                bb[i] = src[(2 * b + 1) * nkx + k];
+            }
         }
-        return zk;
-    });
+#pragma unroll
+        for (int i = 0; i < EMAX; i++) {
+            const int e = threadIdx.x + NTP * i;
+            if (e < nin) {
+                const int k = e >> lgB, b = e & (B - 1);
+                C2<CT> A = a[i], Bv = bb[i];
+                if (k == 0) A.im = Bv.im = (CT)0;
+                Sm[e] = C2<CT>{A.re - Bv.im, A.im + Bv.re};
+                if (k > 0) Sm[((L - k) << lgB) + b] = C2<CT>{A.re + Bv.im, Bv.re - A.im};  // conj(A) + i conj(B)
+            }
+        }
+        // modes kx ∈ [nkx, G0 - nkx] are not retained
+        for (int e = (nkx << lgB) + threadIdx.x; e < ((L - nkx + 1) << lgB); e += NTP) Sm[e] = C2<CT>{(CT)0, (CT)0};
+    }
     load_twiddles<CT, NTP>(twl, tw, L);
     __syncthreads();
-    lds_fft<CT, 1, NTP>(0, lgB, g.ax[0], tw_off);
+    fft_tile<CT, 1, NTP, LS, KIND_P>(0, lgB, g.ax[0], tw_off);
     C2<CT> *dst = rows + r0 * L;
     tile_store<CT, NTP>(Sm, L << lgB, lgB, [&](int x, int b, C2<CT> v) { dst[(int64_t)b * L + x] = v; });
 }
@@ -676,31 +802,13 @@ __global__ __launch_bounds__(NTP) void spec_x_inverse(SpecGeom g, const C2<CT> *
 // ---------------------------------------------------------------------------
 // Host side
 
-AxisPlan plan_axis(int L) {
-    AxisPlan a{};
-    a.L = L;
-    int n = L, np = 0;
-    while (n % 16 == 0 && np < 6) a.radix[np++] = 16, n /= 16;
-    if (n % 8 == 0 && np < 6) a.radix[np++] = 8, n /= 8;
-    if (n % 4 == 0 && np < 6) a.radix[np++] = 4, n /= 4;
-    if (n % 2 == 0 && np < 6) a.radix[np++] = 2, n /= 2;
-    while (n % 3 == 0 && np < 6) a.radix[np++] = 3, n /= 3;
-    while (n % 5 == 0 && np < 6) a.radix[np++] = 5, n /= 5;
-    a.npass = n == 1 ? np : -1;
-    return a;
-}
-
 template <typename CT>
 SpecGeom spec_geom(const vpbs_ctx *c) {
     SpecGeom g{};
     for (int d = 0; d < 3; d++) {
         g.G[d] = c->G[d];
         g.ax[d] = plan_axis(c->G[d]);
-        int lg = 0;
-        while ((c->G[d] << (lg + 1)) <= spec_tile<CT>()) lg++;
-        if (d == 2)
-            while (lg > 0 && sizeof(C2<CT>) * (3 * ((size_t)c->G[d] << lg) + c->G[d]) > c->spec_lds_max) lg--;
-        g.lgB[d] = lg;
+        g.lgB[d] = tile_lgb<CT>(c->G[d], d == 2 ? KIND_Z : KIND_P);
     }
     g.nkx = c->nkx;
     g.nky = c->nky;
@@ -747,16 +855,29 @@ SpecFourier spec_fourier_args(const vpbs_ctx *c) {
     return f;
 }
 
+// Transform lengths compiled with static plans (the σ = 1.5 grids of Ns = 128, 256,
+// 512 and 1024); other lengths run the run-time plan.
+#define SPEC_LAUNCH(len, kern, grid, nt, lds, ...)                          \
+    do {                                                                    \
+        switch (len) {                                                      \
+            case 192: kern<CT, 192><<<(grid), (nt), (lds), s>>>(__VA_ARGS__); break;   \
+            case 384: kern<CT, 384><<<(grid), (nt), (lds), s>>>(__VA_ARGS__); break;   \
+            case 768: kern<CT, 768><<<(grid), (nt), (lds), s>>>(__VA_ARGS__); break;   \
+            case 1536: kern<CT, 1536><<<(grid), (nt), (lds), s>>>(__VA_ARGS__); break; \
+            default: kern<CT, 0><<<(grid), (nt), (lds), s>>>(__VA_ARGS__); break;      \
+        }                                                                   \
+    } while (0)
+
 template <typename CT>
 int forward_impl(vpbs_ctx *c) {
     hipStream_t s = c->s_lr;
     const SpecGeom g = spec_geom<CT>(c);
     const int64_t nrows = 3 * (int64_t)c->G[2] * (c->G[1] / 2);
     const int64_t nkb = (c->nkx + (1 << g.lgB[1]) - 1) >> g.lgB[1];
-    spec_x_forward<CT><<<grid8(nrows >> g.lgB[0]), NTP, spec_lds<CT>(g, 0), s>>>(
-        g, (const C2<CT> *)c->grid, (C2<CT> *)c->ypen, twiddles<CT>(c, 0));
-    spec_y_forward<CT><<<grid8(3 * (int64_t)c->G[2] * nkb), NTP, spec_lds<CT>(g, 1), s>>>(
-        g, (const C2<CT> *)c->ypen, (C2<CT> *)c->zpen, twiddles<CT>(c, 1));
+    SPEC_LAUNCH(c->G[0], spec_x_forward, grid8(nrows >> g.lgB[0]), NTP, spec_lds<CT>(g, 0),
+                g, (const C2<CT> *)c->grid, (C2<CT> *)c->ypen, twiddles<CT>(c, 0));
+    SPEC_LAUNCH(c->G[1], spec_y_forward, grid8(3 * (int64_t)c->G[2] * nkb), NTP, spec_lds<CT>(g, 1),
+                g, (const C2<CT> *)c->ypen, (C2<CT> *)c->zpen, twiddles<CT>(c, 1));
     VPBS_HIP(c, hipGetLastError());
     return VPBS_OK;
 }
@@ -771,8 +892,8 @@ int fourier_impl(vpbs_ctx *c, uint32_t flags) {
     f.keep_uhat = (flags & VPBS_KEEP_UHAT) ? 1 : 0;
     const int64_t nkb = (c->nkx + (1 << g.lgB[2]) - 1) >> g.lgB[2];
     C2<CT> *x2b = f.do_psi ? (C2<CT> *)c->zpen2 : nullptr;
-    spec_z_fourier<CT><<<grid8((int64_t)c->nky * nkb), NTZ, spec_lds<CT>(g, 2, 3), s>>>(
-        g, f, (C2<CT> *)c->zpen, x2b, twiddles<CT>(c, 2));
+    SPEC_LAUNCH(c->G[2], spec_z_fourier, grid8((int64_t)c->nky * nkb), NTZ, spec_lds<CT>(g, 2, 3),
+                g, f, (C2<CT> *)c->zpen, x2b, twiddles<CT>(c, 2));
     VPBS_HIP(c, hipGetLastError());
     return VPBS_OK;
 }
@@ -783,10 +904,10 @@ int inverse_impl(vpbs_ctx *c, void *zp) {
     const SpecGeom g = spec_geom<CT>(c);
     const int64_t nrows = 3 * (int64_t)c->G[2] * (c->G[1] / 2);
     const int64_t nkb = (c->nkx + (1 << g.lgB[1]) - 1) >> g.lgB[1];
-    spec_y_inverse<CT><<<grid8(3 * (int64_t)c->G[2] * nkb), NTP, spec_lds<CT>(g, 1), s>>>(
-        g, (const C2<CT> *)zp, (C2<CT> *)c->ypen, twiddles<CT>(c, 1));
-    spec_x_inverse<CT><<<grid8(nrows >> g.lgB[0]), NTP, spec_lds<CT>(g, 0), s>>>(
-        g, (const C2<CT> *)c->ypen, (C2<CT> *)c->grid, twiddles<CT>(c, 0));
+    SPEC_LAUNCH(c->G[1], spec_y_inverse, grid8(3 * (int64_t)c->G[2] * nkb), NTP, spec_lds<CT>(g, 1),
+                g, (const C2<CT> *)zp, (C2<CT> *)c->ypen, twiddles<CT>(c, 1));
+    SPEC_LAUNCH(c->G[0], spec_x_inverse, grid8(nrows >> g.lgB[0]), NTP, spec_lds<CT>(g, 0),
+                g, (const C2<CT> *)c->ypen, (C2<CT> *)c->grid, twiddles<CT>(c, 0));
     VPBS_HIP(c, hipGetLastError());
     return VPBS_OK;
 }
@@ -797,8 +918,8 @@ int band_impl(vpbs_ctx *c, double kmin2, double kmax2) {
     const SpecGeom g = spec_geom<CT>(c);
     const SpecFourier f = spec_fourier_args<CT>(c);
     const int64_t nkb = (c->nkx + (1 << g.lgB[2]) - 1) >> g.lgB[2];
-    spec_z_band<CT><<<grid8((int64_t)c->nky * nkb), NTZ, spec_lds<CT>(g, 2), s>>>(
-        g, f, kmin2, kmax2, (C2<CT> *)c->zpen, twiddles<CT>(c, 2));
+    SPEC_LAUNCH(c->G[2], spec_z_band, grid8((int64_t)c->nky * nkb), NTZ, spec_lds<CT>(g, 2),
+                g, f, kmin2, kmax2, (C2<CT> *)c->zpen, twiddles<CT>(c, 2));
     VPBS_HIP(c, hipGetLastError());
     return inverse_impl<CT>(c, c->zpen);
 }
@@ -831,22 +952,33 @@ int setup_impl(vpbs_ctx *c) {
     auto attr = [&](const void *k, size_t lds) {
         return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     };
-    VPBS_HIP(c, attr((const void *)spec_x_forward<CT>, lds_p));
-    VPBS_HIP(c, attr((const void *)spec_y_forward<CT>, lds_p));
-    VPBS_HIP(c, attr((const void *)spec_z_fourier<CT>, lds_z));
-    VPBS_HIP(c, attr((const void *)spec_z_band<CT>, lds_z));
-    VPBS_HIP(c, attr((const void *)spec_y_inverse<CT>, lds_p));
-    VPBS_HIP(c, attr((const void *)spec_x_inverse<CT>, lds_p));
+    auto attrs = [&](auto tag) -> int {
+        constexpr int LS = decltype(tag)::value;
+        VPBS_HIP(c, attr((const void *)spec_x_forward<CT, LS>, lds_p));
+        VPBS_HIP(c, attr((const void *)spec_y_forward<CT, LS>, lds_p));
+        VPBS_HIP(c, attr((const void *)spec_z_fourier<CT, LS>, lds_z));
+        VPBS_HIP(c, attr((const void *)spec_z_band<CT, LS>, lds_z));
+        VPBS_HIP(c, attr((const void *)spec_y_inverse<CT, LS>, lds_p));
+        VPBS_HIP(c, attr((const void *)spec_x_inverse<CT, LS>, lds_p));
+        return VPBS_OK;
+    };
+    VPBS_TRY(attrs(std::integral_constant<int, 0>{}));
+    VPBS_TRY(attrs(std::integral_constant<int, 192>{}));
+    VPBS_TRY(attrs(std::integral_constant<int, 384>{}));
+    VPBS_TRY(attrs(std::integral_constant<int, 768>{}));
+    VPBS_TRY(attrs(std::integral_constant<int, 1536>{}));
     return VPBS_OK;
 }
 
 }  // namespace
 
 bool spectral_supported(const vpbs_ctx *c) {
-    const int tile = c->p.precision == VPBS_F32 ? spec_tile<float>() : spec_tile<double>();
+    const bool f32 = c->p.precision == VPBS_F32;
+    const int tile = f32 ? spec_tile<float>() : spec_tile<double>();
+    const int tile_z = f32 ? spec_tile_z<float>() : spec_tile_z<double>();
     for (int d = 0; d < 3; d++) {
         const AxisPlan a = plan_axis(c->G[d]);
-        if (a.npass < 0 || c->G[d] > tile) return false;
+        if (a.npass < 0 || c->G[d] > (d == 2 ? tile_z : tile)) return false;
     }
     return true;
 }

@@ -227,7 +227,7 @@ class HIPLongRangeBackend:
     (``nonuniformffts.jl:107-120``); the defaults give ~6-digit accuracy.
     ``fft`` selects the transforms: ``"fused"`` (hand-written FFTs fused with the
     pencil reorderings and the Fourier-space kernel; grids up to 6144 (FP32) or
-    3072 (FP64) per axis, larger ones use rocFFT) or ``"rocfft"`` (rocFFT 1D
+    3072 (FP64) along x and y and half that along z, larger ones use rocFFT) or ``"rocfft"`` (rocFFT 1D
     transforms with separate reordering and Fourier-space kernels).
     """
 
