@@ -630,3 +630,20 @@ def test_rocfft_path_uhat_and_band(vp, O, cfg2):
     got = cache.ctx.interpolate_band(2.0, 6.0)
     assert rel_l2(got, O.band_field(u, params, pd.nodes, 2.0, 6.0, "nufft")) <= TOL64
     cache.close()
+
+
+@pytest.mark.parametrize("Ns", [256, 512])
+def test_fft_paths_agree_static_lengths(vp, Ns):
+    """Grids 384³ and 768³ (compile-time transform plans): fused vs rocFFT long range on a few rings,
+    FP64 to 1e-12 and FP32 to 2e-6."""
+    fs = vp.tangles.random_rings(8, 0.5, 1.0, TWO_PI / 128, seed=11)
+    for T, tol in ((np.float64, 1e-12), (np.float32, 2e-6)):
+        outs = {}
+        for fft in ("fused", "rocfft"):
+            params = _params(vp, T=T, Ns=Ns, backend_long=vp.HIPLongRangeBackend(fft=fft))
+            cache, outs[fft] = _gpu(vp, params, fs, shortrange=False, LIA=False)
+            cache.close()
+        for k in Q:
+            err = rel_l2(outs["fused"][k], outs["rocfft"][k])
+            print(f"  Ns={Ns} {T.__name__} {k}: fused vs rocFFT {err:.2e}", flush=True)
+            assert err <= tol

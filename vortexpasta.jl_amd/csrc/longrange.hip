@@ -192,8 +192,10 @@ __device__ __forceinline__ void static_for(F &&f) {
 //  2) the candidates are tested 64 at a time for support overlap; the kept ones
 //     are compacted in order and their records (weights, charge) copied to LDS;
 //  3) the wavefront walks the kept points: each lane weights the charge by its
-//     row's wy·wz, and adds wx[k]·(…) to cells sx + k through a switch on the
-//     uniform offset sx whose cases use compile-time register indices.
+//     row's wy·wz and adds w16[x]·(…) to all TX cells of the row, where w16 holds
+//     the point's x-weights already shifted to the tile columns (zeros elsewhere,
+//     written when the record is staged): branch-free, compile-time register
+//     indices, every weight read is an LDS broadcast.
 // Summation order is fixed (neighbour tile, then sorted order): results are
 // deterministic. The tile is written once, transposed through LDS into
 // contiguous pair rows.
@@ -206,8 +208,14 @@ __global__ __launch_bounds__(64) void spread_rows_kernel(LRGeom g, int64_t Nq, c
     constexpr int REC = rec_len(W);
     constexpr int TXP = TX + 1;  // LDS row pitch (odd: conflict-free column writes)
     constexpr int NPAIR = TY / 2;
-    __shared__ CT buf[TY * TZ * TXP];
-    __shared__ CT s_rec[64][REC];
+    // staged kept points: y/z weights and charge (YZQ values), x-weights at their tile
+    // columns (TX values); the write-out buffer reuses the same LDS afterwards
+    constexpr int YZQ = 2 * W + 3;
+    static_assert(TY * TZ * TXP <= 64 * (YZQ + TX), "write-out buffer fits the staging area");
+    __shared__ CT s_stage[64 * (YZQ + TX)];
+    CT(*s_rec)[YZQ] = reinterpret_cast<CT(*)[YZQ]>(s_stage);
+    CT(*s_w16)[TX] = reinterpret_cast<CT(*)[TX]>(s_stage + 64 * YZQ);
+    CT *buf = s_stage;
     __shared__ int4 s_meta[64];  // (sx, ry, rz, -) of the kept points
     __shared__ int nb_start[27];
     __shared__ int nb_pre[28];
@@ -279,6 +287,8 @@ __global__ __launch_bounds__(64) void spread_rows_kernel(LRGeom g, int64_t Nq, c
                 keep = (rx < TX || wrap1(t0x - ix, g.G[0]) < W) && (ry < TY || wrap1(t0y - iy, g.G[1]) < W) &&
                        (rz < TZ || wrap1(t0z - iz, g.G[2]) < W);
                 sx = rx < TX ? rx : rx - g.G[0];  // ∈ (-W, TX)
+                ry = ry < TY ? ry : ry - g.G[1];  // ∈ (-W, TY), likewise below
+                rz = rz < TZ ? rz : rz - g.G[2];
             }
             const unsigned long long mask = __ballot(keep);
             const int nkept = __popcll(mask);
@@ -287,50 +297,34 @@ __global__ __launch_bounds__(64) void spread_rows_kernel(LRGeom g, int64_t Nq, c
                 s_meta[pos] = make_int4(sx, ry, rz, 0);
                 const CT *pr = rec + (int64_t)sidx * REC;
 #pragma unroll
-                for (int k = 0; k < REC; k++) s_rec[pos][k] = pr[k];
+                for (int k = 0; k < YZQ; k++) s_rec[pos][k] = pr[W + k];
+#pragma unroll
+                for (int x = 0; x < TX; x++) s_w16[pos][x] = (CT)0;
+                // x-weights at their tile columns sx + k (mod TX when one tile spans the period)
+#pragma unroll
+                for (int k = 0; k < W; k++) {
+                    int x = sx + k;
+                    if (g.G[0] == TX) x = wrap1(x, TX);
+                    if (x >= 0 && x < TX) s_w16[pos][x] = pr[k];
+                }
             }
             __syncthreads();
-            // 3) accumulate the kept points in order
+            // 3) accumulate the kept points in order: acc[x] += w16[x]·(wy·wz·q) over the
+            //    whole row, w16 = the point's x-weights shifted to the tile (zero outside)
             for (int p = 0; p < nkept; p++) {
                 const int4 m = s_meta[p];
-                const int ky = wrap1(ly - m.y, g.G[1]), kz = wrap1(lz - m.z, g.G[2]);
+                const int ky = ly - m.y, kz = lz - m.z;
                 CT wyz = (CT)0;
-                if (ky < W && kz < W) wyz = s_rec[p][W + ky] * s_rec[p][2 * W + kz];
-                const CT a0 = wyz * s_rec[p][3 * W], a1 = wyz * s_rec[p][3 * W + 1], a2 = wyz * s_rec[p][3 * W + 2];
-                CT wx[W];
-                static_for<0, W>([&](auto K) { wx[decltype(K)::value] = s_rec[p][decltype(K)::value]; });
-                // acc[sx + k] += wx[k]·a: one straight-line case per uniform offset,
-                // all indices compile-time constants (the accumulators stay in registers)
-                auto leaf = [&](auto S) {
-                    constexpr int S0 = decltype(S)::value;
-                    static_for<0, W>([&](auto K) {
-                        constexpr int k = decltype(K)::value, x = S0 + k;
-                        if constexpr (x >= 0 && x < TX) {
-                            acc[x][0] += wx[k] * a0;
-                            acc[x][1] += wx[k] * a1;
-                            acc[x][2] += wx[k] * a2;
-                        }
-                    });
-                };
-                // a single tile along x: the support can also wrap back into the same tile
-                const int sx2 = (g.G[0] == TX && m.x + W > TX) ? m.x - TX : TX;
-                for (int o = m.x; o < TX; o = (o == m.x ? sx2 : TX)) {
-#define VPBS_SX_CASE(v) \
-    case v: leaf(std::integral_constant<int, v>{}); break;
-                    switch (o) {
-                        VPBS_SX_CASE(-15) VPBS_SX_CASE(-14) VPBS_SX_CASE(-13) VPBS_SX_CASE(-12)
-                        VPBS_SX_CASE(-11) VPBS_SX_CASE(-10) VPBS_SX_CASE(-9) VPBS_SX_CASE(-8)
-                        VPBS_SX_CASE(-7) VPBS_SX_CASE(-6) VPBS_SX_CASE(-5) VPBS_SX_CASE(-4)
-                        VPBS_SX_CASE(-3) VPBS_SX_CASE(-2) VPBS_SX_CASE(-1) VPBS_SX_CASE(0)
-                        VPBS_SX_CASE(1) VPBS_SX_CASE(2) VPBS_SX_CASE(3) VPBS_SX_CASE(4)
-                        VPBS_SX_CASE(5) VPBS_SX_CASE(6) VPBS_SX_CASE(7) VPBS_SX_CASE(8)
-                        VPBS_SX_CASE(9) VPBS_SX_CASE(10) VPBS_SX_CASE(11) VPBS_SX_CASE(12)
-                        VPBS_SX_CASE(13) VPBS_SX_CASE(14) VPBS_SX_CASE(15)
-                        default: break;
-                    }
-#undef VPBS_SX_CASE
-                }
-                static_assert(W <= 16 && TX <= 16, "offset cases cover (-16, 16)");
+                if ((unsigned)ky < (unsigned)W && (unsigned)kz < (unsigned)W)
+                    wyz = s_rec[p][ky] * s_rec[p][W + kz];
+                const CT a0 = wyz * s_rec[p][2 * W], a1 = wyz * s_rec[p][2 * W + 1], a2 = wyz * s_rec[p][2 * W + 2];
+                static_for<0, TX>([&](auto X) {
+                    constexpr int x = decltype(X)::value;
+                    const CT w = s_w16[p][x];
+                    acc[x][0] += w * a0;
+                    acc[x][1] += w * a1;
+                    acc[x][2] += w * a2;
+                });
             }
             __syncthreads();
         }
