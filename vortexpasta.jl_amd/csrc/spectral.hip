@@ -41,19 +41,16 @@
 namespace vpbs {
 namespace {
 
-// threads per block of the x and y kernels (NTP) and of the z/Fourier kernels (NTZ)
-constexpr int NTP = 512;
-constexpr int NTZ = 256;
-
-// complex values per tile: x and y kernels (one tile), z kernels (three tiles)
+// Kernel kinds: x kernels (rows along the contiguous axis), y kernels, z kernels
+// (three component tiles). Threads per block and complex values per tile of each;
+// the tiles are sized for several blocks per CU (x: 4-5, y: 2, z: 2).
+enum { KIND_X = 0, KIND_Y = 1, KIND_Z = 2 };
+constexpr int kind_nt(int kind) { return kind == KIND_Y ? 512 : 256; }
 template <typename CT>
-constexpr int spec_tile() {
-    return sizeof(CT) == 4 ? 6144 : 3072;
+constexpr int kind_tile(int kind) {
+    return (sizeof(CT) == 4 ? 6144 : 3072) / (kind == KIND_Y ? 1 : 2);
 }
-template <typename CT>
-constexpr int spec_tile_z() {
-    return sizeof(CT) == 4 ? 3072 : 1536;
-}
+constexpr int NTX = kind_nt(KIND_X), NTY = kind_nt(KIND_Y), NTZ = kind_nt(KIND_Z);
 // LDS budget of the z kernels: two blocks per CU
 constexpr size_t SPEC_LDS_Z = 80 * 1024;
 
@@ -61,6 +58,41 @@ template <typename CT>
 struct C2 {
     CT re, im;
 };
+
+// two consecutive complex values, moved with 16-byte vector accesses (one in FP32,
+// two in FP64) by load_pair / store_pair; p must be 16-byte aligned
+template <typename CT>
+struct C2x2 {
+    C2<CT> a, b;
+};
+
+// Pair transfers pay off in FP32 (one 16-byte access instead of two 8-byte ones);
+// FP64 values are 16 bytes each already.
+template <typename CT>
+constexpr bool use_pairs() {
+    return sizeof(CT) == 4;
+}
+
+template <typename CT>
+__device__ __forceinline__ C2x2<CT> load_pair(const C2<CT> *p) {
+    if constexpr (sizeof(CT) == 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(p);
+        return {{v.x, v.y}, {v.z, v.w}};
+    } else {
+        const double2 u = reinterpret_cast<const double2 *>(p)[0], v = reinterpret_cast<const double2 *>(p)[1];
+        return {{u.x, u.y}, {v.x, v.y}};
+    }
+}
+
+template <typename CT>
+__device__ __forceinline__ void store_pair(C2<CT> *p, const C2x2<CT> &v) {
+    if constexpr (sizeof(CT) == 4) {
+        *reinterpret_cast<float4 *>(p) = float4{v.a.re, v.a.im, v.b.re, v.b.im};
+    } else {
+        reinterpret_cast<double2 *>(p)[0] = double2{v.a.re, v.a.im};
+        reinterpret_cast<double2 *>(p)[1] = double2{v.b.re, v.b.im};
+    }
+}
 
 template <typename CT>
 __device__ __forceinline__ C2<CT> cadd(C2<CT> a, C2<CT> b) {
@@ -114,12 +146,11 @@ __host__ __device__ constexpr AxisPlan plan_axis(int L) {
     return a;
 }
 
-// Tile width 2^lgb of the x/y kernels (KIND_P: one tile of ≤ spec_tile values) and
-// of the z kernels (KIND_Z: three tiles of ≤ spec_tile_z values within SPEC_LDS_Z).
-enum { KIND_P = 0, KIND_Z = 1 };
+// Tile width 2^lgb of a kernel kind: the largest with L·2^lgb ≤ kind_tile (for the z
+// kernels also three tiles and the twiddles within SPEC_LDS_Z, down to one column).
 template <typename CT>
 __host__ __device__ constexpr int tile_lgb(int L, int kind) {
-    const int tile = kind == KIND_Z ? spec_tile_z<CT>() : spec_tile<CT>();
+    const int tile = kind_tile<CT>(kind);
     int lg = 0;
     while ((L << (lg + 1)) <= tile) lg++;
     if (kind == KIND_Z)
@@ -247,11 +278,11 @@ __device__ __forceinline__ void dft(C2<CT> (&v)[R]) {
 // Ns = product of the radices of the previous passes; tw[n] = e^{-2πin/L}.
 // Kept out of line: inlined into the radix switch, the passes' live ranges
 // merge and the kernels need ~50% more registers.
-template <typename CT, int R, int S, int NT>
+template <typename CT, int R, int S, int KIND>
 __device__ __noinline__ void stockham_pass(int s_off, int lgB, int L, int Ns, int tw_off) {
     C2<CT> *Sm = lds<CT>(s_off);
     const C2<CT> *tw = lds<CT>(tw_off);
-    constexpr int TILE = NT == NTZ ? spec_tile_z<CT>() : spec_tile<CT>();
+    constexpr int NT = kind_nt(KIND), TILE = kind_tile<CT>(KIND);
     constexpr int ITEMS = (TILE / R + NT - 1) / NT;
     const int nb = L / R;
     const int nwork = nb << lgB;
@@ -293,18 +324,18 @@ __device__ __noinline__ void stockham_pass(int s_off, int lgB, int L, int Ns, in
 
 // Whole transform of every column of the tile (natural order in and out).
 // s_off, tw_off: element offsets of the tile and of the twiddle table in spec_smem.
-template <typename CT, int S, int NT>
+template <typename CT, int S, int KIND>
 __device__ void lds_fft(int s_off, int lgB, const AxisPlan &ax, int tw_off) {
     int Ns = 1;
     for (int p = 0; p < ax.npass; p++) {
         const int R = ax.radix[p];
         switch (R) {
-            case 16: stockham_pass<CT, 16, S, NT>(s_off, lgB, ax.L, Ns, tw_off); break;
-            case 8: stockham_pass<CT, 8, S, NT>(s_off, lgB, ax.L, Ns, tw_off); break;
-            case 4: stockham_pass<CT, 4, S, NT>(s_off, lgB, ax.L, Ns, tw_off); break;
-            case 2: stockham_pass<CT, 2, S, NT>(s_off, lgB, ax.L, Ns, tw_off); break;
-            case 3: stockham_pass<CT, 3, S, NT>(s_off, lgB, ax.L, Ns, tw_off); break;
-            default: stockham_pass<CT, 5, S, NT>(s_off, lgB, ax.L, Ns, tw_off); break;
+            case 16: stockham_pass<CT, 16, S, KIND>(s_off, lgB, ax.L, Ns, tw_off); break;
+            case 8: stockham_pass<CT, 8, S, KIND>(s_off, lgB, ax.L, Ns, tw_off); break;
+            case 4: stockham_pass<CT, 4, S, KIND>(s_off, lgB, ax.L, Ns, tw_off); break;
+            case 2: stockham_pass<CT, 2, S, KIND>(s_off, lgB, ax.L, Ns, tw_off); break;
+            case 3: stockham_pass<CT, 3, S, KIND>(s_off, lgB, ax.L, Ns, tw_off); break;
+            default: stockham_pass<CT, 5, S, KIND>(s_off, lgB, ax.L, Ns, tw_off); break;
         }
         Ns *= R;
     }
@@ -366,12 +397,12 @@ __device__ __forceinline__ void fft_static(int s_off, int tw_off) {
 
 // Transform of every column of a tile: compile-time length LS, or LS = 0 for the
 // run-time plan ax with tile width 2^lgB.
-template <typename CT, int S, int NT, int LS, int KIND>
+template <typename CT, int S, int KIND, int LS>
 __device__ __forceinline__ void fft_tile(int s_off, int lgB, const AxisPlan &ax, int tw_off) {
     if constexpr (LS > 0)
-        fft_static<CT, S, NT, LS, tile_lgb<CT>(LS, KIND)>(s_off, tw_off);
+        fft_static<CT, S, kind_nt(KIND), LS, tile_lgb<CT>(LS, KIND)>(s_off, tw_off);
     else
-        lds_fft<CT, S, NT>(s_off, lgB, ax, tw_off);
+        lds_fft<CT, S, KIND>(s_off, lgB, ax, tw_off);
 }
 
 // Blocks are dealt to the 8 XCDs round-robin; remap so that consecutive tiles
@@ -390,9 +421,9 @@ __device__ __forceinline__ void load_twiddles(C2<CT> *twl, const C2<CT> *__restr
 // Tile element e = threadIdx.x + NT·i (i < EMAX) ↔ (row e >> lgB, column e & (B-1)):
 // gathers all of a thread's global loads before its LDS stores, so that they are in
 // flight together. get(row, col, ok) returns the value (ok: e inside the tile).
-template <typename CT, int NT, typename F>
+template <typename CT, int KIND, typename F>
 __device__ __forceinline__ void tile_load(C2<CT> *Sm, int n, int lgB, F get) {
-    constexpr int EMAX = (NT == NTZ ? spec_tile_z<CT>() : spec_tile<CT>()) / NT;
+    constexpr int NT = kind_nt(KIND), EMAX = kind_tile<CT>(KIND) / NT;
     C2<CT> t[EMAX];
 #pragma unroll
     for (int i = 0; i < EMAX; i++) {
@@ -406,14 +437,62 @@ __device__ __forceinline__ void tile_load(C2<CT> *Sm, int n, int lgB, F get) {
     }
 }
 
-template <typename CT, int NT, typename F>
+template <typename CT, int KIND, typename F>
 __device__ __forceinline__ void tile_store(const C2<CT> *Sm, int n, int lgB, F put) {
-    constexpr int EMAX = (NT == NTZ ? spec_tile_z<CT>() : spec_tile<CT>()) / NT;
+    constexpr int NT = kind_nt(KIND), EMAX = kind_tile<CT>(KIND) / NT;
 #pragma unroll
     for (int i = 0; i < EMAX; i++) {
         const int e = threadIdx.x + NT * i;
         if (e < n) put(e >> lgB, e & ((1 << lgB) - 1), Sm[e]);
     }
+}
+
+// Column-pair transfers for tiles whose columns are contiguous in memory (tile width
+// B ≥ 2, even mode counts): lane element e ↔ (row e >> (lgB-1), columns 2·(e mod B/2)
+// and the next one), 16-byte accesses in FP32. get2(row, b) / put2(row, b, v) move
+// the pair (b, b+1).
+template <typename CT, int KIND, typename F>
+__device__ __forceinline__ void tile_load2(C2<CT> *Sm, int n, int lgB, F get2) {
+    constexpr int NT = kind_nt(KIND), EMAX = kind_tile<CT>(KIND) / NT / 2;
+    const int n2 = n >> 1, lg2 = lgB - 1;
+    C2x2<CT> t[EMAX];
+#pragma unroll
+    for (int i = 0; i < EMAX; i++) {
+        const int e = threadIdx.x + NT * i;
+        if (e < n2) t[i] = get2(e >> lg2, (e & ((1 << lg2) - 1)) << 1);
+    }
+#pragma unroll
+    for (int i = 0; i < EMAX; i++) {
+        const int e = threadIdx.x + NT * i;
+        if (e < n2) store_pair(Sm + 2 * e, t[i]);
+    }
+}
+
+template <typename CT, int KIND, typename F>
+__device__ __forceinline__ void tile_store2(const C2<CT> *Sm, int n, int lgB, F put2) {
+    constexpr int NT = kind_nt(KIND), EMAX = kind_tile<CT>(KIND) / NT / 2;
+    const int n2 = n >> 1, lg2 = lgB - 1;
+#pragma unroll
+    for (int i = 0; i < EMAX; i++) {
+        const int e = threadIdx.x + NT * i;
+        if (e < n2) put2(e >> lg2, (e & ((1 << lg2) - 1)) << 1, load_pair(Sm + 2 * e));
+    }
+}
+
+// Pair of consecutive complex values at p of which `valid` (≤ 0, 1, ≥ 2) exist.
+template <typename CT>
+__device__ __forceinline__ C2x2<CT> ld2(const C2<CT> *p, int valid) {
+    if (valid >= 2) return load_pair(p);
+    C2x2<CT> v{{(CT)0, (CT)0}, {(CT)0, (CT)0}};
+    if (valid == 1) v.a = p[0];
+    return v;
+}
+template <typename CT>
+__device__ __forceinline__ void st2(C2<CT> *p, const C2x2<CT> &v, int valid) {
+    if (valid >= 2)
+        store_pair(p, v);
+    else if (valid == 1)
+        p[0] = v.a;
 }
 
 struct SpecGeom {
@@ -427,10 +506,10 @@ struct SpecGeom {
 // Forward x: tile = 2^lgB consecutive pair rows r (r = (c·G2 + z)·G1/2 + p).
 // The two real rows y = 2p, 2p + 1 of pair row r are rows 2r, 2r + 1 of x1.
 template <typename CT, int LS>
-__global__ __launch_bounds__(NTP) void spec_x_forward(SpecGeom g, const C2<CT> *__restrict__ rows,
+__global__ __launch_bounds__(NTX) void spec_x_forward(SpecGeom g, const C2<CT> *__restrict__ rows,
                                                       C2<CT> *__restrict__ x1, const C2<CT> *__restrict__ tw) {
     const int L = LS ? LS : g.G[0];
-    const int lgB = LS ? tile_lgb<CT>(LS, KIND_P) : g.lgB[0];
+    const int lgB = LS ? tile_lgb<CT>(LS, KIND_X) : g.lgB[0];
     const int B = 1 << lgB;
     C2<CT> *Sm = lds<CT>(0);
     const int tw_off = L << lgB;
@@ -441,29 +520,67 @@ __global__ __launch_bounds__(NTP) void spec_x_forward(SpecGeom g, const C2<CT> *
     if (tile >= ntiles) return;
     const int64_t r0 = tile << lgB;
     const C2<CT> *src = rows + r0 * L;
-    // S[x·B + b] = row b, column x
-    tile_load<CT, NTP>(Sm, L << lgB, lgB, [&](int x, int b) { return src[(int64_t)b * L + x]; });
-    load_twiddles<CT, NTP>(twl, tw, L);
+    // S[x·B + b] = row b, column x; two x per lane (rows have even length)
+    if constexpr (!use_pairs<CT>()) {
+        tile_load<CT, KIND_X>(Sm, L << lgB, lgB, [&](int x, int b) { return src[(int64_t)b * L + x]; });
+    } else {
+        constexpr int EMAX = kind_tile<CT>(KIND_X) / NTX / 2;
+        const int n2 = (L << lgB) >> 1;
+        C2x2<CT> t[EMAX];
+#pragma unroll
+        for (int i = 0; i < EMAX; i++) {
+            const int e = threadIdx.x + NTX * i;
+            if (e < n2) t[i] = load_pair(src + (int64_t)(e & (B - 1)) * L + (e >> lgB) * 2);
+        }
+#pragma unroll
+        for (int i = 0; i < EMAX; i++) {
+            const int e = threadIdx.x + NTX * i;
+            if (e < n2) {
+                const int b = e & (B - 1), x = (e >> lgB) * 2;
+                Sm[(x << lgB) + b] = t[i].a;
+                Sm[((x + 1) << lgB) + b] = t[i].b;
+            }
+        }
+    }
+    load_twiddles<CT, NTX>(twl, tw, L);
     __syncthreads();
-    fft_tile<CT, -1, NTP, LS, KIND_P>(0, lgB, g.ax[0], tw_off);
+    fft_tile<CT, -1, KIND_X, LS>(0, lgB, g.ax[0], tw_off);
     const int nkx = g.nkx;
     C2<CT> *dst = x1 + 2 * r0 * nkx;
-    for (int e = threadIdx.x; e < (nkx << lgB); e += NTP) {
-        const int b = e / nkx, k = e - b * nkx;
+    auto split = [&](int k, int b, C2<CT> &A, C2<CT> &Bv) {
         const C2<CT> zk = Sm[(k << lgB) + b];
         const C2<CT> zm = Sm[((k == 0 ? 0 : L - k) << lgB) + b];
-        dst[(2 * b) * nkx + k] = C2<CT>{(zk.re + zm.re) * (CT)0.5, (zk.im - zm.im) * (CT)0.5};
-        dst[(2 * b + 1) * nkx + k] = C2<CT>{(zk.im + zm.im) * (CT)0.5, (zm.re - zk.re) * (CT)0.5};
+        A = C2<CT>{(zk.re + zm.re) * (CT)0.5, (zk.im - zm.im) * (CT)0.5};
+        Bv = C2<CT>{(zk.im + zm.im) * (CT)0.5, (zm.re - zk.re) * (CT)0.5};
+    };
+    if (use_pairs<CT>() && !(nkx & 1)) {
+        const int h = nkx >> 1;  // pairs (k, k+1) per row
+        for (int e = threadIdx.x; e < (h << lgB); e += NTX) {
+            const int b = e / h, k = (e - b * h) * 2;
+            C2x2<CT> A, Bv;
+            split(k, b, A.a, Bv.a);
+            split(k + 1, b, A.b, Bv.b);
+            store_pair(dst + (2 * b) * nkx + k, A);
+            store_pair(dst + (2 * b + 1) * nkx + k, Bv);
+        }
+    } else {
+        for (int e = threadIdx.x; e < (nkx << lgB); e += NTX) {
+            const int b = e / nkx, k = e - b * nkx;
+            C2<CT> A, Bv;
+            split(k, b, A, Bv);
+            dst[(2 * b) * nkx + k] = A;
+            dst[(2 * b + 1) * nkx + k] = Bv;
+        }
     }
 }
 
 // Forward y: tile = (c, z, kx-block); x1[c][z][y][kx] → x2[c][jy][z][kx] for the
 // retained ky = jy (jy ≤ K1) or jy - nky.
 template <typename CT, int LS>
-__global__ __launch_bounds__(NTP) void spec_y_forward(SpecGeom g, const C2<CT> *__restrict__ x1,
+__global__ __launch_bounds__(NTY) void spec_y_forward(SpecGeom g, const C2<CT> *__restrict__ x1,
                                                       C2<CT> *__restrict__ x2, const C2<CT> *__restrict__ tw) {
     const int L = LS ? LS : g.G[1];
-    const int lgB = LS ? tile_lgb<CT>(LS, KIND_P) : g.lgB[1];
+    const int lgB = LS ? tile_lgb<CT>(LS, KIND_Y) : g.lgB[1];
     const int B = 1 << lgB;
     C2<CT> *Sm = lds<CT>(0);
     const int tw_off = L << lgB;
@@ -477,23 +594,42 @@ __global__ __launch_bounds__(NTP) void spec_y_forward(SpecGeom g, const C2<CT> *
     const int c = (int)(cz / g.G[2]), z = (int)(cz - (int64_t)c * g.G[2]);
     const int kx0 = kb << lgB;
     const C2<CT> *src = x1 + cz * L * nkx + kx0;
-    tile_load<CT, NTP>(Sm, L << lgB, lgB, [&](int y, int b) {
-        return kx0 + b < nkx ? src[(int64_t)y * nkx + b] : C2<CT>{(CT)0, (CT)0};
-    });
-    load_twiddles<CT, NTP>(twl, tw, L);
+    const bool pairs = use_pairs<CT>() && lgB >= 1 && !(nkx & 1);
+    if (pairs)
+        tile_load2<CT, KIND_Y>(Sm, L << lgB, lgB,
+                               [&](int y, int b) { return ld2(src + (int64_t)y * nkx + b, nkx - kx0 - b); });
+    else
+        tile_load<CT, KIND_Y>(Sm, L << lgB, lgB, [&](int y, int b) {
+            return kx0 + b < nkx ? src[(int64_t)y * nkx + b] : C2<CT>{(CT)0, (CT)0};
+        });
+    load_twiddles<CT, NTY>(twl, tw, L);
     __syncthreads();
-    fft_tile<CT, -1, NTP, LS, KIND_P>(0, lgB, g.ax[1], tw_off);
+    fft_tile<CT, -1, KIND_Y, LS>(0, lgB, g.ax[1], tw_off);
     const int nky = g.nky, K1 = g.K1;
     C2<CT> *dst = x2 + ((int64_t)c * nky * g.G[2] + z) * nkx + kx0;
     const int64_t jstride = (int64_t)g.G[2] * nkx;
-    constexpr int EMAX = spec_tile<CT>() / NTP;
+    if (pairs) {
+        constexpr int EMAX = kind_tile<CT>(KIND_Y) / NTY / 2;
+        const int lg2 = lgB - 1;
 #pragma unroll
-    for (int i = 0; i < EMAX; i++) {
-        const int e = threadIdx.x + NTP * i;
-        const int b = e & (B - 1), jy = e >> lgB;
-        if (jy < nky && kx0 + b < nkx) {
-            const int iy = jy <= K1 ? jy : L - nky + jy;
-            dst[jy * jstride + b] = Sm[(iy << lgB) + b];
+        for (int i = 0; i < EMAX; i++) {
+            const int e = threadIdx.x + NTY * i;
+            const int b = (e & ((1 << lg2) - 1)) << 1, jy = e >> lg2;
+            if (jy < nky) {
+                const int iy = jy <= K1 ? jy : L - nky + jy;
+                st2(dst + jy * jstride + b, load_pair(Sm + (iy << lgB) + b), nkx - kx0 - b);
+            }
+        }
+    } else {
+        constexpr int EMAX = kind_tile<CT>(KIND_Y) / NTY;
+#pragma unroll
+        for (int i = 0; i < EMAX; i++) {
+            const int e = threadIdx.x + NTY * i;
+            const int b = e & (B - 1), jy = e >> lgB;
+            if (jy < nky && kx0 + b < nkx) {
+                const int iy = jy <= K1 ? jy : L - nky + jy;
+                dst[jy * jstride + b] = Sm[(iy << lgB) + b];
+            }
         }
     }
 }
@@ -517,7 +653,7 @@ struct SpecFourier {
 template <typename CT, int LS>
 __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f, C2<CT> *__restrict__ x2,
                                                       C2<CT> *__restrict__ x2b, const C2<CT> *__restrict__ tw) {
-    constexpr int EMAX = spec_tile_z<CT>() / NTZ;
+    constexpr int EMAX = kind_tile<CT>(KIND_Z) / NTZ;
     const int L = LS ? LS : g.G[2];
     const int lgB = LS ? tile_lgb<CT>(LS, KIND_Z) : g.lgB[2];
     const int B = 1 << lgB;
@@ -534,15 +670,37 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
     const int64_t cs = (int64_t)nky * L * nkx;  // complex values per component of x2
     const int64_t off = (int64_t)jy * L * nkx + kx0;
     load_twiddles<CT, NTZ>(twl, tw, L);
+    const bool pairs = use_pairs<CT>() && lgB >= 1 && !(nkx & 1);
+    auto load3 = [&]() {
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const C2<CT> *src = x2 + c * cs + off;
-        tile_load<CT, NTZ>(T[c], n, lgB, [&](int z, int b) {
-            return kx0 + b < nkx ? src[(int64_t)z * nkx + b] : C2<CT>{(CT)0, (CT)0};
-        });
-    }
+        for (int c = 0; c < 3; c++) {
+            const C2<CT> *src = x2 + c * cs + off;
+            if (pairs)
+                tile_load2<CT, KIND_Z>(T[c], n, lgB,
+                                       [&](int z, int b) { return ld2(src + (int64_t)z * nkx + b, nkx - kx0 - b); });
+            else
+                tile_load<CT, KIND_Z>(T[c], n, lgB, [&](int z, int b) {
+                    return kx0 + b < nkx ? src[(int64_t)z * nkx + b] : C2<CT>{(CT)0, (CT)0};
+                });
+        }
+    };
+    auto store3 = [&](C2<CT> *out) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            C2<CT> *dst = out + c * cs + off;
+            if (pairs)
+                tile_store2<CT, KIND_Z>(T[c], n, lgB, [&](int z, int b, const C2x2<CT> &v) {
+                    st2(dst + (int64_t)z * nkx + b, v, nkx - kx0 - b);
+                });
+            else
+                tile_store<CT, KIND_Z>(T[c], n, lgB, [&](int z, int b, C2<CT> v) {
+                    if (kx0 + b < nkx) dst[(int64_t)z * nkx + b] = v;
+                });
+        }
+    };
+    load3();
     __syncthreads();
-    for (int c = 0; c < 3; c++) fft_tile<CT, -1, NTZ, LS, KIND_Z>(c * n, lgB, g.ax[2], tw_off);
+    for (int c = 0; c < 3; c++) fft_tile<CT, -1, KIND_Z, LS>(c * n, lgB, g.ax[2], tw_off);
     // Fourier-space operators on the retained modes
     const int K1 = g.K1, K2 = g.K2, nkz = g.nkz;
     const int ky = jy <= K1 ? jy : jy - nky;
@@ -625,35 +783,16 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
         }
     }
     __syncthreads();
-    C2<CT> *out = f.do_psi ? x2b : x2;
-    for (int c = 0; c < 3; c++) fft_tile<CT, 1, NTZ, LS, KIND_Z>(c * n, lgB, g.ax[2], tw_off);
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        C2<CT> *dst = out + c * cs + off;
-        tile_store<CT, NTZ>(T[c], n, lgB, [&](int z, int b, C2<CT> v) {
-            if (kx0 + b < nkx) dst[(int64_t)z * nkx + b] = v;
-        });
-    }
+    for (int c = 0; c < 3; c++) fft_tile<CT, 1, KIND_Z, LS>(c * n, lgB, g.ax[2], tw_off);
+    store3(f.do_psi ? x2b : x2);
     if (both) {
         // this block's writes to x2 above are visible to its own loads after the barrier
         __threadfence_block();
         __syncthreads();
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const C2<CT> *src = x2 + c * cs + off;
-            tile_load<CT, NTZ>(T[c], n, lgB, [&](int z, int b) {
-                return kx0 + b < nkx ? src[(int64_t)z * nkx + b] : C2<CT>{(CT)0, (CT)0};
-            });
-        }
+        load3();
         __syncthreads();
-        for (int c = 0; c < 3; c++) fft_tile<CT, 1, NTZ, LS, KIND_Z>(c * n, lgB, g.ax[2], tw_off);
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            C2<CT> *dst = x2 + c * cs + off;
-            tile_store<CT, NTZ>(T[c], n, lgB, [&](int z, int b, C2<CT> v) {
-                if (kx0 + b < nkx) dst[(int64_t)z * nkx + b] = v;
-            });
-        }
+        for (int c = 0; c < 3; c++) fft_tile<CT, 1, KIND_Z, LS>(c * n, lgB, g.ax[2], tw_off);
+        store3(x2);
     }
 }
 
@@ -683,7 +822,7 @@ __global__ __launch_bounds__(NTZ) void spec_z_band(SpecGeom g, SpecFourier f, do
     load_twiddles<CT, NTZ>(twl, tw, L);
     const C2<CT> *uh = (const C2<CT> *)f.uhat;
     for (int c = 0; c < 3; c++) {
-        tile_load<CT, NTZ>(Sm, n, lgB, [&](int iz, int b) {
+        tile_load<CT, KIND_Z>(Sm, n, lgB, [&](int iz, int b) {
             const int kx = kx0 + b;
             const int kz = iz <= (L >> 1) ? iz : iz - L;
             C2<CT> v{(CT)0, (CT)0};
@@ -700,9 +839,9 @@ __global__ __launch_bounds__(NTZ) void spec_z_band(SpecGeom g, SpecFourier f, do
             return v;
         });
         __syncthreads();
-        fft_tile<CT, 1, NTZ, LS, KIND_Z>(0, lgB, g.ax[2], tw_off);
+        fft_tile<CT, 1, KIND_Z, LS>(0, lgB, g.ax[2], tw_off);
         C2<CT> *dst = x2 + c * cs + off;
-        tile_store<CT, NTZ>(Sm, n, lgB, [&](int z, int b, C2<CT> v) {
+        tile_store<CT, KIND_Z>(Sm, n, lgB, [&](int z, int b, C2<CT> v) {
             if (kx0 + b < nkx) dst[(int64_t)z * nkx + b] = v;
         });
         __syncthreads();
@@ -712,10 +851,10 @@ __global__ __launch_bounds__(NTZ) void spec_z_band(SpecGeom g, SpecFourier f, do
 // Inverse y: tile = (c, z, kx-block); x2[c][jy][z][kx] on the retained ky (zero
 // elsewhere) → inverse FFT_y → x1[c][z][y][kx].
 template <typename CT, int LS>
-__global__ __launch_bounds__(NTP) void spec_y_inverse(SpecGeom g, const C2<CT> *__restrict__ x2,
+__global__ __launch_bounds__(NTY) void spec_y_inverse(SpecGeom g, const C2<CT> *__restrict__ x2,
                                                       C2<CT> *__restrict__ x1, const C2<CT> *__restrict__ tw) {
     const int L = LS ? LS : g.G[1];
-    const int lgB = LS ? tile_lgb<CT>(LS, KIND_P) : g.lgB[1];
+    const int lgB = LS ? tile_lgb<CT>(LS, KIND_Y) : g.lgB[1];
     const int B = 1 << lgB;
     C2<CT> *Sm = lds<CT>(0);
     const int tw_off = L << lgB;
@@ -730,29 +869,42 @@ __global__ __launch_bounds__(NTP) void spec_y_inverse(SpecGeom g, const C2<CT> *
     const int kx0 = kb << lgB;
     const C2<CT> *src = x2 + ((int64_t)c * nky * g.G[2] + z) * nkx + kx0;
     const int64_t jstride = (int64_t)g.G[2] * nkx;
-    tile_load<CT, NTP>(Sm, L << lgB, lgB, [&](int iy, int b) {
-        const int ky = iy <= (L >> 1) ? iy : iy - L;
-        C2<CT> v{(CT)0, (CT)0};
-        if (ky >= -K1 && ky <= K1 && kx0 + b < nkx) v = src[(ky >= 0 ? ky : ky + nky) * jstride + b];
-        return v;
-    });
-    load_twiddles<CT, NTP>(twl, tw, L);
+    const bool pairs = use_pairs<CT>() && lgB >= 1 && !(nkx & 1);
+    if (pairs)
+        tile_load2<CT, KIND_Y>(Sm, L << lgB, lgB, [&](int iy, int b) {
+            const int ky = iy <= (L >> 1) ? iy : iy - L;
+            const bool kept = ky >= -K1 && ky <= K1;
+            return ld2(src + (kept ? (ky >= 0 ? ky : ky + nky) * jstride + b : 0), kept ? nkx - kx0 - b : 0);
+        });
+    else
+        tile_load<CT, KIND_Y>(Sm, L << lgB, lgB, [&](int iy, int b) {
+            const int ky = iy <= (L >> 1) ? iy : iy - L;
+            C2<CT> v{(CT)0, (CT)0};
+            if (ky >= -K1 && ky <= K1 && kx0 + b < nkx) v = src[(ky >= 0 ? ky : ky + nky) * jstride + b];
+            return v;
+        });
+    load_twiddles<CT, NTY>(twl, tw, L);
     __syncthreads();
-    fft_tile<CT, 1, NTP, LS, KIND_P>(0, lgB, g.ax[1], tw_off);
+    fft_tile<CT, 1, KIND_Y, LS>(0, lgB, g.ax[1], tw_off);
     C2<CT> *dst = x1 + cz * L * nkx + kx0;
-    tile_store<CT, NTP>(Sm, L << lgB, lgB, [&](int y, int b, C2<CT> v) {
-        if (kx0 + b < nkx) dst[(int64_t)y * nkx + b] = v;
-    });
+    if (pairs)
+        tile_store2<CT, KIND_Y>(Sm, L << lgB, lgB, [&](int y, int b, const C2x2<CT> &v) {
+            st2(dst + (int64_t)y * nkx + b, v, nkx - kx0 - b);
+        });
+    else
+        tile_store<CT, KIND_Y>(Sm, L << lgB, lgB, [&](int y, int b, C2<CT> v) {
+            if (kx0 + b < nkx) dst[(int64_t)y * nkx + b] = v;
+        });
 }
 
 // Inverse x: tile = 2^lgB pair rows; Z_k = A_k + i B_k for k < nkx (A, B from rows
 // 2r, 2r + 1 of x1), their conjugates at G0 - k (Hermitian symmetry of real rows),
 // zero otherwise, DC keeps only its real parts; inverse FFT_x → pair rows.
 template <typename CT, int LS>
-__global__ __launch_bounds__(NTP) void spec_x_inverse(SpecGeom g, const C2<CT> *__restrict__ x1,
+__global__ __launch_bounds__(NTX) void spec_x_inverse(SpecGeom g, const C2<CT> *__restrict__ x1,
                                                       C2<CT> *__restrict__ rows, const C2<CT> *__restrict__ tw) {
     const int L = LS ? LS : g.G[0];
-    const int lgB = LS ? tile_lgb<CT>(LS, KIND_P) : g.lgB[0];
+    const int lgB = LS ? tile_lgb<CT>(LS, KIND_X) : g.lgB[0];
     const int B = 1 << lgB;
     C2<CT> *Sm = lds<CT>(0);
     const int tw_off = L << lgB;
@@ -766,37 +918,78 @@ __global__ __launch_bounds__(NTP) void spec_x_inverse(SpecGeom g, const C2<CT> *
     const C2<CT> *src = x1 + 2 * r0 * nkx;
     // each (A_k, B_k) is read once and gives both Z_k and Z_{G0-k}
     {
-        constexpr int EMAX = spec_tile<CT>() / NTP;
-        const int nin = nkx << lgB;
-        C2<CT> a[EMAX], bb[EMAX];
+        auto put = [&](int k, int b, C2<CT> A, C2<CT> Bv) {
+            if (k == 0) A.im = Bv.im = (CT)0;
+            Sm[(k << lgB) + b] = C2<CT>{A.re - Bv.im, A.im + Bv.re};
+            if (k > 0) Sm[((L - k) << lgB) + b] = C2<CT>{A.re + Bv.im, Bv.re - A.im};  // conj(A) + i conj(B)
+        };
+        if (use_pairs<CT>() && !(nkx & 1)) {
+            // pairs (k, k+1) per row: 16-byte loads in FP32
+            constexpr int EMAX = kind_tile<CT>(KIND_X) / NTX / 2;
+            const int nin = (nkx >> 1) << lgB;
+            C2x2<CT> a[EMAX], bb[EMAX];
 #pragma unroll
-        for (int i = 0; i < EMAX; i++) {
-            const int e = threadIdx.x + NTP * i;
-            if (e < nin) {
-                const int k = e >> lgB, b = e & (B - 1);
-                a[i] = src[(2 * b) * nkx + k];
-                bb[i] = src[(2 * b + 1) * nkx + k];
+            for (int i = 0; i < EMAX; i++) {
+                const int e = threadIdx.x + NTX * i;
+                if (e < nin) {
+                    const int k = (e >> lgB) * 2, b = e & (B - 1);
+                    a[i] = load_pair(src + (2 * b) * nkx + k);
+                    bb[i] = load_pair(src + (2 * b + 1) * nkx + k);
+                }
             }
-        }
 #pragma unroll
-        for (int i = 0; i < EMAX; i++) {
-            const int e = threadIdx.x + NTP * i;
-            if (e < nin) {
-                const int k = e >> lgB, b = e & (B - 1);
-                C2<CT> A = a[i], Bv = bb[i];
-                if (k == 0) A.im = Bv.im = (CT)0;
-                Sm[e] = C2<CT>{A.re - Bv.im, A.im + Bv.re};
-                if (k > 0) Sm[((L - k) << lgB) + b] = C2<CT>{A.re + Bv.im, Bv.re - A.im};  // conj(A) + i conj(B)
+            for (int i = 0; i < EMAX; i++) {
+                const int e = threadIdx.x + NTX * i;
+                if (e < nin) {
+                    const int k = (e >> lgB) * 2, b = e & (B - 1);
+                    put(k, b, a[i].a, bb[i].a);
+                    put(k + 1, b, a[i].b, bb[i].b);
+                }
+            }
+        } else {
+            constexpr int EMAX = kind_tile<CT>(KIND_X) / NTX;
+            const int nin = nkx << lgB;
+            C2<CT> a[EMAX], bb[EMAX];
+#pragma unroll
+            for (int i = 0; i < EMAX; i++) {
+                const int e = threadIdx.x + NTX * i;
+                if (e < nin) {
+                    const int k = e >> lgB, b = e & (B - 1);
+                    a[i] = src[(2 * b) * nkx + k];
+                    bb[i] = src[(2 * b + 1) * nkx + k];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < EMAX; i++) {
+                const int e = threadIdx.x + NTX * i;
+                if (e < nin) put(e >> lgB, e & (B - 1), a[i], bb[i]);
             }
         }
         // modes kx ∈ [nkx, G0 - nkx] are not retained
-        for (int e = (nkx << lgB) + threadIdx.x; e < ((L - nkx + 1) << lgB); e += NTP) Sm[e] = C2<CT>{(CT)0, (CT)0};
+        for (int e = (nkx << lgB) + threadIdx.x; e < ((L - nkx + 1) << lgB); e += NTX) Sm[e] = C2<CT>{(CT)0, (CT)0};
     }
-    load_twiddles<CT, NTP>(twl, tw, L);
+    load_twiddles<CT, NTX>(twl, tw, L);
     __syncthreads();
-    fft_tile<CT, 1, NTP, LS, KIND_P>(0, lgB, g.ax[0], tw_off);
+    fft_tile<CT, 1, KIND_X, LS>(0, lgB, g.ax[0], tw_off);
+    // rows out two values per lane (16-byte stores in FP32), lanes along b then x
     C2<CT> *dst = rows + r0 * L;
-    tile_store<CT, NTP>(Sm, L << lgB, lgB, [&](int x, int b, C2<CT> v) { dst[(int64_t)b * L + x] = v; });
+    if constexpr (!use_pairs<CT>()) {
+        tile_store<CT, KIND_X>(Sm, L << lgB, lgB, [&](int x, int b, C2<CT> v) { dst[(int64_t)b * L + x] = v; });
+    } else {
+        constexpr int NT = NTX, EMAX = kind_tile<CT>(KIND_X) / NT / 2;
+        const int n2 = (L << lgB) >> 1;
+#pragma unroll
+        for (int i = 0; i < EMAX; i++) {
+            const int e = threadIdx.x + NT * i;
+            if (e < n2) {
+                const int b = e & (B - 1), x = (e >> lgB) * 2;
+                C2x2<CT> v;
+                v.a = Sm[(x << lgB) + b];
+                v.b = Sm[((x + 1) << lgB) + b];
+                store_pair(dst + (int64_t)b * L + x, v);
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -808,7 +1001,7 @@ SpecGeom spec_geom(const vpbs_ctx *c) {
     for (int d = 0; d < 3; d++) {
         g.G[d] = c->G[d];
         g.ax[d] = plan_axis(c->G[d]);
-        g.lgB[d] = tile_lgb<CT>(c->G[d], d == 2 ? KIND_Z : KIND_P);
+        g.lgB[d] = tile_lgb<CT>(c->G[d], d);  // kind = axis
     }
     g.nkx = c->nkx;
     g.nky = c->nky;
@@ -874,9 +1067,9 @@ int forward_impl(vpbs_ctx *c) {
     const SpecGeom g = spec_geom<CT>(c);
     const int64_t nrows = 3 * (int64_t)c->G[2] * (c->G[1] / 2);
     const int64_t nkb = (c->nkx + (1 << g.lgB[1]) - 1) >> g.lgB[1];
-    SPEC_LAUNCH(c->G[0], spec_x_forward, grid8(nrows >> g.lgB[0]), NTP, spec_lds<CT>(g, 0),
+    SPEC_LAUNCH(c->G[0], spec_x_forward, grid8(nrows >> g.lgB[0]), NTX, spec_lds<CT>(g, 0),
                 g, (const C2<CT> *)c->grid, (C2<CT> *)c->ypen, twiddles<CT>(c, 0));
-    SPEC_LAUNCH(c->G[1], spec_y_forward, grid8(3 * (int64_t)c->G[2] * nkb), NTP, spec_lds<CT>(g, 1),
+    SPEC_LAUNCH(c->G[1], spec_y_forward, grid8(3 * (int64_t)c->G[2] * nkb), NTY, spec_lds<CT>(g, 1),
                 g, (const C2<CT> *)c->ypen, (C2<CT> *)c->zpen, twiddles<CT>(c, 1));
     VPBS_HIP(c, hipGetLastError());
     return VPBS_OK;
@@ -904,9 +1097,9 @@ int inverse_impl(vpbs_ctx *c, void *zp) {
     const SpecGeom g = spec_geom<CT>(c);
     const int64_t nrows = 3 * (int64_t)c->G[2] * (c->G[1] / 2);
     const int64_t nkb = (c->nkx + (1 << g.lgB[1]) - 1) >> g.lgB[1];
-    SPEC_LAUNCH(c->G[1], spec_y_inverse, grid8(3 * (int64_t)c->G[2] * nkb), NTP, spec_lds<CT>(g, 1),
+    SPEC_LAUNCH(c->G[1], spec_y_inverse, grid8(3 * (int64_t)c->G[2] * nkb), NTY, spec_lds<CT>(g, 1),
                 g, (const C2<CT> *)zp, (C2<CT> *)c->ypen, twiddles<CT>(c, 1));
-    SPEC_LAUNCH(c->G[0], spec_x_inverse, grid8(nrows >> g.lgB[0]), NTP, spec_lds<CT>(g, 0),
+    SPEC_LAUNCH(c->G[0], spec_x_inverse, grid8(nrows >> g.lgB[0]), NTX, spec_lds<CT>(g, 0),
                 g, (const C2<CT> *)c->ypen, (C2<CT> *)c->grid, twiddles<CT>(c, 0));
     VPBS_HIP(c, hipGetLastError());
     return VPBS_OK;
@@ -974,11 +1167,9 @@ int setup_impl(vpbs_ctx *c) {
 
 bool spectral_supported(const vpbs_ctx *c) {
     const bool f32 = c->p.precision == VPBS_F32;
-    const int tile = f32 ? spec_tile<float>() : spec_tile<double>();
-    const int tile_z = f32 ? spec_tile_z<float>() : spec_tile_z<double>();
     for (int d = 0; d < 3; d++) {
         const AxisPlan a = plan_axis(c->G[d]);
-        if (a.npass < 0 || c->G[d] > (d == 2 ? tile_z : tile)) return false;
+        if (a.npass < 0 || c->G[d] > (f32 ? kind_tile<float>(d) : kind_tile<double>(d))) return false;
     }
     return true;
 }
