@@ -3,7 +3,8 @@
 //   local_term_kernel   LIA term, FP64 (shortrange/local_integrals.jl:106-165)
 //   cell keys + sort    device cell list (CellLists.jl:131-137,194-204), built by a
 //                       deterministic counting sort instead of an atomic linked list
-//   pair_kernel         erfc/Gaussian pair sum over the (2M+1)³ neighbour cells
+//   pair_wave_kernel    erfc/Gaussian pair sum over the (2M+1)³ neighbour cells,
+//                       one wavefront per target with compaction of the accepted pairs
 //                       (shortrange.jl:216-273, integrands.jl:2-18, gaussian.jl:273-279)
 //   self_kernel         removal of the local-segment contribution (local_integrals.jl:14-95)
 //
@@ -171,102 +172,216 @@ __device__ inline float exp_t(float x) { return expf(x); }
 __device__ inline double sqrt_t(double x) { return sqrt(x); }
 __device__ inline float sqrt_t(float x) { return sqrtf(x); }
 
+// One wavefront per target node (cell-sorted order; XCD-aware block order):
+//  1) lanes build the source runs of the (2M+1)² neighbour cell rows (≤ 2 contiguous
+//     runs per row with the periodic wrap along x) and a wave scan gives each row's
+//     candidate offset;
+//  2) the candidates are tested 64 at a time, consecutive lanes reading consecutive
+//     sources: FP64 separation, minimum-image fold and r² ≤ rcut² exactly as in the
+//     reference (shortrange.jl:255-256), local segments excluded when required; the
+//     accepted pairs are appended in candidate order to an LDS queue (ballot compaction);
+//  3) each time 64 are queued (and at the end) every lane evaluates one queued pair
+//     (erfc/Gaussian weights, cross product; integrands.jl:2-18, gaussian.jl:273-279)
+//     into its own compensated accumulator, so the expensive part runs on full waves;
+//  4) the 64 partial sums are combined by a fixed butterfly in FP64.
+// The accepted pair set is the thread-per-target one; the summation order is fixed
+// (deterministic) but differs from a sequential sum.
+constexpr int PAIR_MAXROWS = 81;  // (2M+1)² for M ≤ 4
+
 template <typename CT, bool VEL, bool PSI, bool STATS>
-__global__ __launch_bounds__(256) void pair_kernel(PairArgs<CT> a) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.Np) return;
+__global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
+    __shared__ int r_pre[PAIR_MAXROWS + 1];  // candidates before row r
+    __shared__ int r_a0[PAIR_MAXROWS], r_n0[PAIR_MAXROWS], r_a1[PAIR_MAXROWS];
+    __shared__ int q_s[128];
+    __shared__ CT q_r[4][128];  // rx, ry, rz, r² of the queued pairs
+
+    const int lane = threadIdx.x;
     const int64_t Np = a.Np, Nq = a.Nq;
+    const int64_t per = ((int64_t)gridDim.x + 7) / 8;
+    const int64_t k = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (k >= Np) return;
     const int32_t i = a.tperm[k];
     const double x = a.nodes[i], y = a.nodes[Np + i], z = a.nodes[2 * Np + i];
     const int cx = min(max(cell_index(x, a.box.L[0], a.Nc.x), 0), a.Nc.x - 1);
     const int cy = min(max(cell_index(y, a.box.L[1], a.Nc.y), 0), a.Nc.y - 1);
     const int cz = min(max(cell_index(z, a.box.L[2], a.Nc.z), 0), a.Nc.z - 1);
     const int64_t jr0 = (int64_t)a.nquad * i, jl0 = (int64_t)a.nquad * a.idx_prev[i];
-    const int M = a.M;
-    Acc3<CT> av, ap;
-    int64_t cnt = 0;
-    uint64_t h = 0;
-    const CT two_over_sqrt_pi = (CT)TWO_OVER_SQRT_PI;
-    for (int dz = -M; dz <= M; dz++) {
-        int zz = cz + dz;
-        zz += (zz < 0) ? a.Nc.z : 0;
-        zz -= (zz >= a.Nc.z) ? a.Nc.z : 0;
-        for (int dy = -M; dy <= M; dy++) {
-            int yy = cy + dy;
+    const int M = a.M, W2 = 2 * M + 1, nrows = W2 * W2;
+
+    // 1) rows (dz, dy) = (r / W2 - M, r % W2 - M); lanes take rows lane and lane + 64
+    int len[2] = {0, 0};
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int r = lane + 64 * h;
+        if (r < nrows) {
+            int zz = cz + r / W2 - M, yy = cy + r % W2 - M;
+            zz += (zz < 0) ? a.Nc.z : 0;
+            zz -= (zz >= a.Nc.z) ? a.Nc.z : 0;
             yy += (yy < 0) ? a.Nc.y : 0;
             yy -= (yy >= a.Nc.y) ? a.Nc.y : 0;
             const int64_t row = ((int64_t)zz * a.Nc.y + yy) * a.Nc.x;
-            int lo = cx - M, hi = cx + M;
-            // up to two contiguous runs of cells along x (periodic wrap)
-            int r0a, r0b, r1a = 0, r1b = -1;
+            const int lo = cx - M, hi = cx + M;
+            int ca0, cb0, ca1 = 0, cb1 = -1;
             if (lo < 0) {
-                r0a = lo + a.Nc.x;
-                r0b = a.Nc.x - 1;
-                r1a = 0;
-                r1b = hi;
+                ca0 = lo + a.Nc.x, cb0 = a.Nc.x - 1, cb1 = hi;
             } else if (hi >= a.Nc.x) {
-                r0a = lo;
-                r0b = a.Nc.x - 1;
-                r1a = 0;
-                r1b = hi - a.Nc.x;
+                ca0 = lo, cb0 = a.Nc.x - 1, cb1 = hi - a.Nc.x;
             } else {
-                r0a = lo;
-                r0b = hi;
+                ca0 = lo, cb0 = hi;
             }
-            for (int run = 0; run < 2; run++) {
-                int ca = run ? r1a : r0a, cb = run ? r1b : r0b;
-                if (cb < ca) continue;
-                const int32_t s0 = a.cstart[row + ca], s1 = a.cstart[row + cb + 1];
-                for (int32_t s = s0; s < s1; s++) {
-                    const int32_t j = a.sidx[s];
-                    if (!a.include_local &&
-                        ((j >= jr0 && j < jr0 + a.nquad) || (j >= jl0 && j < jl0 + a.nquad)))
-                        continue;
-                    const double rx = fold_sep(x - a.spos[s], a.box.L[0], a.box.Lh[0]);
-                    const double ry = fold_sep(y - a.spos[Nq + s], a.box.L[1], a.box.Lh[1]);
-                    const double rz = fold_sep(z - a.spos[2 * Nq + s], a.box.L[2], a.box.Lh[2]);
-                    const double r2 = dist2_nocontract(rx, ry, rz);
-                    if (!(r2 <= a.rcut2)) continue;
-                    if (STATS) {
-                        cnt++;
-                        h += mix64((uint64_t)j);
-                    }
-                    const CT r = sqrt_t((CT)r2);
-                    const CT rinv = (CT)1 / r;
-                    const CT ar = a.alpha * r;
-                    const CT wa = erfc_t(ar);
-                    const CT qx = a.schg[s], qy = a.schg[Nq + s], qz = a.schg[2 * Nq + s];
-                    if (VEL) {
-                        const CT wb = two_over_sqrt_pi * ar * exp_t(-(ar * ar));
-                        const CT f = (wa + wb) * (rinv * rinv * rinv);
-                        const CT crx = (CT)rx, cry = (CT)ry, crz = (CT)rz;
-                        av.add(0, f * (qy * crz - qz * cry));
-                        av.add(1, f * (qz * crx - qx * crz));
-                        av.add(2, f * (qx * cry - qy * crx));
-                    }
-                    if (PSI) {
-                        const CT f = wa * rinv;
-                        ap.add(0, f * qx);
-                        ap.add(1, f * qy);
-                        ap.add(2, f * qz);
-                    }
+            const int s0 = a.cstart[row + ca0], n0 = a.cstart[row + cb0 + 1] - s0;
+            int s1 = 0, n1 = 0;
+            if (cb1 >= ca1) {
+                s1 = a.cstart[row + ca1];
+                n1 = a.cstart[row + cb1 + 1] - s1;
+            }
+            r_a0[r] = s0;
+            r_n0[r] = n0;
+            r_a1[r] = s1;
+            len[h] = n0 + n1;
+        }
+    }
+    int incl0 = len[0], incl1 = len[1];
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u0 = __shfl_up(incl0, off, 64), u1 = __shfl_up(incl1, off, 64);
+        if (lane >= off) incl0 += u0, incl1 += u1;
+    }
+    const int tot0 = __shfl(incl0, 63, 64);
+    if (lane < nrows) r_pre[lane + 1] = incl0;
+    if (lane + 64 < nrows) r_pre[lane + 65] = tot0 + incl1;
+    if (lane == 0) r_pre[0] = 0;
+    const int ncand = tot0 + __shfl(incl1, 63, 64);
+    __syncthreads();
+
+    Acc3<CT> av, ap;
+    int64_t cnt = 0;
+    uint64_t hsum = 0;
+    const CT two_over_sqrt_pi = (CT)TWO_OVER_SQRT_PI;
+    auto eval = [&](int e) {
+        const int s = q_s[e];
+        const CT crx = q_r[0][e], cry = q_r[1][e], crz = q_r[2][e];
+        const CT r = sqrt_t(q_r[3][e]);
+        const CT rinv = (CT)1 / r;
+        const CT ar = a.alpha * r;
+        const CT wa = erfc_t(ar);
+        const CT qx = a.schg[s], qy = a.schg[Nq + s], qz = a.schg[2 * Nq + s];
+        if (VEL) {
+            const CT wb = two_over_sqrt_pi * ar * exp_t(-(ar * ar));
+            const CT f = (wa + wb) * (rinv * rinv * rinv);
+            av.add(0, f * (qy * crz - qz * cry));
+            av.add(1, f * (qz * crx - qx * crz));
+            av.add(2, f * (qx * cry - qy * crx));
+        }
+        if (PSI) {
+            const CT f = wa * rinv;
+            ap.add(0, f * qx);
+            ap.add(1, f * qy);
+            ap.add(2, f * qz);
+        }
+    };
+
+    int nqd = 0;  // queued pairs (uniform)
+    for (int base = 0; base < ncand; base += 64) {
+        // 2) test 64 candidates
+        const int cnd = base + lane;
+        bool acc = false;
+        int s = 0;
+        double rx = 0, ry = 0, rz = 0, r2 = 0;
+        if (cnd < ncand) {
+            int lo = 0, hi = nrows - 1;  // last row with r_pre[row] <= cnd
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (r_pre[mid] <= cnd)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int o = cnd - r_pre[lo];
+            s = o < r_n0[lo] ? r_a0[lo] + o : r_a1[lo] + (o - r_n0[lo]);
+            const int32_t j = a.sidx[s];
+            const bool local = !a.include_local && ((j >= jr0 && j < jr0 + a.nquad) || (j >= jl0 && j < jl0 + a.nquad));
+            if (!local) {
+                rx = fold_sep(x - a.spos[s], a.box.L[0], a.box.Lh[0]);
+                ry = fold_sep(y - a.spos[Nq + s], a.box.L[1], a.box.Lh[1]);
+                rz = fold_sep(z - a.spos[2 * Nq + s], a.box.L[2], a.box.Lh[2]);
+                r2 = dist2_nocontract(rx, ry, rz);
+                acc = r2 <= a.rcut2;
+                if (STATS && acc) {
+                    cnt++;
+                    hsum += mix64((uint64_t)j);
                 }
             }
         }
+        const unsigned long long mask = __ballot(acc);
+        if (acc) {
+            const int pos = nqd + __popcll(mask & ((1ull << lane) - 1ull));
+            q_s[pos] = s;
+            q_r[0][pos] = (CT)rx;
+            q_r[1][pos] = (CT)ry;
+            q_r[2][pos] = (CT)rz;
+            q_r[3][pos] = (CT)r2;
+        }
+        nqd += __popcll(mask);
+        __syncthreads();
+        // 3) full waves of queued pairs
+        if (nqd >= 64) {
+            eval(lane);
+            const int rest = nqd - 64;  // ≤ 63: move entries [64, nqd) to the front
+            int ms = 0;
+            CT mr[4];
+            if (lane < rest) {
+                ms = q_s[64 + lane];
+#pragma unroll
+                for (int d = 0; d < 4; d++) mr[d] = q_r[d][64 + lane];
+            }
+            __syncthreads();
+            if (lane < rest) {
+                q_s[lane] = ms;
+#pragma unroll
+                for (int d = 0; d < 4; d++) q_r[d][lane] = mr[d];
+            }
+            nqd = rest;
+            __syncthreads();
+        }
     }
+    if (lane < nqd) eval(lane);
+
+    // 4) fixed-order reduction of the lane partial sums
     if (VEL) {
-        a.v[i] += a.pref * av.get(0);
-        a.v[Np + i] += a.pref * av.get(1);
-        a.v[2 * Np + i] += a.pref * av.get(2);
+        double v3[3] = {av.get(0), av.get(1), av.get(2)};
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int d = 0; d < 3; d++) v3[d] += __shfl_xor(v3[d], off, 64);
+        if (lane == 0) {
+            a.v[i] += a.pref * v3[0];
+            a.v[Np + i] += a.pref * v3[1];
+            a.v[2 * Np + i] += a.pref * v3[2];
+        }
     }
     if (PSI) {
-        a.psi[i] += a.pref * ap.get(0);
-        a.psi[Np + i] += a.pref * ap.get(1);
-        a.psi[2 * Np + i] += a.pref * ap.get(2);
+        double p3[3] = {ap.get(0), ap.get(1), ap.get(2)};
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int d = 0; d < 3; d++) p3[d] += __shfl_xor(p3[d], off, 64);
+        if (lane == 0) {
+            a.psi[i] += a.pref * p3[0];
+            a.psi[Np + i] += a.pref * p3[1];
+            a.psi[2 * Np + i] += a.pref * p3[2];
+        }
     }
     if (STATS) {
-        a.counts[i] = cnt;
-        a.hashes[i] = h;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            cnt += __shfl_xor(cnt, off, 64);
+            hsum += __shfl_xor(hsum, off, 64);
+        }
+        if (lane == 0) {
+            a.counts[i] = cnt;
+            a.hashes[i] = hsum;
+        }
     }
 }
 
@@ -329,11 +444,11 @@ inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs
 
 template <typename CT, bool VEL, bool PSI>
 void launch_pair(const PairArgs<CT> &a, bool stats, hipStream_t s) {
-    const unsigned nb = nblocks(a.Np, 256);
+    const unsigned nb = (unsigned)(8 * ((a.Np + 7) / 8));  // one wavefront per target, padded for the XCD remap
     if (stats)
-        pair_kernel<CT, VEL, PSI, true><<<nb, 256, 0, s>>>(a);
+        pair_wave_kernel<CT, VEL, PSI, true><<<nb, 64, 0, s>>>(a);
     else
-        pair_kernel<CT, VEL, PSI, false><<<nb, 256, 0, s>>>(a);
+        pair_wave_kernel<CT, VEL, PSI, false><<<nb, 64, 0, s>>>(a);
 }
 
 template <typename CT>

@@ -279,7 +279,7 @@ __device__ __forceinline__ void dft(C2<CT> (&v)[R]) {
 // Kept out of line: inlined into the radix switch, the passes' live ranges
 // merge and the kernels need ~50% more registers.
 template <typename CT, int R, int S, int KIND>
-__device__ __noinline__ void stockham_pass(int s_off, int lgB, int L, int Ns, int tw_off) {
+__device__ __noinline__ void stockham_pass(int s_off, int lgB, int ld, int L, int Ns, int tw_off) {
     C2<CT> *Sm = lds<CT>(s_off);
     const C2<CT> *tw = lds<CT>(tw_off);
     constexpr int NT = kind_nt(KIND), TILE = kind_tile<CT>(KIND);
@@ -295,7 +295,7 @@ __device__ __noinline__ void stockham_pass(int s_off, int lgB, int L, int Ns, in
         if (w < nwork) {
             const int b = w & B1, j = w >> lgB;
 #pragma unroll
-            for (int k = 0; k < R; k++) v[i][k] = Sm[((j + k * nb) << lgB) + b];
+            for (int k = 0; k < R; k++) v[i][k] = Sm[(j + k * nb) * ld + b];
         }
     }
     __syncthreads();
@@ -316,26 +316,27 @@ __device__ __noinline__ void stockham_pass(int s_off, int lgB, int L, int Ns, in
             dft<R, S>(v[i]);
             const int base = q * Ns * R + jm;
 #pragma unroll
-            for (int k = 0; k < R; k++) Sm[((base + k * Ns) << lgB) + b] = v[i][k];
+            for (int k = 0; k < R; k++) Sm[(base + k * Ns) * ld + b] = v[i][k];
         }
     }
     __syncthreads();
 }
 
 // Whole transform of every column of the tile (natural order in and out).
-// s_off, tw_off: element offsets of the tile and of the twiddle table in spec_smem.
+// s_off, tw_off: element offsets of the tile and of the twiddle table in spec_smem;
+// the tile has 2^lgB columns and row stride ld.
 template <typename CT, int S, int KIND>
-__device__ void lds_fft(int s_off, int lgB, const AxisPlan &ax, int tw_off) {
+__device__ void lds_fft(int s_off, int lgB, int ld, const AxisPlan &ax, int tw_off) {
     int Ns = 1;
     for (int p = 0; p < ax.npass; p++) {
         const int R = ax.radix[p];
         switch (R) {
-            case 16: stockham_pass<CT, 16, S, KIND>(s_off, lgB, ax.L, Ns, tw_off); break;
-            case 8: stockham_pass<CT, 8, S, KIND>(s_off, lgB, ax.L, Ns, tw_off); break;
-            case 4: stockham_pass<CT, 4, S, KIND>(s_off, lgB, ax.L, Ns, tw_off); break;
-            case 2: stockham_pass<CT, 2, S, KIND>(s_off, lgB, ax.L, Ns, tw_off); break;
-            case 3: stockham_pass<CT, 3, S, KIND>(s_off, lgB, ax.L, Ns, tw_off); break;
-            default: stockham_pass<CT, 5, S, KIND>(s_off, lgB, ax.L, Ns, tw_off); break;
+            case 16: stockham_pass<CT, 16, S, KIND>(s_off, lgB, ld, ax.L, Ns, tw_off); break;
+            case 8: stockham_pass<CT, 8, S, KIND>(s_off, lgB, ld, ax.L, Ns, tw_off); break;
+            case 4: stockham_pass<CT, 4, S, KIND>(s_off, lgB, ld, ax.L, Ns, tw_off); break;
+            case 2: stockham_pass<CT, 2, S, KIND>(s_off, lgB, ld, ax.L, Ns, tw_off); break;
+            case 3: stockham_pass<CT, 3, S, KIND>(s_off, lgB, ld, ax.L, Ns, tw_off); break;
+            default: stockham_pass<CT, 5, S, KIND>(s_off, lgB, ld, ax.L, Ns, tw_off); break;
         }
         Ns *= R;
     }
@@ -344,9 +345,9 @@ __device__ void lds_fft(int s_off, int lgB, const AxisPlan &ax, int tw_off) {
 // The same pass with the length, tile width and stride known at compile time (the
 // grid sizes of the common configurations): strides fold into immediate offsets
 // and the index arithmetic into shifts and constant multiplies.
-template <typename CT, int R, int S, int NT, int L, int LGB, int NS>
+template <typename CT, int R, int S, int NT, int L, int BW, int NS>
 __device__ __forceinline__ void stockham_static(int s_off, int tw_off) {
-    constexpr int NB = L / R, NWORK = NB << LGB, B1 = (1 << LGB) - 1, TWS = L / (NS * R);
+    constexpr int NB = L / R, NWORK = NB * BW, TWS = L / (NS * R);
     constexpr int ITEMS = (NWORK + NT - 1) / NT;
     C2<CT> *Sm = lds<CT>(s_off);
     const C2<CT> *tw = lds<CT>(tw_off);
@@ -355,9 +356,9 @@ __device__ __forceinline__ void stockham_static(int s_off, int tw_off) {
     for (int i = 0; i < ITEMS; i++) {
         const int w = threadIdx.x + NT * i;
         if (NWORK % NT == 0 || w < NWORK) {
-            const int b = w & B1, j = w >> LGB;
+            const int b = w % BW, j = w / BW;
 #pragma unroll
-            for (int k = 0; k < R; k++) v[i][k] = Sm[((j + k * NB) << LGB) + b];
+            for (int k = 0; k < R; k++) v[i][k] = Sm[(j + k * NB) * BW + b];
         }
     }
     __syncthreads();
@@ -365,7 +366,7 @@ __device__ __forceinline__ void stockham_static(int s_off, int tw_off) {
     for (int i = 0; i < ITEMS; i++) {
         const int w = threadIdx.x + NT * i;
         if (NWORK % NT == 0 || w < NWORK) {
-            const int b = w & B1, j = w >> LGB;
+            const int b = w % BW, j = w / BW;
             const int q = j / NS, jm = j - q * NS;
             if constexpr (NS > 1) {
                 const int t1 = jm * TWS;
@@ -379,30 +380,33 @@ __device__ __forceinline__ void stockham_static(int s_off, int tw_off) {
             dft<R, S>(v[i]);
             const int base = q * NS * R + jm;
 #pragma unroll
-            for (int k = 0; k < R; k++) Sm[((base + k * NS) << LGB) + b] = v[i][k];
+            for (int k = 0; k < R; k++) Sm[(base + k * NS) * BW + b] = v[i][k];
         }
     }
     __syncthreads();
 }
 
-template <typename CT, int S, int NT, int L, int LGB, int P = 0, int NS = 1>
+// All passes of a compile-time plan on a tile of BW columns (row stride BW).
+template <typename CT, int S, int NT, int L, int BW, int P = 0, int NS = 1>
 __device__ __forceinline__ void fft_static(int s_off, int tw_off) {
     constexpr AxisPlan rd = plan_axis(L);
     if constexpr (P < rd.npass) {
         constexpr int R = rd.radix[P];
-        stockham_static<CT, R, S, NT, L, LGB, NS>(s_off, tw_off);
-        fft_static<CT, S, NT, L, LGB, P + 1, NS * R>(s_off, tw_off);
+        stockham_static<CT, R, S, NT, L, BW, NS>(s_off, tw_off);
+        fft_static<CT, S, NT, L, BW, P + 1, NS * R>(s_off, tw_off);
     }
 }
 
-// Transform of every column of a tile: compile-time length LS, or LS = 0 for the
-// run-time plan ax with tile width 2^lgB.
-template <typename CT, int S, int KIND, int LS>
+// Transform of every column of NC side-by-side tiles of 2^lgB columns each (row
+// stride NC·2^lgB): compile-time length LS (one pass sequence over all NC·2^lgB
+// columns), or LS = 0 for the run-time plan ax (tile by tile).
+template <typename CT, int S, int KIND, int LS, int NC = 1>
 __device__ __forceinline__ void fft_tile(int s_off, int lgB, const AxisPlan &ax, int tw_off) {
-    if constexpr (LS > 0)
-        fft_static<CT, S, kind_nt(KIND), LS, tile_lgb<CT>(LS, KIND)>(s_off, tw_off);
-    else
-        lds_fft<CT, S, KIND>(s_off, lgB, ax, tw_off);
+    if constexpr (LS > 0) {
+        fft_static<CT, S, kind_nt(KIND), LS, NC * (1 << tile_lgb<CT>(LS, KIND))>(s_off, tw_off);
+    } else {
+        for (int c = 0; c < NC; c++) lds_fft<CT, S, KIND>(s_off + (c << lgB), lgB, NC << lgB, ax, tw_off);
+    }
 }
 
 // Blocks are dealt to the 8 XCDs round-robin; remap so that consecutive tiles
@@ -421,8 +425,10 @@ __device__ __forceinline__ void load_twiddles(C2<CT> *twl, const C2<CT> *__restr
 // Tile element e = threadIdx.x + NT·i (i < EMAX) ↔ (row e >> lgB, column e & (B-1)):
 // gathers all of a thread's global loads before its LDS stores, so that they are in
 // flight together. get(row, col, ok) returns the value (ok: e inside the tile).
+// (ld, coff): the tile sits at column offset coff of an LDS array with row stride ld
+// (ld = 0: a tile of its own, row stride 2^lgB).
 template <typename CT, int KIND, typename F>
-__device__ __forceinline__ void tile_load(C2<CT> *Sm, int n, int lgB, F get) {
+__device__ __forceinline__ void tile_load(C2<CT> *Sm, int n, int lgB, F get, int ld = 0, int coff = 0) {
     constexpr int NT = kind_nt(KIND), EMAX = kind_tile<CT>(KIND) / NT;
     C2<CT> t[EMAX];
 #pragma unroll
@@ -433,17 +439,20 @@ __device__ __forceinline__ void tile_load(C2<CT> *Sm, int n, int lgB, F get) {
 #pragma unroll
     for (int i = 0; i < EMAX; i++) {
         const int e = threadIdx.x + NT * i;
-        if (e < n) Sm[e] = t[i];
+        if (e < n) Sm[ld ? (e >> lgB) * ld + coff + (e & ((1 << lgB) - 1)) : e] = t[i];
     }
 }
 
 template <typename CT, int KIND, typename F>
-__device__ __forceinline__ void tile_store(const C2<CT> *Sm, int n, int lgB, F put) {
+__device__ __forceinline__ void tile_store(const C2<CT> *Sm, int n, int lgB, F put, int ld = 0, int coff = 0) {
     constexpr int NT = kind_nt(KIND), EMAX = kind_tile<CT>(KIND) / NT;
 #pragma unroll
     for (int i = 0; i < EMAX; i++) {
         const int e = threadIdx.x + NT * i;
-        if (e < n) put(e >> lgB, e & ((1 << lgB) - 1), Sm[e]);
+        if (e < n) {
+            const int r = e >> lgB, b = e & ((1 << lgB) - 1);
+            put(r, b, Sm[ld ? r * ld + coff + b : e]);
+        }
     }
 }
 
@@ -452,7 +461,7 @@ __device__ __forceinline__ void tile_store(const C2<CT> *Sm, int n, int lgB, F p
 // and the next one), 16-byte accesses in FP32. get2(row, b) / put2(row, b, v) move
 // the pair (b, b+1).
 template <typename CT, int KIND, typename F>
-__device__ __forceinline__ void tile_load2(C2<CT> *Sm, int n, int lgB, F get2) {
+__device__ __forceinline__ void tile_load2(C2<CT> *Sm, int n, int lgB, F get2, int ld = 0, int coff = 0) {
     constexpr int NT = kind_nt(KIND), EMAX = kind_tile<CT>(KIND) / NT / 2;
     const int n2 = n >> 1, lg2 = lgB - 1;
     C2x2<CT> t[EMAX];
@@ -464,18 +473,21 @@ __device__ __forceinline__ void tile_load2(C2<CT> *Sm, int n, int lgB, F get2) {
 #pragma unroll
     for (int i = 0; i < EMAX; i++) {
         const int e = threadIdx.x + NT * i;
-        if (e < n2) store_pair(Sm + 2 * e, t[i]);
+        if (e < n2) store_pair(Sm + (ld ? (e >> lg2) * ld + coff + ((e & ((1 << lg2) - 1)) << 1) : 2 * e), t[i]);
     }
 }
 
 template <typename CT, int KIND, typename F>
-__device__ __forceinline__ void tile_store2(const C2<CT> *Sm, int n, int lgB, F put2) {
+__device__ __forceinline__ void tile_store2(const C2<CT> *Sm, int n, int lgB, F put2, int ld = 0, int coff = 0) {
     constexpr int NT = kind_nt(KIND), EMAX = kind_tile<CT>(KIND) / NT / 2;
     const int n2 = n >> 1, lg2 = lgB - 1;
 #pragma unroll
     for (int i = 0; i < EMAX; i++) {
         const int e = threadIdx.x + NT * i;
-        if (e < n2) put2(e >> lg2, (e & ((1 << lg2) - 1)) << 1, load_pair(Sm + 2 * e));
+        if (e < n2) {
+            const int r = e >> lg2, b = (e & ((1 << lg2) - 1)) << 1;
+            put2(r, b, load_pair(Sm + (ld ? r * ld + coff + b : 2 * e)));
+        }
     }
 }
 
@@ -658,7 +670,10 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
     const int lgB = LS ? tile_lgb<CT>(LS, KIND_Z) : g.lgB[2];
     const int B = 1 << lgB;
     const int n = L << lgB;
-    C2<CT> *T[3] = {lds<CT>(0), lds<CT>(n), lds<CT>(2 * n)};
+    // the three component tiles side by side: component c at columns [c·B, c·B + B) of
+    // an LDS array with row stride 3B, so that one transform call covers all three
+    C2<CT> *Sm = lds<CT>(0);
+    const int ld = 3 << lgB;
     const int tw_off = 3 * n;
     C2<CT> *twl = lds<CT>(tw_off);
     const int nkx = g.nkx, nky = g.nky, nkb = (nkx + B - 1) >> lgB;
@@ -676,12 +691,14 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
         for (int c = 0; c < 3; c++) {
             const C2<CT> *src = x2 + c * cs + off;
             if (pairs)
-                tile_load2<CT, KIND_Z>(T[c], n, lgB,
-                                       [&](int z, int b) { return ld2(src + (int64_t)z * nkx + b, nkx - kx0 - b); });
+                tile_load2<CT, KIND_Z>(
+                    Sm, n, lgB, [&](int z, int b) { return ld2(src + (int64_t)z * nkx + b, nkx - kx0 - b); }, ld,
+                    c << lgB);
             else
-                tile_load<CT, KIND_Z>(T[c], n, lgB, [&](int z, int b) {
-                    return kx0 + b < nkx ? src[(int64_t)z * nkx + b] : C2<CT>{(CT)0, (CT)0};
-                });
+                tile_load<CT, KIND_Z>(
+                    Sm, n, lgB,
+                    [&](int z, int b) { return kx0 + b < nkx ? src[(int64_t)z * nkx + b] : C2<CT>{(CT)0, (CT)0}; },
+                    ld, c << lgB);
         }
     };
     auto store3 = [&](C2<CT> *out) {
@@ -689,18 +706,22 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
         for (int c = 0; c < 3; c++) {
             C2<CT> *dst = out + c * cs + off;
             if (pairs)
-                tile_store2<CT, KIND_Z>(T[c], n, lgB, [&](int z, int b, const C2x2<CT> &v) {
-                    st2(dst + (int64_t)z * nkx + b, v, nkx - kx0 - b);
-                });
+                tile_store2<CT, KIND_Z>(
+                    Sm, n, lgB,
+                    [&](int z, int b, const C2x2<CT> &v) { st2(dst + (int64_t)z * nkx + b, v, nkx - kx0 - b); }, ld,
+                    c << lgB);
             else
-                tile_store<CT, KIND_Z>(T[c], n, lgB, [&](int z, int b, C2<CT> v) {
-                    if (kx0 + b < nkx) dst[(int64_t)z * nkx + b] = v;
-                });
+                tile_store<CT, KIND_Z>(
+                    Sm, n, lgB,
+                    [&](int z, int b, C2<CT> v) {
+                        if (kx0 + b < nkx) dst[(int64_t)z * nkx + b] = v;
+                    },
+                    ld, c << lgB);
         }
     };
     load3();
     __syncthreads();
-    for (int c = 0; c < 3; c++) fft_tile<CT, -1, KIND_Z, LS>(c * n, lgB, g.ax[2], tw_off);
+    fft_tile<CT, -1, KIND_Z, LS, 3>(0, lgB, g.ax[2], tw_off);
     // Fourier-space operators on the retained modes
     const int K1 = g.K1, K2 = g.K2, nkz = g.nkz;
     const int ky = jy <= K1 ? jy : jy - nky;
@@ -710,6 +731,7 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
         const int e = threadIdx.x + NTZ * i;
         if (e >= n) break;
         const int b = e & (B - 1), iz = e >> lgB;
+        C2<CT> *T0 = Sm + iz * ld + b;  // component d at T0[d << lgB]
         const int kx = kx0 + b;
         const int kz = iz <= (L >> 1) ? iz : iz - L;
         const bool kz_kept = kz >= -K2 && kz <= K2;
@@ -729,7 +751,7 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
                 for (int d = 0; d < 3; d++) u[d * f.ucstride] = zero;
             }
 #pragma unroll
-            for (int d = 0; d < 3; d++) T[d][e] = zero;
+            for (int d = 0; d < 3; d++) T0[d << lgB] = zero;
             if (both && kx < nkx) {
 #pragma unroll
                 for (int d = 0; d < 3; d++) x2[d * cs + off + (int64_t)iz * nkx + b] = zero;
@@ -739,7 +761,7 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
         double wr[3], wi[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const C2<CT> w = T[d][e];
+            const C2<CT> w = T0[d << lgB];
             wr[d] = (double)w.re;
             wi[d] = (double)w.im;
         }
@@ -765,7 +787,7 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
         for (int d = 0; d < 3; d++) wr[d] *= s, wi[d] *= s;
         if (f.do_psi) {
 #pragma unroll
-            for (int d = 0; d < 3; d++) T[d][e] = C2<CT>{(CT)wr[d], (CT)wi[d]};
+            for (int d = 0; d < 3; d++) T0[d << lgB] = C2<CT>{(CT)wr[d], (CT)wi[d]};
         }
         if (f.do_vel) {
             // v̂ = i k × ψ̂: (i k × w)_x = i (KY w_z - KZ w_y), i (a + ib) = -b + ia
@@ -778,12 +800,12 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
                 for (int d = 0; d < 3; d++) x2[d * cs + off + (int64_t)iz * nkx + b] = v[d];
             } else {
 #pragma unroll
-                for (int d = 0; d < 3; d++) T[d][e] = v[d];
+                for (int d = 0; d < 3; d++) T0[d << lgB] = v[d];
             }
         }
     }
     __syncthreads();
-    for (int c = 0; c < 3; c++) fft_tile<CT, 1, KIND_Z, LS>(c * n, lgB, g.ax[2], tw_off);
+    fft_tile<CT, 1, KIND_Z, LS, 3>(0, lgB, g.ax[2], tw_off);
     store3(f.do_psi ? x2b : x2);
     if (both) {
         // this block's writes to x2 above are visible to its own loads after the barrier
@@ -791,7 +813,7 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
         __syncthreads();
         load3();
         __syncthreads();
-        for (int c = 0; c < 3; c++) fft_tile<CT, 1, KIND_Z, LS>(c * n, lgB, g.ax[2], tw_off);
+        fft_tile<CT, 1, KIND_Z, LS, 3>(0, lgB, g.ax[2], tw_off);
         store3(x2);
     }
 }
