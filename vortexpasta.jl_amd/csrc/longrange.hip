@@ -290,10 +290,18 @@ __global__ __launch_bounds__(64) void spread_rows_kernel(LRGeom g, int64_t Nq, c
                 ry = ry < TY ? ry : ry - g.G[1];  // ∈ (-W, TY), likewise below
                 rz = rz < TZ ? rz : rz - g.G[2];
             }
-            const unsigned long long mask = __ballot(keep);
-            const int nkept = __popcll(mask);
+            // the window sx + [0, W) lies in the left half of the row when sx ≤ 0, in the
+            // right half when sx ≥ TX/2 (W ≤ TX/2, more than one tile along x): the kept
+            // points are compacted in three groups (left, right, both), each then walked
+            // by its own branch-free loop
+            constexpr int H = TX / 2;
+            const int cls = (W <= H && sx <= 0 && g.G[0] != TX) ? 0 : (W <= H && sx >= H) ? 1 : 2;
+            const unsigned long long m0 = __ballot(keep && cls == 0), m1 = __ballot(keep && cls == 1),
+                                     m2 = __ballot(keep && cls == 2);
+            const int n0 = __popcll(m0), n1 = __popcll(m1), nkept = n0 + n1 + __popcll(m2);
             if (keep) {
-                const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+                const unsigned long long lt = (1ull << lane) - 1ull;
+                const int pos = cls == 0 ? __popcll(m0 & lt) : cls == 1 ? n0 + __popcll(m1 & lt) : n0 + n1 + __popcll(m2 & lt);
                 s_meta[pos] = make_int4(sx, ry, rz, 0);
                 const CT *pr = rec + (int64_t)sidx * REC;
 #pragma unroll
@@ -310,22 +318,29 @@ __global__ __launch_bounds__(64) void spread_rows_kernel(LRGeom g, int64_t Nq, c
             }
             __syncthreads();
             // 3) accumulate the kept points in order: acc[x] += w16[x]·(wy·wz·q) over the
-            //    whole row, w16 = the point's x-weights shifted to the tile (zero outside)
-            for (int p = 0; p < nkept; p++) {
-                const int4 m = s_meta[p];
-                const int ky = ly - m.y, kz = lz - m.z;
-                CT wyz = (CT)0;
-                if ((unsigned)ky < (unsigned)W && (unsigned)kz < (unsigned)W)
-                    wyz = s_rec[p][ky] * s_rec[p][W + kz];
-                const CT a0 = wyz * s_rec[p][2 * W], a1 = wyz * s_rec[p][2 * W + 1], a2 = wyz * s_rec[p][2 * W + 2];
-                static_for<0, TX>([&](auto X) {
-                    constexpr int x = decltype(X)::value;
-                    const CT w = s_w16[p][x];
-                    acc[x][0] += w * a0;
-                    acc[x][1] += w * a1;
-                    acc[x][2] += w * a2;
-                });
-            }
+            //    columns [X0, X1) of the row, w16 = the point's x-weights shifted to the
+            //    tile (zero outside)
+            auto walk = [&](int p0, int p1, auto X0, auto X1) {
+                for (int p = p0; p < p1; p++) {
+                    const int4 m = s_meta[p];
+                    const int ky = ly - m.y, kz = lz - m.z;
+                    CT wyz = (CT)0;
+                    if ((unsigned)ky < (unsigned)W && (unsigned)kz < (unsigned)W)
+                        wyz = s_rec[p][ky] * s_rec[p][W + kz];
+                    const CT a0 = wyz * s_rec[p][2 * W], a1 = wyz * s_rec[p][2 * W + 1],
+                             a2 = wyz * s_rec[p][2 * W + 2];
+                    static_for<decltype(X0)::value, decltype(X1)::value>([&](auto X) {
+                        constexpr int x = decltype(X)::value;
+                        const CT w = s_w16[p][x];
+                        acc[x][0] += w * a0;
+                        acc[x][1] += w * a1;
+                        acc[x][2] += w * a2;
+                    });
+                }
+            };
+            walk(0, n0, std::integral_constant<int, 0>{}, std::integral_constant<int, H>{});
+            walk(n0, n0 + n1, std::integral_constant<int, H>{}, std::integral_constant<int, TX>{});
+            walk(n0 + n1, nkept, std::integral_constant<int, 0>{}, std::integral_constant<int, TX>{});
             __syncthreads();
         }
     }

@@ -727,6 +727,19 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
     const int ky = jy <= K1 ? jy : jy - nky;
     const double KY = f.kscale[1] * ky;
     const bool both = f.do_vel && f.do_psi;
+    // a thread's elements share one column (NTZ is a multiple of B): the kx and ky
+    // factors are loaded once, the kz factors all at once before the loop
+    const double fxy = threadIdx.x < n && kx0 + (threadIdx.x & (B - 1)) < nkx
+                           ? f.pref * f.ex[kx0 + (threadIdx.x & (B - 1))] * f.ey[ky + K1]
+                           : 0.0;
+    double fz[EMAX];
+#pragma unroll
+    for (int i = 0; i < EMAX; i++) {
+        const int e = threadIdx.x + NTZ * i;
+        const int iz = e >> lgB, kz = iz <= (L >> 1) ? iz : iz - L;
+        fz[i] = (e < n && kz >= -K2 && kz <= K2) ? f.ez[kz + K2] : 0.0;
+    }
+#pragma unroll
     for (int i = 0; i < EMAX; i++) {
         const int e = threadIdx.x + NTZ * i;
         if (e >= n) break;
@@ -782,7 +795,7 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
             for (int d = 0; d < 3; d++) u[d * f.ucstride] = C2<CT>{(CT)ur[d], (CT)ui[d]};
         }
         // ω̂ = Γ/V · B̂/ψ̂; ψ̂_out = ω̂·e^{-k²/4α²}/(ψ̂ k²) (deconvolution of type 1 and type 2)
-        const double s = f.pref * f.ex[kx] * f.ey[ky + K1] * f.ez[kz + K2] * inv_k2;
+        const double s = fxy * fz[i] * inv_k2;
 #pragma unroll
         for (int d = 0; d < 3; d++) wr[d] *= s, wi[d] *= s;
         if (f.do_psi) {
