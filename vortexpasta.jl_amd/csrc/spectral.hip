@@ -132,16 +132,18 @@ struct AxisPlan {
     int radix[8];
 };
 
-// Radices of a transform length: 16s first, then one of 8/4/2, then 3s and 5s.
+// Radices of a transform length: 3s and 5s first, then 16s, then one of 8/4/2. The
+// first pass writes rows R apart; with an odd radix there those writes spread over
+// the LDS banks (rows 16 apart all fall on the same banks).
 __host__ __device__ constexpr AxisPlan plan_axis(int L) {
     AxisPlan a{L, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
     int n = L;
+    while (n % 3 == 0 && a.npass < 8) a.radix[a.npass++] = 3, n /= 3;
+    while (n % 5 == 0 && a.npass < 8) a.radix[a.npass++] = 5, n /= 5;
     while (n % 16 == 0 && a.npass < 8) a.radix[a.npass++] = 16, n /= 16;
     if (n % 8 == 0 && a.npass < 8) a.radix[a.npass++] = 8, n /= 8;
     if (n % 4 == 0 && a.npass < 8) a.radix[a.npass++] = 4, n /= 4;
     if (n % 2 == 0 && a.npass < 8) a.radix[a.npass++] = 2, n /= 2;
-    while (n % 3 == 0 && a.npass < 8) a.radix[a.npass++] = 3, n /= 3;
-    while (n % 5 == 0 && a.npass < 8) a.radix[a.npass++] = 5, n /= 5;
     if (n != 1) a.npass = -1;
     return a;
 }
