@@ -647,3 +647,18 @@ def test_fft_paths_agree_static_lengths(vp, Ns):
             err = rel_l2(outs["fused"][k], outs["rocfft"][k])
             print(f"  Ns={Ns} {T.__name__} {k}: fused vs rocFFT {err:.2e}", flush=True)
             assert err <= tol
+
+
+def test_empty_and_minimal_inputs(vp, O):
+    """No filaments returns the empty outputs; a five-node filament (the fewest a closed quintic spline
+    takes) and a mix of very different filament sizes match the oracle (FP64, 1e-12)."""
+    params = _params(vp)
+    cache = vp.init_cache(params)
+    out = {"velocity": []}
+    assert vp.compute_on_nodes_(out, cache, []) is out and out["velocity"] == []
+    cache.close()
+    R = math.pi / 3
+    small = vp.filaments.init(vp.define_curve(vp.Ring(), scale=0.5, translate=(1.0, 2.0, 3.0)), 5)
+    big = vp.filaments.init(vp.define_curve(vp.Ring(), scale=R, translate=math.pi), 96)
+    _check(vp, O, params, [small])
+    _check(vp, O, params, [small, big])

@@ -163,6 +163,9 @@ def compute_on_nodes_(
         if len(us) != len(fs) or any(np.shape(u) != (len(f), 3) for u, f in zip(us, fs)):
             raise ValueError(f"wrong dimensions of output vector '{name}'")
     params = cache.params
+    if len(fs) == 0:
+        # no filaments: nothing to evaluate (the output lists are empty as well)
+        return fields
     if batch is None:
         batch = FilamentBatch(fs)
     if cache.pointdata_mode == "device":

@@ -703,91 +703,94 @@ __global__ void spectrum_reduce(const double *__restrict__ partial, int nblocks,
 // ----------------------------------------------------------------------------
 // Interpolation (type 2).
 
-// Type-2 interpolation with a W³ stencil (W = 2m): one wavefront per node.
-// Lane (a, b) = (lane % W, lane / W) owns stencil column x = i0x + a and rows
-// y = i0y + b + R·pass (R = ⌊64/W⌋ rows per pass) and walks the W z-planes, so a
-// load instruction touches R rows of W consecutive values; the partial sums
-// are combined by a butterfly reduction (pairwise summation). Every lane
-// evaluates the W z-weights itself: broadcasting FP64 weights with a wave
-// shuffle was measured to lose precision (~1e-7 relative against the oracle),
-// and the redundant evaluation is cheap next to the grid loads.
+// Type-2 interpolation with a W³ stencil (W = 2m): one wavefront (block) per node.
+// Lanes 0..3W-1 evaluate the node's 3·W kernel weights once and share them through
+// LDS (exact copies); lane (a, b) = (lane % W, lane / W) then owns stencil column
+// x = i0x + a and rows y = i0y + b + R·pass (R = ⌊64/W⌋ rows per pass) and walks the
+// W z-planes, so a load instruction touches R rows of W consecutive values; the
+// partial sums are combined by a butterfly reduction (pairwise summation).
 template <typename CT, int W>
-__global__ __launch_bounds__(256) void interp_kernel_wave(LRGeom g, int64_t Np, const double *__restrict__ nodes,
-                                                          CT beta, const CT *__restrict__ grid,
-                                                          double *__restrict__ out) {
+__global__ __launch_bounds__(64) void interp_kernel_wave(LRGeom g, int64_t Np, const double *__restrict__ nodes,
+                                                         CT beta, const CT *__restrict__ grid,
+                                                         double *__restrict__ out) {
     constexpr int M = W / 2;
     constexpr int R = 64 / W;                  // y rows per pass
     constexpr int PASSES = (W + R - 1) / R;
-    const int lane = threadIdx.x & 63;
+    __shared__ CT s_w[3][W];
+    const int lane = threadIdx.x;
     const int a = lane % W, b = lane / W;
     const bool lane_on = b < R;                // lanes beyond R·W idle
     // Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD works
     // on a contiguous range of nodes (consecutive nodes share stencil rows,
     // which then stay in that XCD's L2).
     const int64_t per = ((int64_t)gridDim.x + 7) / 8;
-    const int64_t blk = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
-    {
-        const int64_t i = blk * 4 + (threadIdx.x >> 6);
-        if (i >= Np) return;
-        const double ux = nodes[i] * g.scale[0];
-        const double uy = nodes[Np + i] * g.scale[1];
-        const double uz = nodes[2 * Np + i] * g.scale[2];
-        const int i0x = (int)floor(ux) - M + 1, i0y = (int)floor(uy) - M + 1, i0z = (int)floor(uz) - M + 1;
-        const CT wx = es_kernel<CT>((CT)(((double)(i0x + a) - ux) / (double)M), beta);
-        CT wz[W];
-        int64_t gzo[W];
-        const int64_t plane = (int64_t)g.G[0] * g.G[1];  // values per z-plane
+    const int64_t i = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (i >= Np) return;  // the whole (one-wavefront) block
+    const double ux = nodes[i] * g.scale[0];
+    const double uy = nodes[Np + i] * g.scale[1];
+    const double uz = nodes[2 * Np + i] * g.scale[2];
+    const int i0x = (int)floor(ux) - M + 1, i0y = (int)floor(uy) - M + 1, i0z = (int)floor(uz) - M + 1;
+    if (lane < 3 * W) {
+        const int t = lane / W, c = lane - t * W;
+        const double u = t == 0 ? ux : t == 1 ? uy : uz;
+        const int i0 = t == 0 ? i0x : t == 1 ? i0y : i0z;
+        s_w[t][c] = es_kernel<CT>((CT)(((double)(i0 + c) - u) / (double)M), beta);
+    }
+    __syncthreads();
+    const CT wx = s_w[0][a];
+    CT wz[W];
+    int64_t gzo[W];
+    const int64_t plane = (int64_t)g.G[0] * g.G[1];  // values per z-plane
 #pragma unroll
-        for (int c = 0; c < W; c++) {
-            wz[c] = es_kernel<CT>((CT)(((double)(i0z + c) - uz) / (double)M), beta);
-            gzo[c] = (int64_t)wrapi(i0z + c, g.G[2]) * plane;
-        }
-        const int64_t gx = 2 * (int64_t)wrapi(i0x + a, g.G[0]);
-        CT acc0 = 0, acc1 = 0, acc2 = 0;
+    for (int c = 0; c < W; c++) {
+        wz[c] = s_w[2][c];
+        gzo[c] = (int64_t)wrapi(i0z + c, g.G[2]) * plane;
+    }
+    const int64_t gx = 2 * (int64_t)wrapi(i0x + a, g.G[0]);
+    CT acc0 = 0, acc1 = 0, acc2 = 0;
 #pragma unroll
-        for (int pass = 0; pass < PASSES; pass++) {
-            const int bb = b + R * pass;
-            if (lane_on && bb < W) {
-                const CT wy = es_kernel<CT>((CT)(((double)(i0y + bb) - uy) / (double)M), beta);
-                const int iy = wrapi(i0y + bb, g.G[1]);
-                const int64_t gy = (int64_t)(iy >> 1) * 2 * g.G[0] + (iy & 1);
-                const CT wxy = wx * wy;
+    for (int pass = 0; pass < PASSES; pass++) {
+        const int bb = b + R * pass;
+        if (lane_on && bb < W) {
+            const CT wy = s_w[1][bb];
+            const int iy = wrapi(i0y + bb, g.G[1]);
+            const int64_t gy = (int64_t)(iy >> 1) * 2 * g.G[0] + (iy & 1);
+            const CT wxy = wx * wy;
 #pragma unroll
-                for (int c = 0; c < W; c++) {
-                    const int64_t off = gzo[c] + gy + gx;
-                    const CT w = wxy * wz[c];
-                    acc0 += w * grid[off];
-                    acc1 += w * grid[g.cstride + off];
-                    acc2 += w * grid[2 * g.cstride + off];
-                }
+            for (int c = 0; c < W; c++) {
+                const int64_t off = gzo[c] + gy + gx;
+                const CT w = wxy * wz[c];
+                acc0 += w * grid[off];
+                acc1 += w * grid[g.cstride + off];
+                acc2 += w * grid[2 * g.cstride + off];
             }
         }
+    }
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            acc0 += __shfl_xor(acc0, o, 64);
-            acc1 += __shfl_xor(acc1, o, 64);
-            acc2 += __shfl_xor(acc2, o, 64);
-        }
-        if (lane == 0) {
-            out[i] = (double)acc0;
-            out[Np + i] = (double)acc1;
-            out[2 * Np + i] = (double)acc2;
-        }
+    for (int o = 32; o > 0; o >>= 1) {
+        acc0 += __shfl_xor(acc0, o, 64);
+        acc1 += __shfl_xor(acc1, o, 64);
+        acc2 += __shfl_xor(acc2, o, 64);
+    }
+    if (lane == 0) {
+        out[i] = (double)acc0;
+        out[Np + i] = (double)acc1;
+        out[2 * Np + i] = (double)acc2;
     }
 }
 
 template <typename CT>
 void launch_interp(const LRGeom &g, int64_t Np, const double *nodes, CT beta, const CT *grid, double *out,
                    hipStream_t s) {
-    const unsigned nb = (unsigned)(8 * (((Np + 3) / 4 + 7) / 8));  // one wavefront per node, 4 per block
+    const unsigned nb = (unsigned)(8 * ((Np + 7) / 8));  // one wavefront per node and block
     switch (g.M) {
-        case 2: interp_kernel_wave<CT, 4><<<nb, 256, 0, s>>>(g, Np, nodes, beta, grid, out); break;
-        case 3: interp_kernel_wave<CT, 6><<<nb, 256, 0, s>>>(g, Np, nodes, beta, grid, out); break;
-        case 4: interp_kernel_wave<CT, 8><<<nb, 256, 0, s>>>(g, Np, nodes, beta, grid, out); break;
-        case 5: interp_kernel_wave<CT, 10><<<nb, 256, 0, s>>>(g, Np, nodes, beta, grid, out); break;
-        case 6: interp_kernel_wave<CT, 12><<<nb, 256, 0, s>>>(g, Np, nodes, beta, grid, out); break;
-        case 7: interp_kernel_wave<CT, 14><<<nb, 256, 0, s>>>(g, Np, nodes, beta, grid, out); break;
-        default: interp_kernel_wave<CT, 16><<<nb, 256, 0, s>>>(g, Np, nodes, beta, grid, out); break;
+        case 2: interp_kernel_wave<CT, 4><<<nb, 64, 0, s>>>(g, Np, nodes, beta, grid, out); break;
+        case 3: interp_kernel_wave<CT, 6><<<nb, 64, 0, s>>>(g, Np, nodes, beta, grid, out); break;
+        case 4: interp_kernel_wave<CT, 8><<<nb, 64, 0, s>>>(g, Np, nodes, beta, grid, out); break;
+        case 5: interp_kernel_wave<CT, 10><<<nb, 64, 0, s>>>(g, Np, nodes, beta, grid, out); break;
+        case 6: interp_kernel_wave<CT, 12><<<nb, 64, 0, s>>>(g, Np, nodes, beta, grid, out); break;
+        case 7: interp_kernel_wave<CT, 14><<<nb, 64, 0, s>>>(g, Np, nodes, beta, grid, out); break;
+        default: interp_kernel_wave<CT, 16><<<nb, 64, 0, s>>>(g, Np, nodes, beta, grid, out); break;
     }
 }
 
