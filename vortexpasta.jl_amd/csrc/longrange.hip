@@ -118,24 +118,72 @@ __global__ void lr_tile_keys(int64_t Nq, const double *__restrict__ pts, LRGeom 
 // overlap tests) and one record (rec_len(2m) values) holding the 2m kernel
 // weights φ((i0_d + k - u_d)/m) per direction and the charge, computed once
 // here instead of once per tile in the spreading kernel.
-template <typename CT>
+// Records are rec_len(W) values (a multiple of 16 bytes) moved with 16-byte vector
+// accesses: one thread writes or reads a whole record.
+template <typename CT, int N>
+__device__ __forceinline__ void store_rec(CT *__restrict__ p, const CT (&r)[N]) {
+    static_assert((N * sizeof(CT)) % 16 == 0, "records are whole 16-byte vectors");
+    if constexpr (sizeof(CT) == 4) {
+#pragma unroll
+        for (int k = 0; k < N; k += 4) reinterpret_cast<float4 *>(p)[k / 4] = float4{r[k], r[k + 1], r[k + 2], r[k + 3]};
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; k += 2) reinterpret_cast<double2 *>(p)[k / 2] = double2{r[k], r[k + 1]};
+    }
+}
+template <typename CT, int N>
+__device__ __forceinline__ void load_rec(const CT *__restrict__ p, CT (&r)[N]) {
+    if constexpr (sizeof(CT) == 4) {
+#pragma unroll
+        for (int k = 0; k < N; k += 4) {
+            const float4 v = reinterpret_cast<const float4 *>(p)[k / 4];
+            r[k] = v.x, r[k + 1] = v.y, r[k + 2] = v.z, r[k + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; k += 2) {
+            const double2 v = reinterpret_cast<const double2 *>(p)[k / 2];
+            r[k] = v.x, r[k + 1] = v.y;
+        }
+    }
+}
+
+template <typename CT, int W>
 __global__ void gather_points(int64_t Nq, const int32_t *__restrict__ perm, const double *__restrict__ pts,
                               const double *__restrict__ chg, LRGeom g, CT beta, int32_t *__restrict__ si0,
                               CT *__restrict__ rec) {
+    constexpr int M = W / 2, REC = rec_len(W);
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= Nq) return;
     const int32_t j = perm[s];
-    const int M = g.M, W = 2 * M;
-    CT *r = rec + s * rec_len(W);
+    CT r[REC];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const double u = pts[d * Nq + j] * g.scale[d];
         const int i0 = (int)floor(u) - M + 1;
         si0[d * Nq + s] = wrapi(i0, g.G[d]);
+#pragma unroll
         for (int k = 0; k < W; k++) r[d * W + k] = es_kernel<CT>((CT)(((double)(i0 + k) - u) / (double)M), beta);
         r[3 * W + d] = (CT)chg[d * Nq + j];
     }
-    for (int k = 3 * W + 3; k < rec_len(W); k++) r[k] = (CT)0;
+#pragma unroll
+    for (int k = 3 * W + 3; k < REC; k++) r[k] = (CT)0;
+    store_rec(rec + s * REC, r);
+}
+
+template <typename CT>
+void launch_gather(int64_t Nq, const int32_t *perm, const double *pts, const double *chg, const LRGeom &g, CT beta,
+                   int32_t *si0, CT *rec, hipStream_t s) {
+    const unsigned nb = (unsigned)((Nq + 255) / 256);
+    switch (g.M) {
+        case 2: gather_points<CT, 4><<<nb, 256, 0, s>>>(Nq, perm, pts, chg, g, beta, si0, rec); break;
+        case 3: gather_points<CT, 6><<<nb, 256, 0, s>>>(Nq, perm, pts, chg, g, beta, si0, rec); break;
+        case 4: gather_points<CT, 8><<<nb, 256, 0, s>>>(Nq, perm, pts, chg, g, beta, si0, rec); break;
+        case 5: gather_points<CT, 10><<<nb, 256, 0, s>>>(Nq, perm, pts, chg, g, beta, si0, rec); break;
+        case 6: gather_points<CT, 12><<<nb, 256, 0, s>>>(Nq, perm, pts, chg, g, beta, si0, rec); break;
+        case 7: gather_points<CT, 14><<<nb, 256, 0, s>>>(Nq, perm, pts, chg, g, beta, si0, rec); break;
+        default: gather_points<CT, 16><<<nb, 256, 0, s>>>(Nq, perm, pts, chg, g, beta, si0, rec); break;
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -303,7 +351,8 @@ __global__ __launch_bounds__(64) void spread_rows_kernel(LRGeom g, int64_t Nq, c
                 const unsigned long long lt = (1ull << lane) - 1ull;
                 const int pos = cls == 0 ? __popcll(m0 & lt) : cls == 1 ? n0 + __popcll(m1 & lt) : n0 + n1 + __popcll(m2 & lt);
                 s_meta[pos] = make_int4(sx, ry, rz, 0);
-                const CT *pr = rec + (int64_t)sidx * REC;
+                CT pr[REC];
+                load_rec(rec + (int64_t)sidx * REC, pr);
 #pragma unroll
                 for (int k = 0; k < YZQ; k++) s_rec[pos][k] = pr[W + k];
 #pragma unroll
@@ -916,8 +965,7 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
     stage_begin(c, ST_LRSORT, s);
     lr_tile_keys<<<nblocks(Nq, 256), 256, 0, s>>>(Nq, c->points, g, c->lr_key);
     VPBS_TRY(counting_sort(c, c->sort_lr, c->lr_key, Nq, c->ntiles, c->lr_start, c->lr_perm, s));
-    gather_points<CT><<<nblocks(Nq, 256), 256, 0, s>>>(Nq, c->lr_perm, c->points, c->charges, g, beta, c->lr_i0,
-                                                        (CT *)c->lr_w);
+    launch_gather<CT>(Nq, c->lr_perm, c->points, c->charges, g, beta, c->lr_i0, (CT *)c->lr_w, s);
     stage_end(c, ST_LRSORT, s);
 
     CT *grid = (CT *)c->grid;
