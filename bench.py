@@ -39,11 +39,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(vp, cfg: int, sigma: float = 1.5, m: int = 4):
+def build_workload(vp, cfg: int, sigma: float = 1.5, m: int = 4, fft: str = "fused"):
     fs = vp.tangles.config_tangle(cfg)
     Ns = {1: 64, 2: 128, 3: 512, 4: 256}[cfg]
     spl = vp.GaussianSplitting.create(Ls=2 * math.pi, β=3.5, Ns=Ns)
-    bl = vp.HIPLongRangeBackend(σ=sigma, m=m)
+    bl = vp.HIPLongRangeBackend(σ=sigma, m=m, fft=fft)
     p32 = vp.ParamsBiotSavart.build(np.float32, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl, backend_long=bl)
     p64 = vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl, backend_long=bl)
     batch = vp.FilamentBatch(fs)
@@ -292,6 +292,8 @@ def main():
     ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3, 4])
     ap.add_argument("--sigma", type=float, default=1.5, help="NUFFT oversampling factor")
     ap.add_argument("--m", type=int, default=4, help="NUFFT kernel half-support")
+    ap.add_argument("--fft", default="fused", choices=["fused", "rocfft"],
+                    help="long-range transforms: fused hand-written kernels (default) or rocFFT")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 path (and the error figures)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -307,7 +309,7 @@ def main():
 
     vp = ge.load_package()
     t = time.perf_counter()
-    fs, spl, p32, p64, pd = build_workload(vp, args.config, args.sigma, args.m)
+    fs, spl, p32, p64, pd = build_workload(vp, args.config, args.sigma, args.m, args.fft)
     log(f"workload: {len(fs)} filaments, Np = {pd.Np}, Nq = {pd.Nq}, Ns = {spl.Ns}, α = {spl.α:.4f}, "
         f"rcut = {spl.rcut:.5f} (host prep {time.perf_counter() - t:.1f} s)")
 
@@ -337,7 +339,7 @@ def main():
 
     kname = {"spread": f"spread_rows_kernel<float, {2 * args.m}>", "fourier": "fourier_kernel<float>",
              "interp": f"interp_kernel_wave<float, {2 * args.m}>"}[dom]
-    default_workload = (args.config, args.sigma, args.m) == (3, 1.5, 4)
+    default_workload = (args.config, args.sigma, args.m, args.fft) == (3, 1.5, 4, "fused")
     traffic = measured_traffic(kname) if default_workload else None
 
     result = {
@@ -356,10 +358,10 @@ def main():
         "config": {
             "workload": f"config {args.config}: {len(fs)} random vortex rings, {pd.Np} nodes, {pd.Nq} quadrature "
                         f"points, Fourier grid {spl.Ns[0]}^3 (NUFFT grid {grid[0]}^3, sigma {args.sigma}, m {args.m}), beta 3.5, "
-                        f"FP32-bulk path",
+                        f"FP32-bulk path, {'fused hand-written' if args.fft == 'fused' else 'rocFFT'} transforms",
             "nodes": pd.Np, "quadrature_points": pd.Nq, "fourier_grid": list(spl.Ns),
             "nufft_grid": list(grid), "nufft_sigma": args.sigma, "nufft_m": args.m, "cells": list(cells),
-            "alpha": spl.α, "rcut": spl.rcut,
+            "alpha": spl.α, "rcut": spl.rcut, "fft": args.fft,
         },
         "roofline": {
             "kernel": kname,
