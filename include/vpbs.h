@@ -40,8 +40,11 @@ extern "C" {
 #define VPBS_F32 1 /* FP32 bulk, FP64 node differencing / LIA / local self terms */
 
 /* transforms of the long-range part (vpbs_params.fft) */
-#define VPBS_FFT_FUSED 0  /* hand-written FFTs fused with the pencil reorderings and the Fourier-space kernel */
-#define VPBS_FFT_ROCFFT 1 /* rocFFT 1D transforms with separate reordering and Fourier-space kernels */
+#define VPBS_FFT_FUSED 0    /* hand-written FFTs fused with the pencil reorderings and the Fourier-space kernel;
+                               on x-grids of 192, 384 or 768 cells (768: FP32 only) also the inverse x
+                               transform fused with the interpolation (no real grid written on that side) */
+#define VPBS_FFT_ROCFFT 1   /* rocFFT 1D transforms with separate reordering and Fourier-space kernels */
+#define VPBS_FFT_SEPARATE 2 /* the hand-written FFTs of VPBS_FFT_FUSED with the interpolation in a kernel of its own */
 
 /* vpbs_compute flags */
 #define VPBS_VELOCITY 1u
@@ -80,8 +83,9 @@ typedef struct vpbs_params {
     int32_t avoid_explicit_erf; /* params.jl:289-298 */
     int32_t truncate_spherical; /* params.jl:243-246 */
     int32_t device;        /* HIP device ordinal */
-    int32_t fft;           /* VPBS_FFT_FUSED (default) or VPBS_FFT_ROCFFT; grids the fused transforms
-                              do not cover (x, y > 6144 FP32 / 3072 FP64, z > 3072 / 1536) use rocFFT */
+    int32_t fft;           /* VPBS_FFT_FUSED (default), VPBS_FFT_ROCFFT or VPBS_FFT_SEPARATE; grids the
+                              hand-written transforms do not cover (x, y > 6144 FP32 / 3072 FP64,
+                              z > 3072 / 1536) use rocFFT */
 } vpbs_params;
 
 /* Per-stage GPU times of the last vpbs_compute, in milliseconds (HIP events). */
@@ -95,8 +99,10 @@ typedef struct vpbs_timings {
     double spread;       /* type-1 spreading (all 3 components) */
     double fft_forward;  /* x and y transforms (fused: with the pencil reorderings), 3 components */
     double fourier;      /* Fourier-space kernel (fused: with the forward and inverse z transforms) */
-    double fft_backward; /* inverse y and x transforms of the velocity */
-    double interp;       /* type-2 interpolation */
+    double fft_backward; /* inverse y and x transforms of the velocity (only y when the interpolation
+                            runs the x transform) */
+    double interp;       /* type-2 interpolation (with the inverse x transform when fused; with the
+                            streamfunction, also its inverse transforms) */
     double merge;        /* SR + LR sum */
     int64_t pair_candidates; /* unused (reserved) */
     int64_t grid[3];     /* oversampled NUFFT grid */

@@ -161,9 +161,11 @@ def test_fp32_bulk_path(vp, O, cfg2):
     cache.close()
 
 
-def test_deterministic_repeat(vp, cfg2):
-    """Two evaluations of the same input are bit-identical (deterministic sorts, no float atomics)."""
-    params = _params(vp, Ns=128)
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+def test_deterministic_repeat(vp, cfg2, T):
+    """Two evaluations of the same input are bit-identical (deterministic sorts, no float atomics;
+    the slab interpolation sums each node's slab parts in a fixed order)."""
+    params = _params(vp, T=T, Ns=128)
     cache = vp.init_cache(params)
     a = vp.velocity_on_nodes(cache, cfg2)
     b = vp.velocity_on_nodes(cache, cfg2)
@@ -580,21 +582,25 @@ def test_device_stepper_errors(vp, cfg1):
 
 
 # ---------------------------------------------------------------------------
-# Transform paths: the fused hand-written FFTs (default, every test above) and rocFFT.
+# Transform paths: the fused hand-written FFTs (default, every test above; on 192, 384
+# and (FP32) 768-cell x-grids with the inverse x transform inside the interpolation), the same FFTs with the interpolation in its own kernel ("separate"),
+# and rocFFT.
 
 @pytest.mark.parametrize("T", [np.float64, np.float32])
 def test_fft_paths_agree(vp, O, cfg2, T):
-    """Velocity and streamfunction from the fused transforms and from rocFFT on cfg2; FP64 also vs the oracle."""
+    """Velocity and streamfunction from the fused transforms, the separate-interpolation variant and
+    rocFFT on cfg2; FP64 also vs the oracle."""
     outs = {}
-    for fft in ("fused", "rocfft"):
+    for fft in ("fused", "separate", "rocfft"):
         params = _params(vp, T=T, Ns=128, backend_long=vp.HIPLongRangeBackend(fft=fft))
         cache, outs[fft] = _gpu(vp, params, cfg2, longrange=True, shortrange=False, LIA=False)
         cache.close()
     tol = 1e-12 if T is np.float64 else 2e-6
     for k in Q:
-        err = rel_l2(outs["fused"][k], outs["rocfft"][k])
-        print(f"  {T.__name__} {k}: fused vs rocFFT {err:.2e}", flush=True)
-        assert err <= tol
+        for fft in ("fused", "separate"):
+            err = rel_l2(outs[fft][k], outs["rocfft"][k])
+            print(f"  {T.__name__} {k}: {fft} vs rocFFT {err:.2e}", flush=True)
+            assert err <= tol
     if T is np.float64:
         params = _params(vp, Ns=128)
         want = O.compute(vp.add_point_charges(cfg2, params), params, Q, LIA=False, shortrange=False)
@@ -634,19 +640,20 @@ def test_rocfft_path_uhat_and_band(vp, O, cfg2):
 
 @pytest.mark.parametrize("Ns", [256, 512])
 def test_fft_paths_agree_static_lengths(vp, Ns):
-    """Grids 384³ and 768³ (compile-time transform plans): fused vs rocFFT long range on a few rings,
-    FP64 to 1e-12 and FP32 to 2e-6."""
+    """Grids 384³ and 768³ (compile-time transform plans): fused and separate-interpolation vs rocFFT
+    long range on a few rings (most slabs without nodes), FP64 to 1e-12 and FP32 to 2e-6."""
     fs = vp.tangles.random_rings(8, 0.5, 1.0, TWO_PI / 128, seed=11)
     for T, tol in ((np.float64, 1e-12), (np.float32, 2e-6)):
         outs = {}
-        for fft in ("fused", "rocfft"):
+        for fft in ("fused", "separate", "rocfft"):
             params = _params(vp, T=T, Ns=Ns, backend_long=vp.HIPLongRangeBackend(fft=fft))
             cache, outs[fft] = _gpu(vp, params, fs, shortrange=False, LIA=False)
             cache.close()
         for k in Q:
-            err = rel_l2(outs["fused"][k], outs["rocfft"][k])
-            print(f"  Ns={Ns} {T.__name__} {k}: fused vs rocFFT {err:.2e}", flush=True)
-            assert err <= tol
+            for fft in ("fused", "separate"):
+                err = rel_l2(outs[fft][k], outs["rocfft"][k])
+                print(f"  Ns={Ns} {T.__name__} {k}: {fft} vs rocFFT {err:.2e}", flush=True)
+                assert err <= tol
 
 
 def test_empty_and_minimal_inputs(vp, O):

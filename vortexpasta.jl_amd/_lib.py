@@ -26,7 +26,7 @@ NO_SHORTRANGE = 16
 NO_LONGRANGE = 32
 NEIGHBOUR_STATS = 64
 KEEP_UHAT = 128
-FFT_FUSED, FFT_ROCFFT = 0, 1  # vpbs_params.fft
+FFT_FUSED, FFT_ROCFFT, FFT_SEPARATE = 0, 1, 2  # vpbs_params.fft
 UHAT_VELOCITY = 1
 UHAT_STREAMFUNCTION = 2
 
@@ -181,7 +181,7 @@ def make_params(params, device: int = 0) -> VpbsParams:
     p.avoid_explicit_erf = int(params.avoid_explicit_erf)
     p.truncate_spherical = int(params.longrange_truncate_spherical)
     p.device = device
-    p.fft = FFT_ROCFFT if params.backend_long.fft == "rocfft" else FFT_FUSED
+    p.fft = {"fused": FFT_FUSED, "rocfft": FFT_ROCFFT, "separate": FFT_SEPARATE}[params.backend_long.fft]
     return p
 
 

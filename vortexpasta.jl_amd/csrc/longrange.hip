@@ -186,6 +186,60 @@ void launch_gather(int64_t Nq, const int32_t *perm, const double *pts, const dou
     }
 }
 
+// Home slab of each node for the interpolation fused with the inverse x transform:
+// the slab (SLAB_HY y-rows × SLAB_HZ z-rows) of its first stencil cell (y, z), the
+// cell gather_nodes stores.
+__global__ void slab_keys(int64_t n, const double *__restrict__ pos, LRGeom g, int32_t *__restrict__ key) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int i0[3];
+#pragma unroll
+    for (int d = 1; d < 3; d++) {
+        const double u = pos[d * n + j] * g.scale[d];
+        i0[d] = wrapi((int)floor(u) - g.M + 1, g.G[d]);
+    }
+    key[j] = (i0[2] / SLAB_HZ) * (g.G[1] / SLAB_HY) + i0[1] / SLAB_HY;
+}
+
+// Slab-sorted copy of the nodes: first stencil cell per direction (wrapped into
+// [0, G_d), component-major) and one record of the W kernel weights per direction,
+// φ((i0_d + k - u_d)/m) as interp_kernel_wave evaluates them, padded with zeros.
+template <typename CT, int W>
+__global__ void gather_nodes(int64_t Np, const int32_t *__restrict__ perm, const double *__restrict__ nodes, LRGeom g,
+                             CT beta, int32_t *__restrict__ si0, CT *__restrict__ rec) {
+    constexpr int M = W / 2, REC = slab_rec_len(W, (int)sizeof(CT));
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= Np) return;
+    const int32_t j = perm[s];
+    CT r[REC];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double u = nodes[d * Np + j] * g.scale[d];
+        const int i0 = (int)floor(u) - M + 1;
+        si0[d * Np + s] = wrapi(i0, g.G[d]);
+#pragma unroll
+        for (int k = 0; k < W; k++) r[d * W + k] = es_kernel<CT>((CT)(((double)(i0 + k) - u) / (double)M), beta);
+    }
+#pragma unroll
+    for (int k = 3 * W; k < REC; k++) r[k] = (CT)0;
+    store_rec(rec + s * REC, r);
+}
+
+template <typename CT>
+void launch_gather_nodes(int64_t Np, const int32_t *perm, const double *nodes, const LRGeom &g, CT beta, int32_t *si0,
+                         CT *rec, hipStream_t s) {
+    const unsigned nb = (unsigned)((Np + 255) / 256);
+    switch (g.M) {
+        case 2: gather_nodes<CT, 4><<<nb, 256, 0, s>>>(Np, perm, nodes, g, beta, si0, rec); break;
+        case 3: gather_nodes<CT, 6><<<nb, 256, 0, s>>>(Np, perm, nodes, g, beta, si0, rec); break;
+        case 4: gather_nodes<CT, 8><<<nb, 256, 0, s>>>(Np, perm, nodes, g, beta, si0, rec); break;
+        case 5: gather_nodes<CT, 10><<<nb, 256, 0, s>>>(Np, perm, nodes, g, beta, si0, rec); break;
+        case 6: gather_nodes<CT, 12><<<nb, 256, 0, s>>>(Np, perm, nodes, g, beta, si0, rec); break;
+        case 7: gather_nodes<CT, 14><<<nb, 256, 0, s>>>(Np, perm, nodes, g, beta, si0, rec); break;
+        default: gather_nodes<CT, 16><<<nb, 256, 0, s>>>(Np, perm, nodes, g, beta, si0, rec); break;
+    }
+}
+
 // ----------------------------------------------------------------------------
 // Spreading (type 1), gather form.
 
@@ -954,6 +1008,27 @@ int inverse_to_grid(vpbs_ctx *c, Cx<CT> *zp) {
     return VPBS_OK;
 }
 
+// Node buffers of the slab interpolation, grown with Np.
+int slab_node_buffers(vpbs_ctx *c) {
+    const int64_t Np = c->Np;
+    if (Np <= c->cap_nd && c->nd_key) return VPBS_OK;
+    void **bufs[] = {(void **)&c->nd_key, (void **)&c->nd_perm, (void **)&c->nd_i0, &c->nd_rec, &c->slab_part};
+    for (void **b : bufs)
+        if (*b) {
+            VPBS_HIP(c, hipFree(*b));
+            *b = nullptr;
+        }
+    const size_t es = c->p.precision == VPBS_F32 ? sizeof(float) : sizeof(double);
+    const size_t rec = (size_t)slab_rec_len(2 * c->p.nufft_m, (int)es);
+    VPBS_HIP(c, hipMalloc((void **)&c->nd_key, sizeof(int32_t) * (size_t)Np));
+    VPBS_HIP(c, hipMalloc((void **)&c->nd_perm, sizeof(int32_t) * (size_t)Np));
+    VPBS_HIP(c, hipMalloc((void **)&c->nd_i0, sizeof(int32_t) * 3 * (size_t)Np));
+    VPBS_HIP(c, hipMalloc(&c->nd_rec, es * rec * (size_t)Np));
+    VPBS_HIP(c, hipMalloc(&c->slab_part, es * 3 * (size_t)spectral_slab_slots(c) * (size_t)Np));
+    c->cap_nd = Np;
+    return VPBS_OK;
+}
+
 template <typename CT>
 int launch_lr(vpbs_ctx *c, uint32_t flags) {
     hipStream_t s = c->s_lr;
@@ -964,8 +1039,14 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
 
     stage_begin(c, ST_LRSORT, s);
     lr_tile_keys<<<nblocks(Nq, 256), 256, 0, s>>>(Nq, c->points, g, c->lr_key);
-    VPBS_TRY(counting_sort(c, c->sort_lr, c->lr_key, Nq, c->ntiles, c->lr_start, c->lr_perm, s));
+    VPBS_TRY(radix_sort_bins(c, c->sort_lr, c->lr_key, Nq, c->ntiles, c->lr_start, c->lr_perm, s));
     launch_gather<CT>(Nq, c->lr_perm, c->points, c->charges, g, beta, c->lr_i0, (CT *)c->lr_w, s);
+    if (c->slab_interp && Np > 0) {
+        VPBS_TRY(slab_node_buffers(c));
+        slab_keys<<<nblocks(Np, 256), 256, 0, s>>>(Np, c->nodes, g, c->nd_key);
+        VPBS_TRY(radix_sort_bins(c, c->sort_nd, c->nd_key, Np, c->nslabs, c->nd_start, c->nd_perm, s));
+        launch_gather_nodes<CT>(Np, c->nd_perm, c->nodes, g, beta, c->nd_i0, (CT *)c->nd_rec, s);
+    }
     stage_end(c, ST_LRSORT, s);
 
     CT *grid = (CT *)c->grid;
@@ -983,14 +1064,26 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
         stage_begin(c, ST_FOURIER, s);
         VPBS_TRY(spectral_fourier(c, flags));
         stage_end(c, ST_FOURIER, s);
+        // inverse transforms to the nodes: with slab interpolation the inverse x transform
+        // runs inside the interpolation kernel (the real grid is not written)
         stage_begin(c, ST_FFTB, s);
-        if (vel) VPBS_TRY(spectral_inverse(c, c->zpen));
+        if (vel) VPBS_TRY(c->slab_interp ? spectral_y_inverse(c, c->zpen) : spectral_inverse(c, c->zpen));
         stage_end(c, ST_FFTB, s);
         stage_begin(c, ST_INTERP, s);
-        if (vel) launch_interp<CT>(g, Np, c->nodes, beta, grid, c->v_lr, s);
+        if (vel) {
+            if (c->slab_interp)
+                VPBS_TRY(spectral_x_inverse_interp(c, c->v_lr));
+            else
+                launch_interp<CT>(g, Np, c->nodes, beta, grid, c->v_lr, s);
+        }
         if (psi) {
-            VPBS_TRY(spectral_inverse(c, c->zpen2));
-            launch_interp<CT>(g, Np, c->nodes, beta, grid, c->psi_lr, s);
+            if (c->slab_interp) {
+                VPBS_TRY(spectral_y_inverse(c, c->zpen2));
+                VPBS_TRY(spectral_x_inverse_interp(c, c->psi_lr));
+            } else {
+                VPBS_TRY(spectral_inverse(c, c->zpen2));
+                launch_interp<CT>(g, Np, c->nodes, beta, grid, c->psi_lr, s);
+            }
         }
         stage_end(c, ST_INTERP, s);
         VPBS_HIP(c, hipGetLastError());
@@ -1072,6 +1165,7 @@ int longrange_setup(vpbs_ctx *c) {
         c->ntile[d] = c->G[d] / (d == 0 ? tx : (d == 1 ? TY : TZ));
     }
     c->ntiles = (int64_t)c->ntile[0] * c->ntile[1] * c->ntile[2];
+    c->nslabs = (int64_t)(c->G[1] / SLAB_HY) * (c->G[2] / SLAB_HZ);
     c->comp_stride = (int64_t)c->G[0] * c->G[1] * c->G[2];
     c->es_beta = es_beta(m, c->p.nufft_sigma);
     // retained modes (longrange.jl:556-583)
@@ -1101,9 +1195,11 @@ int longrange_setup(vpbs_ctx *c) {
     VPBS_HIP(c, hipMalloc(&c->ypen, 2 * es * 3 * (size_t)c->ypen_stride));
     VPBS_HIP(c, hipMalloc(&c->zpen, 2 * es * 3 * (size_t)c->zpen_stride));
 
-    c->fft_fused = c->p.fft == VPBS_FFT_FUSED && spectral_supported(c);
+    c->fft_fused = (c->p.fft == VPBS_FFT_FUSED || c->p.fft == VPBS_FFT_SEPARATE) && spectral_supported(c);
+    c->slab_interp = false;
     if (c->fft_fused) {
         VPBS_TRY(spectral_setup(c));
+        c->slab_interp = spectral_slab_supported(c);
         c->plans_ready = true;
         return VPBS_OK;
     }

@@ -1,4 +1,4 @@
-// Deterministic counting sort of integer keys (cell or tile indices).
+// Deterministic sorts of integer keys (cell, tile or slab indices).
 //
 // Replaces the reference's atomic linked-list cell build (CellLists/gpu.jl:28-61,
 // whose intra-cell order depends on thread timing) and its target counting sort
@@ -9,7 +9,13 @@
 //   2. exclusive scan of the histogram (multi-level, 1024 elements per block)
 //   3. scatter to start[key] + rank
 //   4. one thread per bin sorts its (short) segment by original index
+//
+// radix_sort_bins gives the same order with a stable LSD radix sort of the keys
+// (original indices as values) and derives the bin starts from the sorted keys;
+// it suits bins too large for step 4.
 #include "vpbs_internal.hpp"
+
+#include <hipcub/hipcub.hpp>
 
 namespace vpbs {
 
@@ -95,6 +101,22 @@ __global__ void sort_segments(const int32_t *__restrict__ start, int64_t nbins, 
     }
 }
 
+__global__ void iota_i32(int32_t *__restrict__ a, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (int32_t)i;
+}
+
+// start[b] for every bin b ∈ [0, nbins] from the sorted keys: thread i ∈ [0, n] sets
+// the bins in (key[i-1], key[i]] to i (key[-1] = -1, key[n] = nbins), so each bin
+// is written exactly once.
+__global__ void bin_starts(const int32_t *__restrict__ skeys, int64_t n, int64_t nbins, int32_t *__restrict__ start) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    const int64_t lo = i > 0 ? skeys[i - 1] : -1;
+    const int64_t hi = i < n ? skeys[i] : nbins;
+    for (int64_t b = lo + 1; b <= hi; b++) start[b] = (int32_t)i;
+}
+
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // Storage needed by the multi-level scan of n elements.
@@ -128,6 +150,26 @@ int counting_sort(vpbs_ctx *c, SortBuffers &sb, const int32_t *keys, int64_t n, 
     exclusive_scan(start, nbins + 1, sb.scan_tmp, s);
     if (n > 0) scatter_ranked<<<nblocks(n, 256), 256, 0, s>>>(keys, sb.rank, n, start, perm);
     sort_segments<<<nblocks(nbins, 256), 256, 0, s>>>(start, nbins, perm);
+    VPBS_HIP(c, hipGetLastError());
+    return VPBS_OK;
+}
+
+int radix_sort_bins(vpbs_ctx *c, SortBuffers &sb, const int32_t *keys, int64_t n, int64_t nbins, int32_t *start,
+                    int32_t *perm, hipStream_t s) {
+    if (n > INT32_MAX || nbins >= INT32_MAX) return fail(c, VPBS_ERR_INVALID, "radix sort: too many elements or bins");
+    int bits = 1;
+    while (bits < 31 && ((int64_t)1 << bits) < nbins) bits++;
+    if (n > 0) {
+        VPBS_TRY(grow(c, &sb.keys_out, &sb.cap_radix, 2 * n));  // sorted keys, then the original indices
+        int32_t *iota = sb.keys_out + n;
+        size_t tmp = 0;
+        VPBS_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, sb.keys_out, iota, perm, (int)n, 0, bits, s));
+        VPBS_TRY(grow(c, &sb.radix_tmp, &sb.cap_radix_tmp, (int64_t)tmp));
+        iota_i32<<<nblocks(n, 256), 256, 0, s>>>(iota, n);
+        tmp = (size_t)sb.cap_radix_tmp;
+        VPBS_HIP(c, hipcub::DeviceRadixSort::SortPairs(sb.radix_tmp, tmp, keys, sb.keys_out, iota, perm, (int)n, 0, bits, s));
+    }
+    bin_starts<<<nblocks(n + 1, 256), 256, 0, s>>>(sb.keys_out, n, nbins, start);
     VPBS_HIP(c, hipGetLastError());
     return VPBS_OK;
 }

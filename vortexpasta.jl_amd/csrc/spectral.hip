@@ -347,12 +347,14 @@ __device__ void lds_fft(int s_off, int lgB, int ld, const AxisPlan &ax, int tw_o
 // The same pass with the length, tile width and stride known at compile time (the
 // grid sizes of the common configurations): strides fold into immediate offsets
 // and the index arithmetic into shifts and constant multiplies.
-template <typename CT, int R, int S, int NT, int L, int BW, int NS>
-__device__ __forceinline__ void stockham_static(int s_off, int tw_off) {
+// LD: row stride of the tile (≥ BW); TWG: twiddles read from the global table twg
+// instead of the LDS copy at tw_off.
+template <typename CT, int R, int S, int NT, int L, int BW, int NS, int LD = BW, bool TWG = false>
+__device__ __forceinline__ void stockham_static(int s_off, int tw_off, const C2<CT> *__restrict__ twg = nullptr) {
     constexpr int NB = L / R, NWORK = NB * BW, TWS = L / (NS * R);
     constexpr int ITEMS = (NWORK + NT - 1) / NT;
     C2<CT> *Sm = lds<CT>(s_off);
-    const C2<CT> *tw = lds<CT>(tw_off);
+    const C2<CT> *tw = TWG ? twg : lds<CT>(tw_off);
     C2<CT> v[ITEMS][R];
 #pragma unroll
     for (int i = 0; i < ITEMS; i++) {
@@ -360,7 +362,7 @@ __device__ __forceinline__ void stockham_static(int s_off, int tw_off) {
         if (NWORK % NT == 0 || w < NWORK) {
             const int b = w % BW, j = w / BW;
 #pragma unroll
-            for (int k = 0; k < R; k++) v[i][k] = Sm[(j + k * NB) * BW + b];
+            for (int k = 0; k < R; k++) v[i][k] = Sm[(j + k * NB) * LD + b];
         }
     }
     __syncthreads();
@@ -382,20 +384,20 @@ __device__ __forceinline__ void stockham_static(int s_off, int tw_off) {
             dft<R, S>(v[i]);
             const int base = q * NS * R + jm;
 #pragma unroll
-            for (int k = 0; k < R; k++) Sm[(base + k * NS) * BW + b] = v[i][k];
+            for (int k = 0; k < R; k++) Sm[(base + k * NS) * LD + b] = v[i][k];
         }
     }
     __syncthreads();
 }
 
-// All passes of a compile-time plan on a tile of BW columns (row stride BW).
-template <typename CT, int S, int NT, int L, int BW, int P = 0, int NS = 1>
-__device__ __forceinline__ void fft_static(int s_off, int tw_off) {
+// All passes of a compile-time plan on a tile of BW columns (row stride LD).
+template <typename CT, int S, int NT, int L, int BW, int P = 0, int NS = 1, int LD = BW, bool TWG = false>
+__device__ __forceinline__ void fft_static(int s_off, int tw_off, const C2<CT> *__restrict__ twg = nullptr) {
     constexpr AxisPlan rd = plan_axis(L);
     if constexpr (P < rd.npass) {
         constexpr int R = rd.radix[P];
-        stockham_static<CT, R, S, NT, L, BW, NS>(s_off, tw_off);
-        fft_static<CT, S, NT, L, BW, P + 1, NS * R>(s_off, tw_off);
+        stockham_static<CT, R, S, NT, L, BW, NS, LD, TWG>(s_off, tw_off, twg);
+        fft_static<CT, S, NT, L, BW, P + 1, NS * R, LD, TWG>(s_off, tw_off, twg);
     }
 }
 
@@ -1030,6 +1032,224 @@ __global__ __launch_bounds__(NTX) void spec_x_inverse(SpecGeom g, const C2<CT> *
 }
 
 // ---------------------------------------------------------------------------
+// Inverse x transform fused with the interpolation (type 2).
+//
+// Slab = SLAB_HY y-rows × SLAB_HZ z-rows × all G0 x-cells of the 3 components,
+// held in LDS as SLAB_BW complex columns b = (c·SLAB_HZ + zz)·SLAB_HP + pp (pair
+// row pp of z-row zz, component c) at S[x·SLAB_LD + b]. The block rebuilds the
+// columns from x1 as spec_x_inverse does and transforms them: S then holds the
+// grid values of the slab rows (real part: even y, imaginary part: odd y). Every
+// node whose W³ stencil reaches the slab has its first stencil cell in one of
+// NYH × NZH home slabs (nodes come sorted by home slab); eight lanes per node,
+// one per slab row, sum their row's W cells with the x-weights, scale by the
+// row's y- and z-weights, and a butterfly over the eight gives the part of the
+// node's stencil sum that lies in this slab. It goes to slot q of the node, q =
+// the home-slab range it was found in (one slab per slot); slab_reduce adds the
+// slots in a fixed order. No grid written, no atomics: deterministic.
+constexpr int SLAB_HP = SLAB_HY / 2;
+constexpr int SLAB_ROWS = SLAB_HY * SLAB_HZ;
+constexpr int SLAB_NT = 512;
+constexpr int SLAB_BW = 3 * SLAB_HZ * SLAB_HP;
+constexpr int SLAB_LD = SLAB_BW + 1;  // odd row stride: the x-cells of a slab row spread over the LDS banks
+static_assert(SLAB_ROWS == 8 && SLAB_HY % 2 == 0, "eight slab rows of whole pair rows: eight lanes per node");
+
+// Home slabs along y (z) of the nodes whose W-cell stencil reaches a slab.
+constexpr int slab_nyh(int W) { return 1 + (W - 1 + SLAB_HY - 1) / SLAB_HY; }
+constexpr int slab_nzh(int W) { return 1 + (W - 1 + SLAB_HZ - 1) / SLAB_HZ; }
+// Node record: W x-weights, W y-weights, W z-weights, padded to whole 16-byte vectors.
+template <typename CT>
+constexpr int slab_rec(int W) {
+    return slab_rec_len(W, (int)sizeof(CT));
+}
+
+// i in (-n, 2n) -> i mod n
+__device__ __forceinline__ int wrap1(int i, int n) {
+    i += (i < 0) ? n : 0;
+    i -= (i >= n) ? n : 0;
+    return i;
+}
+
+struct SlabArgs {
+    int G[3];
+    int nkx;
+    int64_t Np;
+    const int32_t *start;  // start[sz·nsy + sy]: first node of home slab (sy, sz); start[nsy·nsz] = Np
+    const int32_t *si0;    // [3][Np] first stencil cell per direction of the slab-sorted nodes, in [0, G_d)
+    const void *rec;       // [Np][slab_rec(W)] node records (compute type)
+    void *part;            // [NYH·NZH][3][Np] stencil sums per slot (compute type)
+};
+
+template <typename CT, int W, int LS>
+__global__ __launch_bounds__(SLAB_NT) void x_inverse_interp(SlabArgs a, const C2<CT> *__restrict__ x1,
+                                                            const C2<CT> *__restrict__ tw) {
+    constexpr int REC = slab_rec<CT>(W);
+    constexpr int NYH = slab_nyh(W), NZH = slab_nzh(W), NRG = NYH * NZH;
+    static_assert(NRG <= 64, "one lane per home slab");
+    constexpr int CST = 2 * SLAB_HZ * SLAB_HP;  // values between the columns of consecutive components
+    const int G1 = a.G[1], G2 = a.G[2];
+    const int nsy = G1 / SLAB_HY, nsz = G2 / SLAB_HZ;
+    const int64_t nslab = (int64_t)nsy * nsz;
+    const int64_t sl = xcd_remap(nslab);
+    if (sl >= nslab) return;
+    // slabs go in 8×8 blocks (ty fastest): the slabs in flight on one XCD share x1 rows and nodes
+    const int blk = (int)(sl >> 6), in = (int)(sl & 63), nby = nsy >> 3;
+    const int ty = ((blk % nby) << 3) + (in & 7), tz = ((blk / nby) << 3) + (in >> 3);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+    // nodes: the home-slab ranges, concatenated (lane r < NRG: range r)
+    int s0 = 0, len = 0;
+    if (lane < NRG) {
+        const int sy = wrap1(ty - (NYH - 1) + lane % NYH, nsy);
+        const int sz = wrap1(tz - (NZH - 1) + lane / NYH, nsz);
+        const int key = sz * nsy + sy;
+        s0 = a.start[key];
+        len = a.start[key + 1] - s0;
+    }
+    int incl = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    const int ncand = __shfl(incl, NRG - 1, 64);  // the same in every wavefront
+    if (ncand == 0) return;
+    // candidate cnd of range q is node off[q] + cnd (cnd ≥ pre[q]); wavefront-uniform
+    const int excl = incl - len;
+    int pre[NRG], off[NRG];
+#pragma unroll
+    for (int q = 0; q < NRG; q++) {
+        pre[q] = __builtin_amdgcn_readlane(excl, q);
+        off[q] = __builtin_amdgcn_readlane(s0 - excl, q);
+    }
+
+    // columns: Z_k = A_k + i B_k (k < nkx; DC real) and conj(A_k) + i conj(B_k) at L - k
+    C2<CT> *Sm = lds<CT>(0);
+    const int nkx = a.nkx;
+    auto row_of = [&](int b) {  // x1 row of the even y of column b
+        const int c = b / (SLAB_HZ * SLAB_HP), rr = b - c * (SLAB_HZ * SLAB_HP);
+        const int zz = rr / SLAB_HP, pp = rr - zz * SLAB_HP;
+        return ((int64_t)c * G2 + tz * SLAB_HZ + zz) * G1 + ty * SLAB_HY + 2 * pp;
+    };
+    auto put = [&](int k, int b, C2<CT> A, C2<CT> Bv) {
+        if (k == 0) A.im = Bv.im = (CT)0;
+        Sm[k * SLAB_LD + b] = C2<CT>{A.re - Bv.im, A.im + Bv.re};
+        if (k > 0) Sm[(LS - k) * SLAB_LD + b] = C2<CT>{A.re + Bv.im, Bv.re - A.im};
+    };
+    if (use_pairs<CT>() && !(nkx & 1)) {
+        const int h = nkx >> 1;  // pairs (k, k+1) per row: 16-byte loads
+        for (int e = threadIdx.x; e < h * SLAB_BW; e += SLAB_NT) {
+            const int b = e / h, k = (e - b * h) * 2;
+            const C2<CT> *src = x1 + row_of(b) * nkx + k;
+            const C2x2<CT> A = load_pair(src), Bv = load_pair(src + nkx);
+            put(k, b, A.a, Bv.a);
+            put(k + 1, b, A.b, Bv.b);
+        }
+    } else {
+        for (int e = threadIdx.x; e < nkx * SLAB_BW; e += SLAB_NT) {
+            const int b = e / nkx, k = e - b * nkx;
+            const C2<CT> *src = x1 + row_of(b) * nkx + k;
+            put(k, b, src[0], src[nkx]);
+        }
+    }
+    // modes kx ∈ [nkx, L - nkx] are not retained
+    for (int e = threadIdx.x; e < (LS - 2 * nkx + 1) * SLAB_BW; e += SLAB_NT) {
+        const int k = nkx + e / SLAB_BW, b = e - (e / SLAB_BW) * SLAB_BW;
+        Sm[k * SLAB_LD + b] = C2<CT>{(CT)0, (CT)0};
+    }
+    __syncthreads();
+    fft_static<CT, 1, SLAB_NT, LS, SLAB_BW, 0, 1, SLAB_LD, true>(0, 0, tw);
+
+    // stencil sums: lane (j, r) = node j of the group, slab row r
+    const CT *Sf = reinterpret_cast<const CT *>(spec_smem);
+    const int j = lane >> 3, r = lane & 7;
+    const int yy = r % SLAB_HY, zz = r / SLAB_HY;
+    const int y = ty * SLAB_HY + yy, z = tz * SLAB_HZ + zz;
+    const CT *row = Sf + (zz * SLAB_HP + (yy >> 1)) * 2 + (yy & 1);  // + 2·SLAB_LD·x + CST·c
+    const int64_t Np = a.Np;
+    const int32_t *si0 = a.si0;
+    const CT *rec = static_cast<const CT *>(a.rec);
+    CT *part = static_cast<CT *>(a.part);
+    for (int g = wave; g * 8 < ncand; g += SLAB_NT / 64) {
+        const int cnd = g * 8 + j;
+        int q = 0, sidx = 0;
+        CT v0 = (CT)0, v1 = (CT)0, v2 = (CT)0;
+        if (cnd < ncand) {
+            int o = off[0];
+#pragma unroll
+            for (int t = 1; t < NRG; t++) {
+                const bool ge = cnd >= pre[t];
+                o = ge ? off[t] : o;
+                q = ge ? t : q;
+            }
+            sidx = cnd + o;
+            const int ix = si0[sidx], iy = si0[Np + sidx], iz = si0[2 * Np + sidx];
+            CT w[REC];
+            const CT *rp = rec + (int64_t)sidx * REC;
+            if constexpr (sizeof(CT) == 4) {
+#pragma unroll
+                for (int k = 0; k < REC / 4; k++) {
+                    const float4 v = reinterpret_cast<const float4 *>(rp)[k];
+                    w[4 * k] = v.x, w[4 * k + 1] = v.y, w[4 * k + 2] = v.z, w[4 * k + 3] = v.w;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < REC / 2; k++) {
+                    const double2 v = reinterpret_cast<const double2 *>(rp)[k];
+                    w[2 * k] = v.x, w[2 * k + 1] = v.y;
+                }
+            }
+            const int ky = wrap1(y - iy, G1), kz = wrap1(z - iz, G2);
+            if (ky < W && kz < W) {
+                CT wy = w[W], wz = w[2 * W];
+#pragma unroll
+                for (int k = 1; k < W; k++) {
+                    wy = ky == k ? w[W + k] : wy;
+                    wz = kz == k ? w[2 * W + k] : wz;
+                }
+                int x = ix;
+#pragma unroll
+                for (int k = 0; k < W; k++) {
+                    const CT *p = row + x * (2 * SLAB_LD);
+                    v0 += w[k] * p[0];
+                    v1 += w[k] * p[CST];
+                    v2 += w[k] * p[2 * CST];
+                    x = x + 1 == LS ? 0 : x + 1;
+                }
+                const CT wyz = wy * wz;
+                v0 *= wyz, v1 *= wyz, v2 *= wyz;
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < SLAB_ROWS; o <<= 1) {
+            v0 += __shfl_xor(v0, o, 64);
+            v1 += __shfl_xor(v1, o, 64);
+            v2 += __shfl_xor(v2, o, 64);
+        }
+        if (cnd < ncand && r == 0) {
+            CT *dst = part + (int64_t)q * 3 * Np + sidx;
+            dst[0] = v0;
+            dst[Np] = v1;
+            dst[2 * Np] = v2;
+        }
+    }
+}
+
+// out[c][perm[s]] = Σ_q part[q][c][s] over the nslot slots, in slot order.
+template <typename CT>
+__global__ void slab_reduce(int64_t Np, int nslot, const CT *__restrict__ part, const int32_t *__restrict__ perm,
+                            double *__restrict__ out) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= Np) return;
+    const int64_t i = perm[s];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double acc = 0;
+        for (int q = 0; q < nslot; q++) acc += (double)part[((int64_t)q * 3 + c) * Np + s];
+        out[c * Np + i] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 
 template <typename CT>
@@ -1099,15 +1319,89 @@ SpecFourier spec_fourier_args(const vpbs_ctx *c) {
     } while (0)
 
 template <typename CT>
+int forward_y_impl(vpbs_ctx *c) {
+    hipStream_t s = c->s_lr;
+    const SpecGeom g = spec_geom<CT>(c);
+    const int64_t nkb = (c->nkx + (1 << g.lgB[1]) - 1) >> g.lgB[1];
+    SPEC_LAUNCH(c->G[1], spec_y_forward, grid8(3 * (int64_t)c->G[2] * nkb), NTY, spec_lds<CT>(g, 1),
+                g, (const C2<CT> *)c->ypen, (C2<CT> *)c->zpen, twiddles<CT>(c, 1));
+    VPBS_HIP(c, hipGetLastError());
+    return VPBS_OK;
+}
+
+template <typename CT>
 int forward_impl(vpbs_ctx *c) {
     hipStream_t s = c->s_lr;
     const SpecGeom g = spec_geom<CT>(c);
     const int64_t nrows = 3 * (int64_t)c->G[2] * (c->G[1] / 2);
-    const int64_t nkb = (c->nkx + (1 << g.lgB[1]) - 1) >> g.lgB[1];
     SPEC_LAUNCH(c->G[0], spec_x_forward, grid8(nrows >> g.lgB[0]), NTX, spec_lds<CT>(g, 0),
                 g, (const C2<CT> *)c->grid, (C2<CT> *)c->ypen, twiddles<CT>(c, 0));
-    SPEC_LAUNCH(c->G[1], spec_y_forward, grid8(3 * (int64_t)c->G[2] * nkb), NTY, spec_lds<CT>(g, 1),
-                g, (const C2<CT> *)c->ypen, (C2<CT> *)c->zpen, twiddles<CT>(c, 1));
+    VPBS_HIP(c, hipGetLastError());
+    return forward_y_impl<CT>(c);
+}
+
+// Slab kernels: stencil width W (2m), x-grid length L (compile-time plans).
+template <typename CT, int W>
+const void *slab_kernel_w(int L) {
+    switch (L) {
+        case 192: return (const void *)x_inverse_interp<CT, W, 192>;
+        case 384: return (const void *)x_inverse_interp<CT, W, 384>;
+        case 768: return (const void *)x_inverse_interp<CT, W, 768>;
+        default: return nullptr;  // 1536 and more: one block per CU at most (see spectral_slab_supported)
+    }
+}
+
+template <typename CT>
+const void *slab_kernel(int W, int L) {
+    switch (W) {
+        case 4: return slab_kernel_w<CT, 4>(L);
+        case 6: return slab_kernel_w<CT, 6>(L);
+        case 8: return slab_kernel_w<CT, 8>(L);
+        case 10: return slab_kernel_w<CT, 10>(L);
+        case 12: return slab_kernel_w<CT, 12>(L);
+        case 14: return slab_kernel_w<CT, 14>(L);
+        case 16: return slab_kernel_w<CT, 16>(L);
+        default: return nullptr;
+    }
+}
+
+template <typename CT>
+size_t slab_lds(int L) {
+    return sizeof(C2<CT>) * (size_t)L * SLAB_LD;
+}
+
+template <typename CT>
+int slab_interp_impl(vpbs_ctx *c, double *out) {
+    if (!c->slab_interp) return fail(c, VPBS_ERR_STATE, "slab interpolation is not set up for this grid");
+    const int W = 2 * c->p.nufft_m;
+    const void *k = slab_kernel<CT>(W, c->G[0]);
+    if (c->Np == 0) return VPBS_OK;
+    SlabArgs a{};
+    for (int d = 0; d < 3; d++) a.G[d] = c->G[d];
+    a.nkx = c->nkx;
+    a.Np = c->Np;
+    a.start = c->nd_start;
+    a.si0 = c->nd_i0;
+    a.rec = c->nd_rec;
+    a.part = c->slab_part;
+    const C2<CT> *x1 = (const C2<CT> *)c->ypen;
+    const C2<CT> *tw = twiddles<CT>(c, 0);
+    void *args[] = {(void *)&a, (void *)&x1, (void *)&tw};
+    VPBS_HIP(c, hipLaunchKernel(k, dim3(grid8(c->nslabs)), dim3(SLAB_NT), args, slab_lds<CT>(c->G[0]), c->s_lr));
+    const int nslot = slab_nyh(W) * slab_nzh(W);
+    slab_reduce<CT><<<(unsigned)((c->Np + 255) / 256), 256, 0, c->s_lr>>>(c->Np, nslot, (const CT *)c->slab_part,
+                                                                          c->nd_perm, out);
+    VPBS_HIP(c, hipGetLastError());
+    return VPBS_OK;
+}
+
+template <typename CT>
+int inverse_y_impl(vpbs_ctx *c, void *zp) {
+    hipStream_t s = c->s_lr;
+    const SpecGeom g = spec_geom<CT>(c);
+    const int64_t nkb = (c->nkx + (1 << g.lgB[1]) - 1) >> g.lgB[1];
+    SPEC_LAUNCH(c->G[1], spec_y_inverse, grid8(3 * (int64_t)c->G[2] * nkb), NTY, spec_lds<CT>(g, 1),
+                g, (const C2<CT> *)zp, (C2<CT> *)c->ypen, twiddles<CT>(c, 1));
     VPBS_HIP(c, hipGetLastError());
     return VPBS_OK;
 }
@@ -1197,6 +1491,7 @@ int setup_impl(vpbs_ctx *c) {
     VPBS_TRY(attrs(std::integral_constant<int, 384>{}));
     VPBS_TRY(attrs(std::integral_constant<int, 768>{}));
     VPBS_TRY(attrs(std::integral_constant<int, 1536>{}));
+    if (spectral_slab_supported(c)) VPBS_HIP(c, attr(slab_kernel<CT>(2 * c->p.nufft_m, c->G[0]), slab_lds<CT>(c->G[0])));
     return VPBS_OK;
 }
 
@@ -1234,6 +1529,31 @@ int spectral_setup(vpbs_ctx *c) {
 int spectral_forward(vpbs_ctx *c) {
     return c->p.precision == VPBS_F32 ? forward_impl<float>(c) : forward_impl<double>(c);
 }
+
+bool spectral_slab_supported(const vpbs_ctx *c) {
+    if (!c->fft_fused || c->p.fft != VPBS_FFT_FUSED) return false;
+    const bool f32 = c->p.precision == VPBS_F32;
+    const int W = 2 * c->p.nufft_m;
+    if (!(f32 ? slab_kernel<float>(W, c->G[0]) : slab_kernel<double>(W, c->G[0]))) return false;
+    // whole 8×8 blocks of slabs, and no home slab counted twice
+    if (c->G[1] % (8 * SLAB_HY) || c->G[2] % (8 * SLAB_HZ)) return false;
+    if (c->G[1] / SLAB_HY < slab_nyh(W) || c->G[2] / SLAB_HZ < slab_nzh(W)) return false;
+    // two blocks per CU: one loads x1 while the other transforms and interpolates (with one
+    // block per CU, FP64 on 768 cells, the separate kernels are faster)
+    return 2 * (f32 ? slab_lds<float>(c->G[0]) : slab_lds<double>(c->G[0])) <= c->spec_lds_max;
+}
+
+int spectral_slab_slots(const vpbs_ctx *c) { return slab_nyh(2 * c->p.nufft_m) * slab_nzh(2 * c->p.nufft_m); }
+
+int spectral_y_inverse(vpbs_ctx *c, void *zp) {
+    return c->p.precision == VPBS_F32 ? inverse_y_impl<float>(c, zp) : inverse_y_impl<double>(c, zp);
+}
+
+int spectral_x_inverse_interp(vpbs_ctx *c, double *out) {
+    return c->p.precision == VPBS_F32 ? slab_interp_impl<float>(c, out) : slab_interp_impl<double>(c, out);
+}
+
+
 
 int spectral_fourier(vpbs_ctx *c, uint32_t flags) {
     return c->p.precision == VPBS_F32 ? fourier_impl<float>(c, flags) : fourier_impl<double>(c, flags);

@@ -81,8 +81,8 @@ static int validate(const vpbs_params *p, std::string &msg) {
         msg = "nufft_sigma must be > 1";
         return VPBS_ERR_INVALID;
     }
-    if (p->fft != VPBS_FFT_FUSED && p->fft != VPBS_FFT_ROCFFT) {
-        msg = "fft must be VPBS_FFT_FUSED (0) or VPBS_FFT_ROCFFT (1)";
+    if (p->fft != VPBS_FFT_FUSED && p->fft != VPBS_FFT_ROCFFT && p->fft != VPBS_FFT_SEPARATE) {
+        msg = "fft must be VPBS_FFT_FUSED (0), VPBS_FFT_ROCFFT (1) or VPBS_FFT_SEPARATE (2)";
         return VPBS_ERR_INVALID;
     }
     const double Lmin = std::fmin(p->Ls[0], std::fmin(p->Ls[1], p->Ls[2]));
@@ -115,7 +115,10 @@ static void release(vpbs_ctx *c) {
                     c->tgt_start, c->tgt_perm,  c->src_pos,  c->src_chg,   c->src_idx,    c->stat_counts,
                     c->stat_hashes, c->lr_key,  c->lr_start, c->lr_active, c->lr_perm,   c->lr_i0,      c->lr_w,
                     c->sort_sr.rank, c->sort_sr.scan_tmp, c->sort_tg.rank, c->sort_tg.scan_tmp,
-                    c->sort_lr.rank, c->sort_lr.scan_tmp, c->fil_x, c->fil_ts, c->fil_c, c->fil_d, c->fil_e,
+                    c->sort_lr.rank, c->sort_lr.scan_tmp, c->sort_lr.keys_out, c->sort_lr.radix_tmp,
+                    c->sort_nd.rank, c->sort_nd.scan_tmp, c->sort_nd.keys_out, c->sort_nd.radix_tmp,
+                    c->nd_key,    c->nd_start,  c->nd_perm,  c->nd_i0,    c->nd_rec,     c->slab_part,
+                    c->fil_x, c->fil_ts, c->fil_c, c->fil_d, c->fil_e,
                     c->fil_fid, c->fil_n, c->fil_off, c->fil_T, c->fil_xoff, c->step_buf, c->solve_buf};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -329,6 +332,7 @@ int vpbs_create(vpbs_ctx **out, const vpbs_params *p) {
     rc = longrange_setup(c);
     if (rc != VPBS_OK) return bail(rc);
     if (hipMalloc((void **)&c->lr_start, sizeof(int32_t) * (size_t)(c->ntiles + 1)) != hipSuccess ||
+        (c->slab_interp && hipMalloc((void **)&c->nd_start, sizeof(int32_t) * (size_t)(c->nslabs + 1)) != hipSuccess) ||
         hipMalloc((void **)&c->lr_active, (size_t)c->ntiles) != hipSuccess) {
         fail(c, VPBS_ERR_HIP, "tile allocation failed");
         return bail(VPBS_ERR_HIP);

@@ -18,13 +18,25 @@ struct Box {
     double Lh[3];
 };
 
-// Deterministic counting sort scratch (grow-only device buffers).
+// Deterministic sort scratch (grow-only device buffers).
 struct SortBuffers {
     int32_t *rank = nullptr;   // [n] rank within bin from the histogram pass
     int64_t cap_n = 0;
     int32_t *scan_tmp = nullptr;  // block sums for the multi-level scan
     int64_t cap_tmp = 0;
+    // radix sort: sorted keys, then the original indices ([2n]); library scratch (bytes)
+    int32_t *keys_out = nullptr;
+    int64_t cap_radix = 0;
+    unsigned char *radix_tmp = nullptr;
+    int64_t cap_radix_tmp = 0;
 };
+
+// Slabs of the interpolation fused with the inverse x transform: SLAB_HY y-rows ×
+// SLAB_HZ z-rows × all x-cells (spectral.hip; nodes sorted by slab in longrange.hip).
+constexpr int SLAB_HY = 4, SLAB_HZ = 2;
+// Node record of that interpolation: the W kernel weights per direction, padded to
+// whole 16-byte vectors (elem = bytes per value).
+constexpr int slab_rec_len(int W, int elem) { return elem == 4 ? (3 * W + 3) & ~3 : (3 * W + 1) & ~1; }
 
 // Per-stage event pairs.
 enum Stage {
@@ -100,13 +112,22 @@ struct vpbs_ctx {
     int64_t *stat_counts = nullptr;
     uint64_t *stat_hashes = nullptr;
     bool have_stats = false;
-    vpbs::SortBuffers sort_sr, sort_tg, sort_lr;
+    vpbs::SortBuffers sort_sr, sort_tg, sort_lr, sort_nd;
 
     // long range
     int32_t G[3] = {0, 0, 0};   // oversampled grid
     int64_t comp_stride = 0;    // real grid values per component (G0·G1·G2)
     int32_t ntile[3] = {0, 0, 0};
     int64_t ntiles = 0;
+    // interpolation fused with the inverse x transform (VPBS_FFT_FUSED): the grid is cut
+    // into slabs of SLAB_HY y-rows × SLAB_HZ z-rows × all x-cells; nodes are sorted by
+    // the slab of their first stencil cell (nd_*), each slab writes its part of the
+    // stencil sums of the nodes it reaches into slab_part ([slots][3][Np])
+    bool slab_interp = false;
+    int64_t nslabs = 0;
+    int64_t cap_nd = 0;
+    int32_t *nd_key = nullptr, *nd_start = nullptr, *nd_perm = nullptr, *nd_i0 = nullptr;
+    void *nd_rec = nullptr, *slab_part = nullptr;
     // Real grid, 3 components, rows of y = 2p and y = 2p+1 interleaved as one
     // complex row (pair p): value (x, y, z) at ((z·G1/2 + y/2)·G0 + x)·2 + (y & 1).
     // Spreading writes it, the inverse x transform rewrites it, interpolation reads it.
@@ -181,6 +202,10 @@ int grow(vpbs_ctx *c, T **ptr, int64_t *cap, int64_t n) {
 // sort.hip
 int counting_sort(vpbs_ctx *c, SortBuffers &sb, const int32_t *keys, int64_t n, int64_t nbins, int32_t *start,
                   int32_t *perm, hipStream_t s);
+// The same result (order by key, then original index; start[b] = first position of
+// bin b, start[nbins] = n) by a stable LSD radix sort: no per-bin pass, for large bins.
+int radix_sort_bins(vpbs_ctx *c, SortBuffers &sb, const int32_t *keys, int64_t n, int64_t nbins, int32_t *start,
+                    int32_t *perm, hipStream_t s);
 
 // shortrange.hip
 int shortrange_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff);
@@ -198,6 +223,12 @@ int spectral_setup(vpbs_ctx *c);
 int spectral_forward(vpbs_ctx *c);
 int spectral_fourier(vpbs_ctx *c, uint32_t flags);
 int spectral_inverse(vpbs_ctx *c, void *zp);
+// The inverse y transform alone (zp → x1), and the inverse x transform fused with the
+// interpolation of x1 onto the nodes (out: FP64 [3][Np]; slab-sorted node data nd_*).
+bool spectral_slab_supported(const vpbs_ctx *c);
+int spectral_slab_slots(const vpbs_ctx *c);
+int spectral_y_inverse(vpbs_ctx *c, void *zp);
+int spectral_x_inverse_interp(vpbs_ctx *c, double *out);
 int spectral_band(vpbs_ctx *c, double kmin2, double kmax2);
 void spectral_release(vpbs_ctx *c);
 int longrange_spectrum(vpbs_ctx *c, int64_t nk, double dk, double *Ek);

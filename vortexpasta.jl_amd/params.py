@@ -227,7 +227,10 @@ class HIPLongRangeBackend:
     (``nonuniformffts.jl:107-120``); the defaults give ~6-digit accuracy.
     ``fft`` selects the transforms: ``"fused"`` (hand-written FFTs fused with the
     pencil reorderings and the Fourier-space kernel; grids up to 6144 (FP32) or
-    3072 (FP64) along x and y and half that along z, larger ones use rocFFT) or ``"rocfft"`` (rocFFT 1D
+    3072 (FP64) along x and y and half that along z, larger ones use rocFFT; on
+    x-grids of 192, 384 or 768 cells (768: FP32 only) the inverse x transform also
+    runs inside the interpolation), ``"separate"`` (the same
+    transforms, interpolation in a kernel of its own) or ``"rocfft"`` (rocFFT 1D
     transforms with separate reordering and Fourier-space kernels).
     """
 
@@ -237,8 +240,8 @@ class HIPLongRangeBackend:
     fft: str = "fused"
 
     def __post_init__(self):
-        if self.fft not in ("fused", "rocfft"):
-            raise ValueError(f"fft must be 'fused' or 'rocfft', got {self.fft!r}")
+        if self.fft not in ("fused", "separate", "rocfft"):
+            raise ValueError(f"fft must be 'fused', 'separate' or 'rocfft', got {self.fft!r}")
 
     def has_real_to_complex(self) -> bool:
         return True
