@@ -347,14 +347,12 @@ __device__ void lds_fft(int s_off, int lgB, int ld, const AxisPlan &ax, int tw_o
 // The same pass with the length, tile width and stride known at compile time (the
 // grid sizes of the common configurations): strides fold into immediate offsets
 // and the index arithmetic into shifts and constant multiplies.
-// LD: row stride of the tile (≥ BW); TWG: twiddles read from the global table twg
-// instead of the LDS copy at tw_off.
-template <typename CT, int R, int S, int NT, int L, int BW, int NS, int LD = BW, bool TWG = false>
-__device__ __forceinline__ void stockham_static(int s_off, int tw_off, const C2<CT> *__restrict__ twg = nullptr) {
+template <typename CT, int R, int S, int NT, int L, int BW, int NS>
+__device__ __forceinline__ void stockham_static(int s_off, int tw_off) {
     constexpr int NB = L / R, NWORK = NB * BW, TWS = L / (NS * R);
     constexpr int ITEMS = (NWORK + NT - 1) / NT;
     C2<CT> *Sm = lds<CT>(s_off);
-    const C2<CT> *tw = TWG ? twg : lds<CT>(tw_off);
+    const C2<CT> *tw = lds<CT>(tw_off);
     C2<CT> v[ITEMS][R];
 #pragma unroll
     for (int i = 0; i < ITEMS; i++) {
@@ -362,7 +360,7 @@ __device__ __forceinline__ void stockham_static(int s_off, int tw_off, const C2<
         if (NWORK % NT == 0 || w < NWORK) {
             const int b = w % BW, j = w / BW;
 #pragma unroll
-            for (int k = 0; k < R; k++) v[i][k] = Sm[(j + k * NB) * LD + b];
+            for (int k = 0; k < R; k++) v[i][k] = Sm[(j + k * NB) * BW + b];
         }
     }
     __syncthreads();
@@ -384,20 +382,20 @@ __device__ __forceinline__ void stockham_static(int s_off, int tw_off, const C2<
             dft<R, S>(v[i]);
             const int base = q * NS * R + jm;
 #pragma unroll
-            for (int k = 0; k < R; k++) Sm[(base + k * NS) * LD + b] = v[i][k];
+            for (int k = 0; k < R; k++) Sm[(base + k * NS) * BW + b] = v[i][k];
         }
     }
     __syncthreads();
 }
 
-// All passes of a compile-time plan on a tile of BW columns (row stride LD).
-template <typename CT, int S, int NT, int L, int BW, int P = 0, int NS = 1, int LD = BW, bool TWG = false>
-__device__ __forceinline__ void fft_static(int s_off, int tw_off, const C2<CT> *__restrict__ twg = nullptr) {
+// All passes of a compile-time plan on a tile of BW columns (row stride BW).
+template <typename CT, int S, int NT, int L, int BW, int P = 0, int NS = 1>
+__device__ __forceinline__ void fft_static(int s_off, int tw_off) {
     constexpr AxisPlan rd = plan_axis(L);
     if constexpr (P < rd.npass) {
         constexpr int R = rd.radix[P];
-        stockham_static<CT, R, S, NT, L, BW, NS, LD, TWG>(s_off, tw_off, twg);
-        fft_static<CT, S, NT, L, BW, P + 1, NS * R, LD, TWG>(s_off, tw_off, twg);
+        stockham_static<CT, R, S, NT, L, BW, NS>(s_off, tw_off);
+        fft_static<CT, S, NT, L, BW, P + 1, NS * R>(s_off, tw_off);
     }
 }
 
@@ -1036,7 +1034,7 @@ __global__ __launch_bounds__(NTX) void spec_x_inverse(SpecGeom g, const C2<CT> *
 //
 // Slab = SLAB_HY y-rows × SLAB_HZ z-rows × all G0 x-cells of the 3 components,
 // held in LDS as SLAB_BW complex columns b = (c·SLAB_HZ + zz)·SLAB_HP + pp (pair
-// row pp of z-row zz, component c) at S[x·SLAB_LD + b]. The block rebuilds the
+// row pp of z-row zz, component c) at S[x·SLAB_BW + b], then the twiddles. The block rebuilds the
 // columns from x1 as spec_x_inverse does and transforms them: S then holds the
 // grid values of the slab rows (real part: even y, imaginary part: odd y). Every
 // node whose W³ stencil reaches the slab has its first stencil cell in one of
@@ -1048,9 +1046,8 @@ __global__ __launch_bounds__(NTX) void spec_x_inverse(SpecGeom g, const C2<CT> *
 // slots in a fixed order. No grid written, no atomics: deterministic.
 constexpr int SLAB_HP = SLAB_HY / 2;
 constexpr int SLAB_ROWS = SLAB_HY * SLAB_HZ;
-constexpr int SLAB_NT = 512;
+constexpr int SLAB_NT = 768;  // 768-cell slabs: 4 radix-3 butterflies per thread, then at most 1 radix-16
 constexpr int SLAB_BW = 3 * SLAB_HZ * SLAB_HP;
-constexpr int SLAB_LD = SLAB_BW + 1;  // odd row stride: the x-cells of a slab row spread over the LDS banks
 static_assert(SLAB_ROWS == 8 && SLAB_HY % 2 == 0, "eight slab rows of whole pair rows: eight lanes per node");
 
 // Home slabs along y (z) of the nodes whose W-cell stencil reaches a slab.
@@ -1132,8 +1129,8 @@ __global__ __launch_bounds__(SLAB_NT) void x_inverse_interp(SlabArgs a, const C2
     };
     auto put = [&](int k, int b, C2<CT> A, C2<CT> Bv) {
         if (k == 0) A.im = Bv.im = (CT)0;
-        Sm[k * SLAB_LD + b] = C2<CT>{A.re - Bv.im, A.im + Bv.re};
-        if (k > 0) Sm[(LS - k) * SLAB_LD + b] = C2<CT>{A.re + Bv.im, Bv.re - A.im};
+        Sm[k * SLAB_BW + b] = C2<CT>{A.re - Bv.im, A.im + Bv.re};
+        if (k > 0) Sm[(LS - k) * SLAB_BW + b] = C2<CT>{A.re + Bv.im, Bv.re - A.im};
     };
     if (use_pairs<CT>() && !(nkx & 1)) {
         const int h = nkx >> 1;  // pairs (k, k+1) per row: 16-byte loads
@@ -1151,20 +1148,21 @@ __global__ __launch_bounds__(SLAB_NT) void x_inverse_interp(SlabArgs a, const C2
             put(k, b, src[0], src[nkx]);
         }
     }
+    load_twiddles<CT, SLAB_NT>(Sm + LS * SLAB_BW, tw, LS);
     // modes kx ∈ [nkx, L - nkx] are not retained
     for (int e = threadIdx.x; e < (LS - 2 * nkx + 1) * SLAB_BW; e += SLAB_NT) {
         const int k = nkx + e / SLAB_BW, b = e - (e / SLAB_BW) * SLAB_BW;
-        Sm[k * SLAB_LD + b] = C2<CT>{(CT)0, (CT)0};
+        Sm[k * SLAB_BW + b] = C2<CT>{(CT)0, (CT)0};
     }
     __syncthreads();
-    fft_static<CT, 1, SLAB_NT, LS, SLAB_BW, 0, 1, SLAB_LD, true>(0, 0, tw);
+    fft_static<CT, 1, SLAB_NT, LS, SLAB_BW>(0, LS * SLAB_BW);
 
     // stencil sums: lane (j, r) = node j of the group, slab row r
     const CT *Sf = reinterpret_cast<const CT *>(spec_smem);
     const int j = lane >> 3, r = lane & 7;
     const int yy = r % SLAB_HY, zz = r / SLAB_HY;
     const int y = ty * SLAB_HY + yy, z = tz * SLAB_HZ + zz;
-    const CT *row = Sf + (zz * SLAB_HP + (yy >> 1)) * 2 + (yy & 1);  // + 2·SLAB_LD·x + CST·c
+    const CT *row = Sf + (zz * SLAB_HP + (yy >> 1)) * 2 + (yy & 1);  // + 2·SLAB_BW·x + CST·c
     const int64_t Np = a.Np;
     const int32_t *si0 = a.si0;
     const CT *rec = static_cast<const CT *>(a.rec);
@@ -1209,7 +1207,7 @@ __global__ __launch_bounds__(SLAB_NT) void x_inverse_interp(SlabArgs a, const C2
                 int x = ix;
 #pragma unroll
                 for (int k = 0; k < W; k++) {
-                    const CT *p = row + x * (2 * SLAB_LD);
+                    const CT *p = row + x * (2 * SLAB_BW);
                     v0 += w[k] * p[0];
                     v1 += w[k] * p[CST];
                     v2 += w[k] * p[2 * CST];
@@ -1367,7 +1365,7 @@ const void *slab_kernel(int W, int L) {
 
 template <typename CT>
 size_t slab_lds(int L) {
-    return sizeof(C2<CT>) * (size_t)L * SLAB_LD;
+    return sizeof(C2<CT>) * (size_t)L * (SLAB_BW + 1);  // tile and twiddles
 }
 
 template <typename CT>
