@@ -447,9 +447,12 @@ __global__ __launch_bounds__(64) void spread_rows_kernel(LRGeom g, int64_t Nq, c
             __syncthreads();
         }
     }
-    // write: per component, rows to LDS, then contiguous pair rows (2·TX values) to the grid
+    // write: per component, rows to LDS, then contiguous pair rows (2·TX values) to the grid;
+    // 32-bit offsets from the tile origin (a tile spans 8 z-planes)
     constexpr int VPR = 2 * TX;     // values per pair row
     constexpr int RPI = 64 / VPR;   // pair rows per store instruction
+    CT *const tile0 = grid + (((int64_t)t0z * (g.G[1] >> 1) + (t0y >> 1)) * g.G[0] + t0x) * 2;
+    const int rstride = 2 * g.G[0], zstride = (g.G[1] >> 1) * rstride;
     static_for<0, 3>([&](auto C) {
         constexpr int c = decltype(C)::value;
         static_for<0, TX>([&](auto X) {
@@ -462,8 +465,7 @@ __global__ __launch_bounds__(64) void spread_rows_kernel(LRGeom g, int64_t Nq, c
             const int v = lane % VPR;
             const int z = row / NPAIR, pp = row - z * NPAIR;
             const CT val = buf[(z * TY + 2 * pp + (v & 1)) * TXP + (v >> 1)];
-            grid[c * g.cstride + (((int64_t)(t0z + z) * (g.G[1] >> 1) + (t0y >> 1) + pp) * g.G[0] + t0x) * 2 + v] =
-                val;
+            tile0[c * g.cstride + (z * zstride + pp * rstride + v)] = val;
         }
         __syncthreads();
     });
