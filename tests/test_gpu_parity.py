@@ -656,6 +656,32 @@ def test_fft_paths_agree_static_lengths(vp, Ns):
                 assert err <= tol
 
 
+@pytest.mark.parametrize("Ns", [127, (128, 120, 100)], ids=["odd_nkx", "noncubic"])
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+def test_slab_interpolation_shapes(vp, O, T, Ns):
+    """The slab interpolation (inverse x transform fused with the interpolation) on a 192-cell
+    x-grid with an odd number of retained kx (63: single-value loads) and on a 192×192×160 grid
+    (non-cubic slab counts): fused vs the separate interpolation and rocFFT (FP64 1e-12, FP32
+    2e-6); FP64 also against the oracle."""
+    fs = vp.tangles.random_rings(8, 0.5, 1.0, TWO_PI / 128, seed=5)
+    outs = {}
+    for fft in ("fused", "separate", "rocfft"):
+        params = _params(vp, T=T, Ns=Ns, backend_long=vp.HIPLongRangeBackend(fft=fft))
+        cache, outs[fft] = _gpu(vp, params, fs, shortrange=False, LIA=False)
+        cache.close()
+    tol = 1e-12 if T is np.float64 else 2e-6
+    for k in Q:
+        for fft in ("fused", "separate"):
+            err = rel_l2(outs[fft][k], outs["rocfft"][k])
+            print(f"  Ns={Ns} {T.__name__} {k}: {fft} vs rocFFT {err:.2e}", flush=True)
+            assert err <= tol
+    if T is np.float64:
+        params = _params(vp, Ns=Ns)
+        want = O.compute(vp.add_point_charges(fs, params), params, Q, LIA=False, shortrange=False)
+        for k in Q:
+            assert rel_l2(outs["fused"][k], want[k]) <= TOL64
+
+
 def test_empty_and_minimal_inputs(vp, O):
     """No filaments returns the empty outputs; a five-node filament (the fewest a closed quintic spline
     takes) and a mix of very different filament sizes match the oracle (FP64, 1e-12)."""
