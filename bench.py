@@ -31,8 +31,7 @@ import __graft_entry__ as ge
 
 METRIC = "Biot–Savart node-velocities/s at 1M nodes; rel L2 error vs FP64 oracle"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-FP64_VALU_PEAK_TFS = 78.6
-FP32_VALU_PEAK_TFS = 157.3
+VALU_PEAK_TFS = 78.6  # MI355X vector FP64, and FP32 without packed instructions (disabled in this build)
 
 
 def log(*a):
@@ -188,7 +187,7 @@ def time_end_to_end(vp, params, pd, reps: int = 3):
     return res
 
 
-def stage_bytes(stage: str, s: int, G, Ns, Np: int, Nq: int) -> float:
+def stage_bytes(stage: str, s: int, G, Ns, Np: int, Nq: int, slab: bool = False) -> float:
     """Algorithmic HBM bytes per launch of each long-range stage (velocity only).
 
     G: oversampled grid, Ns: Fourier grid, s: bytes per real of the compute type.
@@ -197,18 +196,53 @@ def stage_bytes(stage: str, s: int, G, Ns, Np: int, Nq: int) -> float:
     kernel 12Zs (z-pencils in and out, 3 complex components); FFT backward
     6Zs + 3Gs; interpolation 3Gs + 6Np·s, with G = ∏G_d and Z = nkx·nky·G_2 the
     complex values per component of the z-pencils (nkx = N_0/2, nky = 2⌊(N_1-1)/2⌋+1).
+    With the slab interpolation (``slab``) the inverse x transform moves from the
+    backward stage into the interpolation: backward 6Zs + 6X·s (x2 in, x1 out),
+    interpolation 6X·s + 6Np·s, X = nkx·G_1·G_2 retained x-modes per component.
     """
     Gc = float(np.prod(G))
     nkx = Ns[0] // 2
     nky = 2 * ((Ns[1] - 1) // 2) + 1
     Z = float(nkx) * nky * G[2]
+    X = float(nkx) * G[1] * G[2]
     return {
         "spread": 3 * Gc * s + 6 * Nq * s,
         "fft_forward": 3 * Gc * s + 6 * Z * s,
         "fourier": 12 * Z * s,
-        "fft_backward": 6 * Z * s + 3 * Gc * s,
-        "interp": 3 * Gc * s + 6 * Np * s,
+        "fft_backward": 6 * Z * s + (6 * X * s if slab else 3 * Gc * s),
+        "interp": (6 * X * s if slab else 3 * Gc * s) + 6 * Np * s,
     }[stage]
+
+
+def pair_kernel_report(vp, params, pd, stage_ms: float, reps: int = 5):
+    """Short-range pair kernel against the VALU peak.
+
+    One untimed evaluation with VPBS_NEIGHBOUR_STATS gives the candidate sources
+    tested (the (2M+1)³ neighbour cells of every node) and the accepted pairs;
+    flops = 11 per candidate (FP64 separation, fold, r²) + 40 per accepted pair
+    (sqrt, reciprocal, erfc, exp, weights, cross product, accumulation). The
+    kernel is timed alone (short range only, HIP events) and beside the long range
+    (``stage_ms``, from the timed evaluations).
+    """
+    L = vp._lib
+    ctx = L.Context(params)
+    ctx.set_points(pd)
+    ctx.compute(L.VELOCITY | L.NEIGHBOUR_STATS, fetch=False)
+    cand = int(ctx.timings()["pair_candidates"])
+    counts, _ = ctx.neighbour_stats()
+    acc = int(counts.sum())
+    ctx.compute(L.VELOCITY | L.NO_LONGRANGE, fetch=False)
+    alone = 0.0
+    for _ in range(reps):
+        ctx.compute(L.VELOCITY | L.NO_LONGRANGE, fetch=False)
+        alone += ctx.timings()["pair"] / reps
+    ctx.close()
+    flops = 11.0 * cand + 40.0 * acc
+    tf = flops / (alone * 1e-3) / 1e12
+    return {"bound": "valu", "candidates": cand, "accepted": acc,
+            "flops_model": "11 per candidate + 40 per accepted pair", "flops": flops,
+            "ms_alone": round(alone, 4), "ms_beside_long_range": round(stage_ms, 4),
+            "achieved_tflops": round(tf, 3), "peak_tflops": VALU_PEAK_TFS, "frac": round(tf / VALU_PEAK_TFS, 4)}
 
 
 def cpu_baseline(O, pd, params, target_s: float = 12.0):
@@ -292,8 +326,9 @@ def main():
     ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3, 4])
     ap.add_argument("--sigma", type=float, default=1.5, help="NUFFT oversampling factor")
     ap.add_argument("--m", type=int, default=4, help="NUFFT kernel half-support")
-    ap.add_argument("--fft", default="fused", choices=["fused", "rocfft"],
-                    help="long-range transforms: fused hand-written kernels (default) or rocFFT")
+    ap.add_argument("--fft", default="fused", choices=["fused", "separate", "rocfft"],
+                    help="long-range transforms: fused hand-written kernels (default), the same with the "
+                         "interpolation in its own kernel, or rocFFT")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 path (and the error figures)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -329,16 +364,25 @@ def main():
     e2e = time_end_to_end(vp, p32, pd, reps=2)
     log(f"end to end: {json.dumps(e2e)}")
 
-    # roofline of the dominant long-range kernel of the FP32 path (our own kernels only)
+    # roofline of the dominant long-range kernel of the FP32 path (our own kernels only); the
+    # hand-written transforms take the inverse x transform into the interpolation on 192-,
+    # 384- and 768-cell x-grids (FP32)
+    fused = args.fft == "fused"
+    slab = fused and grid[0] in (192, 384, 768)
     lr_kernels = ["spread", "fourier", "interp"]
     dom = max(lr_kernels, key=lambda k: st32.get(k, 0.0))
-    nbytes = stage_bytes(dom, 4, grid, spl.Ns, pd.Np, pd.Nq)
+    nbytes = stage_bytes(dom, 4, grid, spl.Ns, pd.Np, pd.Nq, slab)
     achieved = nbytes / (st32[dom] * 1e-3) / 1e9
-    per_stage = {k: round(stage_bytes(k, 4, grid, spl.Ns, pd.Np, pd.Nq) / (st32[k] * 1e-3) / 1e9, 1)
+    per_stage = {k: round(stage_bytes(k, 4, grid, spl.Ns, pd.Np, pd.Nq, slab) / (st32[k] * 1e-3) / 1e9, 1)
                  for k in ("spread", "fft_forward", "fourier", "fft_backward", "interp") if st32.get(k, 0) > 0}
 
-    kname = {"spread": f"spread_rows_kernel<float, {2 * args.m}>", "fourier": "fourier_kernel<float>",
-             "interp": f"interp_kernel_wave<float, {2 * args.m}>"}[dom]
+    kname = {"spread": f"spread_rows_kernel<float, {2 * args.m}>",
+             "fourier": f"spec_z_fourier<float, {grid[2]}>" if fused or args.fft == "separate" else "fourier_kernel<float>",
+             "interp": f"x_inverse_interp<float, {2 * args.m}, {grid[0]}>" if slab
+             else f"interp_kernel_wave<float, {2 * args.m}>"}[dom]
+    pair = pair_kernel_report(vp, p32, pd, st32.get("pair", 0.0))
+    pair["kernel"] = "pair_wave_kernel<float, true, false, false>"
+    log(f"pair kernel: {json.dumps(pair)}")
     default_workload = (args.config, args.sigma, args.m, args.fft) == (3, 1.5, 4, "fused")
     traffic = measured_traffic(kname) if default_workload else None
 
@@ -358,7 +402,9 @@ def main():
         "config": {
             "workload": f"config {args.config}: {len(fs)} random vortex rings, {pd.Np} nodes, {pd.Nq} quadrature "
                         f"points, Fourier grid {spl.Ns[0]}^3 (NUFFT grid {grid[0]}^3, sigma {args.sigma}, m {args.m}), beta 3.5, "
-                        f"FP32-bulk path, {'fused hand-written' if args.fft == 'fused' else 'rocFFT'} transforms",
+                        f"FP32-bulk path, " + {"fused": "fused hand-written transforms",
+                                               "separate": "hand-written transforms, separate interpolation",
+                                               "rocfft": "rocFFT transforms"}[args.fft],
             "nodes": pd.Np, "quadrature_points": pd.Nq, "fourier_grid": list(spl.Ns),
             "nufft_grid": list(grid), "nufft_sigma": args.sigma, "nufft_m": args.m, "cells": list(cells),
             "alpha": spl.α, "rcut": spl.rcut, "fft": args.fft,
@@ -378,6 +424,7 @@ def main():
         },
         "stages_ms": {k: round(v, 4) for k, v in st32.items()},
         "stage_algorithmic_GBps": per_stage,
+        "pair_kernel": pair,
         "point_data": pdt,
         "end_to_end_ms": e2e,
     }

@@ -104,7 +104,8 @@ typedef struct vpbs_timings {
     double interp;       /* type-2 interpolation (with the inverse x transform when fused; with the
                             streamfunction, also its inverse transforms) */
     double merge;        /* SR + LR sum */
-    int64_t pair_candidates; /* unused (reserved) */
+    int64_t pair_candidates; /* candidate sources the pair kernel tested (sum over nodes of the sources in
+                                the (2M+1)^3 neighbour cells); set by calls with VPBS_NEIGHBOUR_STATS, else 0 */
     int64_t grid[3];     /* oversampled NUFFT grid */
     int64_t cells[3];    /* cell-list cells per direction */
     double point_data;   /* kernels of the last vpbs_set_filaments (0 after vpbs_set_points*) */

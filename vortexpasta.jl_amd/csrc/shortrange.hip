@@ -141,6 +141,7 @@ struct PairArgs {
     double *psi;
     int64_t *counts;
     uint64_t *hashes;
+    int32_t *cands;  // stats: candidate sources tested per node
 };
 
 __device__ inline uint64_t mix64(uint64_t x) {
@@ -381,6 +382,7 @@ __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
         if (lane == 0) {
             a.counts[i] = cnt;
             a.hashes[i] = hsum;
+            a.cands[i] = ncand;
         }
     }
 }
@@ -487,6 +489,7 @@ int run_pairs(vpbs_ctx *c, bool vel, bool psi, bool stats, bool include_local) {
     a.psi = c->psi_sr;
     a.counts = c->stat_counts;
     a.hashes = c->stat_hashes;
+    a.cands = c->stat_cands;
     stage_begin(c, ST_PAIR, s);
     if (vel && psi)
         launch_pair<CT, true, true>(a, stats, s);

@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <new>
+#include <vector>
 
 namespace vpbs {
 
@@ -112,7 +113,7 @@ static void release(vpbs_ctx *c) {
     void *bufs[] = {c->nodes,     c->sp,        c->spp,      c->dlm,       c->dlp,        c->idx_prev,
                     c->points,    c->charges,   c->v_sr,     c->psi_sr,    c->v_lr,       c->psi_lr,
                     c->v_out,     c->psi_out,   c->src_key,  c->src_start, c->src_perm,   c->tgt_key,
-                    c->tgt_start, c->tgt_perm,  c->src_pos,  c->src_chg,   c->src_idx,    c->stat_counts,
+                    c->tgt_start, c->tgt_perm,  c->src_pos,  c->src_chg,   c->src_idx,    c->stat_counts, c->stat_cands,
                     c->stat_hashes, c->lr_key,  c->lr_start, c->lr_active, c->lr_perm,   c->lr_i0,      c->lr_w,
                     c->sort_sr.rank, c->sort_sr.scan_tmp, c->sort_tg.rank, c->sort_tg.scan_tmp,
                     c->sort_lr.rank, c->sort_lr.scan_tmp, c->sort_lr.keys_out, c->sort_lr.radix_tmp,
@@ -160,6 +161,7 @@ int ensure_capacity(vpbs_ctx *c, int64_t Np, int64_t Nq) {
         VPBS_TRY(alloc(c, &c->tgt_key, Np));
         VPBS_TRY(alloc(c, &c->tgt_perm, Np));
         VPBS_TRY(alloc(c, &c->stat_counts, Np));
+        VPBS_TRY(alloc(c, &c->stat_cands, Np));
         VPBS_TRY(alloc(c, &c->stat_hashes, Np));
         c->cap_np = Np;
     }
@@ -453,6 +455,12 @@ int vpbs_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff, double 
             *slots[st] = ms;
     }
     T.total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    T.pair_candidates = 0;
+    if (with_sr && (flags & VPBS_NEIGHBOUR_STATS) && !(flags & VPBS_LIA_ONLY) && c->Np > 0) {
+        std::vector<int32_t> nc((size_t)c->Np);
+        VPBS_HIP(c, hipMemcpy(nc.data(), c->stat_cands, sizeof(int32_t) * (size_t)c->Np, hipMemcpyDeviceToHost));
+        for (int32_t v : nc) T.pair_candidates += v;
+    }
     for (int d = 0; d < 3; d++) {
         T.grid[d] = c->G[d];
         T.cells[d] = c->Nc[d];

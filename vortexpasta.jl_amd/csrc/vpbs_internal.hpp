@@ -110,6 +110,7 @@ struct vpbs_ctx {
     void *src_chg = nullptr;     // sorted charges, 3*Nq of the compute type
     int32_t *src_idx = nullptr;  // original index of each sorted source
     int64_t *stat_counts = nullptr;
+    int32_t *stat_cands = nullptr;  // per node: candidate sources tested by the pair kernel (stats calls)
     uint64_t *stat_hashes = nullptr;
     bool have_stats = false;
     vpbs::SortBuffers sort_sr, sort_tg, sort_lr, sort_nd;
