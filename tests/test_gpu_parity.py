@@ -161,6 +161,38 @@ def test_fp32_bulk_path(vp, O, cfg2):
     cache.close()
 
 
+def test_pair_candidate_count(vp, cfg2):
+    """vpbs_timings.pair_candidates after a VPBS_NEIGHBOUR_STATS call equals the number of sources in
+    the (2M+1)³ neighbour cells of every node (periodic 3D histogram of the sources, cell index
+    trunc(x/L·N) as in the cell-list build), and is at least the accepted pair count; 0 otherwise."""
+    params = _params(vp, Ns=128)
+    pd = vp.add_point_charges(cfg2, params)
+    ctx = vp._lib.Context(params)
+    ctx.set_points(pd)
+    ctx.compute(vp._lib.VELOCITY | vp._lib.NEIGHBOUR_STATS, fetch=False)
+    got = ctx.timings()["pair_candidates"]
+    counts, _ = ctx.neighbour_stats()
+    _, Nc = ctx.sizes()
+    ctx.compute(vp._lib.VELOCITY, fetch=False)
+    assert ctx.timings()["pair_candidates"] == 0
+    ctx.close()
+    M = params.backend_short.nsubdiv
+    L = np.asarray(params.Ls, dtype=np.float64)
+    cell = lambda X: [((X[d] / L[d]) * Nc[d]).astype(np.int64) for d in range(3)]  # noqa: E731 (X: [3][n])
+    sx, sy, sz = cell(np.asarray(pd.points))
+    H = np.zeros(Nc[::-1], dtype=np.int64)
+    np.add.at(H, (sz, sy, sx), 1)
+    S = np.zeros_like(H)
+    for dz in range(-M, M + 1):
+        for dy in range(-M, M + 1):
+            for dx in range(-M, M + 1):
+                S += np.roll(H, (dz, dy, dx), axis=(0, 1, 2))
+    nx, ny, nz = cell(np.asarray(pd.nodes))
+    want = int(S[nz, ny, nx].sum())
+    print(f"  candidates {got} (expected {want}), accepted {int(counts.sum())}", flush=True)
+    assert got == want and got >= counts.sum() > 0
+
+
 @pytest.mark.parametrize("T", [np.float64, np.float32])
 def test_deterministic_repeat(vp, cfg2, T):
     """Two evaluations of the same input are bit-identical (deterministic sorts, no float atomics;
