@@ -88,7 +88,10 @@ __global__ void zero_f64(double *__restrict__ a, int64_t n) {
 // ----------------------------------------------------------------------------
 // Cell list
 
-__global__ void cell_keys(int64_t n, const double *__restrict__ pos, Box box, int3 Nc, int32_t *__restrict__ key) {
+// key[j] = linear cell of point j; if cell != nullptr also cell[j] = (ix, iy, iz, key), so that the
+// pair kernel reads its target's cell instead of recomputing three FP64 divisions per node.
+__global__ void cell_keys(int64_t n, const double *__restrict__ pos, Box box, int3 Nc, int32_t *__restrict__ key,
+                          int4 *__restrict__ cell) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     int ix = cell_index(pos[j], box.L[0], Nc.x);
@@ -98,7 +101,9 @@ __global__ void cell_keys(int64_t n, const double *__restrict__ pos, Box box, in
     ix = min(max(ix, 0), Nc.x - 1);
     iy = min(max(iy, 0), Nc.y - 1);
     iz = min(max(iz, 0), Nc.z - 1);
-    key[j] = (iz * Nc.y + iy) * Nc.x + ix;
+    const int k = (iz * Nc.y + iy) * Nc.x + ix;
+    key[j] = k;
+    if (cell) cell[j] = make_int4(ix, iy, iz, k);
 }
 
 template <typename CT>
@@ -123,6 +128,7 @@ template <typename CT>
 struct PairArgs {
     int64_t Np, Nq;
     const int32_t *tperm;
+    const int4 *tcell;  // per node: (ix, iy, iz, key) of its cell
     const double *nodes;
     const int32_t *cstart;
     const double *spos;
@@ -184,6 +190,19 @@ __device__ inline void rinv_r(float r2, float &rinv, float &r) {
     r = r2 * rinv;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wavefront with DPP row operations: shifts by
+// 1, 2, 4 and 8 lanes within each row of 16, then the last lane of row 0 (row 2) added to
+// row 1 (row 3), then lane 31 added to rows 2 and 3. Lanes a shift does not reach add 0.
+__device__ inline int wave_incl_scan(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return v;
+}
+
 // One wavefront per target node (cell-sorted order; XCD-aware block order):
 //  1) lanes build the source runs of the (2M+1)² neighbour cell rows (≤ 2 contiguous
 //     runs per row with the periodic wrap along x) and a wave scan gives each row's
@@ -196,14 +215,16 @@ __device__ inline void rinv_r(float r2, float &rinv, float &r) {
 //     (erfc/Gaussian weights, cross product; integrands.jl:2-18, gaussian.jl:273-279)
 //     into its own compensated accumulator, so the expensive part runs on full waves;
 //  4) the 64 partial sums are combined by a fixed butterfly in FP64.
-// The accepted pair set is the thread-per-target one; the summation order is fixed
-// (deterministic) but differs from a sequential sum.
+// A candidate's row is found by walking the row boundaries in order with a wave-uniform
+// row pointer (each boundary is passed once per node). The accepted pair set is the
+// thread-per-target one; the summation order is fixed (deterministic) but differs from a
+// sequential sum.
 constexpr int PAIR_MAXROWS = 81;  // (2M+1)² for M ≤ 4
 
 template <typename CT, bool VEL, bool PSI, bool STATS>
 __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
-    __shared__ int r_pre[PAIR_MAXROWS + 1];  // candidates before row r
-    __shared__ int r_a0[PAIR_MAXROWS], r_n0[PAIR_MAXROWS], r_a1[PAIR_MAXROWS];
+    // per row: (candidates before it, run 0 start, run 0 length, run 1 start - run 0 length)
+    __shared__ int4 r_run[PAIR_MAXROWS];
     __shared__ int q_s[128];
     __shared__ CT q_r[4][128];  // rx, ry, rz, r² of the queued pairs
 
@@ -214,20 +235,25 @@ __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
     if (k >= Np) return;
     const int32_t i = a.tperm[k];
     const double x = a.nodes[i], y = a.nodes[Np + i], z = a.nodes[2 * Np + i];
-    const int cx = min(max(cell_index(x, a.box.L[0], a.Nc.x), 0), a.Nc.x - 1);
-    const int cy = min(max(cell_index(y, a.box.L[1], a.Nc.y), 0), a.Nc.y - 1);
-    const int cz = min(max(cell_index(z, a.box.L[2], a.Nc.z), 0), a.Nc.z - 1);
+    const int4 tc = a.tcell[i];  // the node's cell, as computed for the sort
+    const int cx = tc.x, cy = tc.y, cz = tc.z;
     // first quadrature points of the node's right and left segments (source indices are int32)
     const int jr0 = a.nquad * i, jl0 = a.nquad * a.idx_prev[i];
     const int M = a.M, W2 = 2 * M + 1, nrows = W2 * W2;
+    // row r = (dz, dy) = (r / W2 - M, r % W2 - M); the quotient from a float product with the
+    // hardware reciprocal (1 ulp) is exact: (r + 1/2) / W2 stays ≥ 1/18 away from an integer
+    // for r < 128, W2 ≤ 9
+    const float rcp_w2 = __builtin_amdgcn_rcpf((float)W2);
 
-    // 1) rows (dz, dy) = (r / W2 - M, r % W2 - M); lanes take rows lane and lane + 64
+    // 1) lanes take rows lane and lane + 64
     int len[2] = {0, 0};
+    int4 run[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         const int r = lane + 64 * h;
         if (r < nrows) {
-            int zz = cz + r / W2 - M, yy = cy + r % W2 - M;
+            const int qz = (int)(((float)r + 0.5f) * rcp_w2);
+            int zz = cz + qz - M, yy = cy + (r - qz * W2) - M;
             zz += (zz < 0) ? a.Nc.z : 0;
             zz -= (zz >= a.Nc.z) ? a.Nc.z : 0;
             yy += (yy < 0) ? a.Nc.y : 0;
@@ -248,25 +274,24 @@ __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
                 s1 = a.cstart[row + ca1];
                 n1 = a.cstart[row + cb1 + 1] - s1;
             }
-            r_a0[r] = s0;
-            r_n0[r] = n0;
-            r_a1[r] = s1;
+            run[h] = make_int4(0, s0, n0, s1 - n0);
             len[h] = n0 + n1;
         }
     }
-    int incl0 = len[0], incl1 = len[1];
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int u0 = __shfl_up(incl0, off, 64), u1 = __shfl_up(incl1, off, 64);
-        if (lane >= off) incl0 += u0, incl1 += u1;
+    // inclusive prefix: lane r holds the candidates before row r + 1 (incl0) or row r + 65 (incl1)
+    const int incl0 = wave_incl_scan(len[0]);
+    const int tot0 = __builtin_amdgcn_readlane(incl0, 63);
+    const int incl1 = nrows > 64 ? tot0 + wave_incl_scan(len[1]) : tot0;
+    if (lane < nrows) {
+        run[0].x = incl0 - len[0];
+        r_run[lane] = run[0];
     }
-    const int tot0 = __shfl(incl0, 63, 64);
-    if (lane < nrows) r_pre[lane + 1] = incl0;
-    if (lane + 64 < nrows) r_pre[lane + 65] = tot0 + incl1;
-    if (lane == 0) r_pre[0] = 0;
-    const int ncand = tot0 + __shfl(incl1, 63, 64);
+    if (lane + 64 < nrows) {
+        run[1].x = incl1 - len[1];
+        r_run[lane + 64] = run[1];
+    }
+    const int ncand = nrows > 64 ? __builtin_amdgcn_readlane(incl1, nrows - 65) : __builtin_amdgcn_readlane(incl0, nrows - 1);
     __syncthreads();
-
     Acc3<CT> av, ap;
     int64_t cnt = 0;
     uint64_t hsum = 0;
@@ -294,37 +319,40 @@ __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
         }
     };
 
-    int nqd = 0;  // queued pairs (uniform)
+    int nqd = 0;   // queued pairs (uniform)
+    int rw = 0;    // uniform: last row starting at or before the current candidate batch's end
+    int row = 0;   // per lane: last row starting at or before its candidate
     for (int base = 0; base < ncand; base += 64) {
-        // 2) test 64 candidates
+        // 2) test 64 candidates; first pass the row boundaries that fall in this batch
         const int cnd = base + lane;
+        row = rw;  // every boundary passed so far lies at or before base
+        while (rw + 1 < nrows) {
+            const int next = rw < 64 ? __builtin_amdgcn_readlane(incl0, rw) : __builtin_amdgcn_readlane(incl1, rw - 64);
+            if (next > base + 63) break;
+            rw++;
+            row += cnd >= next;  // boundaries are non-decreasing: row counts those at or before cnd
+        }
         bool acc = false;
         int s = 0;
         double rx = 0, ry = 0, rz = 0, r2 = 0;
         if (cnd < ncand) {
-            int lo = 0, hi = nrows - 1;  // last row with r_pre[row] <= cnd
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (r_pre[mid] <= cnd)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            const int o = cnd - r_pre[lo];
-            s = o < r_n0[lo] ? r_a0[lo] + o : r_a1[lo] + (o - r_n0[lo]);
+            const int4 rr = r_run[row];
+            const int o = cnd - rr.x;
+            s = (o < rr.z ? rr.y : rr.w) + o;
+            // branch-free: the index and position loads issue together (the local-segment
+            // test no longer gates the position loads)
             const int32_t j = a.sidx[s];
+            const double sx = a.spos[s], sy = a.spos[Nq + s], sz = a.spos[2 * Nq + s];
             const bool local = !a.include_local &&
                                ((unsigned)(j - jr0) < (unsigned)a.nquad || (unsigned)(j - jl0) < (unsigned)a.nquad);
-            if (!local) {
-                rx = fold_sep(x - a.spos[s], a.box.L[0], a.box.Lh[0]);
-                ry = fold_sep(y - a.spos[Nq + s], a.box.L[1], a.box.Lh[1]);
-                rz = fold_sep(z - a.spos[2 * Nq + s], a.box.L[2], a.box.Lh[2]);
-                r2 = dist2_nocontract(rx, ry, rz);
-                acc = r2 <= a.rcut2;
-                if (STATS && acc) {
-                    cnt++;
-                    hsum += mix64((uint64_t)j);
-                }
+            rx = fold_sep(x - sx, a.box.L[0], a.box.Lh[0]);
+            ry = fold_sep(y - sy, a.box.L[1], a.box.Lh[1]);
+            rz = fold_sep(z - sz, a.box.L[2], a.box.Lh[2]);
+            r2 = dist2_nocontract(rx, ry, rz);
+            acc = !local && r2 <= a.rcut2;
+            if (STATS && acc) {
+                cnt++;
+                hsum += mix64((uint64_t)j);
             }
         }
         const unsigned long long mask = __ballot(acc);
@@ -472,11 +500,11 @@ int run_pairs(vpbs_ctx *c, bool vel, bool psi, bool stats, bool include_local) {
     hipStream_t s = c->s_sr;
     const int3 Nc = make_int3(c->Nc[0], c->Nc[1], c->Nc[2]);
     stage_begin(c, ST_CELL, s);
-    cell_keys<<<nblocks(Nq, 256), 256, 0, s>>>(Nq, c->points, c->box, Nc, c->src_key);
+    cell_keys<<<nblocks(Nq, 256), 256, 0, s>>>(Nq, c->points, c->box, Nc, c->src_key, nullptr);
     VPBS_TRY(counting_sort(c, c->sort_sr, c->src_key, Nq, c->ncells, c->src_start, c->src_perm, s));
     gather_sources<CT><<<nblocks(Nq, 256), 256, 0, s>>>(Nq, c->src_perm, c->points, c->charges, c->src_pos,
                                                           (CT *)c->src_chg, c->src_idx);
-    cell_keys<<<nblocks(Np, 256), 256, 0, s>>>(Np, c->nodes, c->box, Nc, c->tgt_key);
+    cell_keys<<<nblocks(Np, 256), 256, 0, s>>>(Np, c->nodes, c->box, Nc, c->tgt_key, (int4 *)c->tgt_cell);
     VPBS_TRY(counting_sort(c, c->sort_tg, c->tgt_key, Np, c->ncells, c->tgt_start, c->tgt_perm, s));
     stage_end(c, ST_CELL, s);
 
@@ -484,6 +512,7 @@ int run_pairs(vpbs_ctx *c, bool vel, bool psi, bool stats, bool include_local) {
     a.Np = Np;
     a.Nq = Nq;
     a.tperm = c->tgt_perm;
+    a.tcell = (const int4 *)c->tgt_cell;
     a.nodes = c->nodes;
     a.cstart = c->src_start;
     a.spos = c->src_pos;

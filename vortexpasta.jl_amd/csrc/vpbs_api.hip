@@ -112,7 +112,7 @@ static void release(vpbs_ctx *c) {
     longrange_release(c);
     void *bufs[] = {c->nodes,     c->sp,        c->spp,      c->dlm,       c->dlp,        c->idx_prev,
                     c->points,    c->charges,   c->v_sr,     c->psi_sr,    c->v_lr,       c->psi_lr,
-                    c->v_out,     c->psi_out,   c->src_key,  c->src_start, c->src_perm,   c->tgt_key,
+                    c->v_out,     c->psi_out,   c->src_key,  c->src_start, c->src_perm,   c->tgt_key,   c->tgt_cell,
                     c->tgt_start, c->tgt_perm,  c->src_pos,  c->src_chg,   c->src_idx,    c->stat_counts, c->stat_cands,
                     c->stat_hashes, c->lr_key,  c->lr_start, c->lr_active, c->lr_perm,   c->lr_i0,      c->lr_w,
                     c->sort_sr.rank, c->sort_sr.scan_tmp, c->sort_tg.rank, c->sort_tg.scan_tmp,
@@ -159,6 +159,7 @@ int ensure_capacity(vpbs_ctx *c, int64_t Np, int64_t Nq) {
         VPBS_TRY(alloc(c, &c->v_out, 3 * Np));
         VPBS_TRY(alloc(c, &c->psi_out, 3 * Np));
         VPBS_TRY(alloc(c, &c->tgt_key, Np));
+        VPBS_TRY(alloc(c, &c->tgt_cell, 4 * Np));
         VPBS_TRY(alloc(c, &c->tgt_perm, Np));
         VPBS_TRY(alloc(c, &c->stat_counts, Np));
         VPBS_TRY(alloc(c, &c->stat_cands, Np));

@@ -106,6 +106,7 @@ struct vpbs_ctx {
     int64_t ncells = 0;
     int32_t *src_key = nullptr, *src_start = nullptr, *src_perm = nullptr;
     int32_t *tgt_key = nullptr, *tgt_start = nullptr, *tgt_perm = nullptr;
+    int32_t *tgt_cell = nullptr;  // per node (original order): cell (ix, iy, iz, key), 4*Np
     double *src_pos = nullptr;   // sorted sources, 3*Nq (component-major)
     void *src_chg = nullptr;     // sorted charges, 3*Nq of the compute type
     int32_t *src_idx = nullptr;  // original index of each sorted source
