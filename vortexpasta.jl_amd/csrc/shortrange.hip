@@ -320,40 +320,52 @@ __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
     };
 
     int nqd = 0;   // queued pairs (uniform)
-    int rw = 0;    // uniform: last row starting at or before the current candidate batch's end
+    int rw = 0;    // uniform: last row starting at or before the fetched batch's end
     int row = 0;   // per lane: last row starting at or before its candidate
-    for (int base = 0; base < ncand; base += 64) {
-        // 2) test 64 candidates; first pass the row boundaries that fall in this batch
-        const int cnd = base + lane;
-        row = rw;  // every boundary passed so far lies at or before base
+    // Locates candidate b + lane (passing the row boundaries that fall in batch b) and issues
+    // the loads of its source, index and position together. Branch-free, so that the loads of
+    // one batch can stay in flight while the previous batch is tested: lanes past the last
+    // candidate read source 0 (there is at least one source when there are candidates).
+    auto fetch = [&](int b, int &fs, int32_t &fj, double &fx, double &fy, double &fz) {
+        const int cnd = b + lane;
+        row = rw;  // every boundary passed so far lies at or before b
         while (rw + 1 < nrows) {
             const int next = rw < 64 ? __builtin_amdgcn_readlane(incl0, rw) : __builtin_amdgcn_readlane(incl1, rw - 64);
-            if (next > base + 63) break;
+            if (next > b + 63) break;
             rw++;
             row += cnd >= next;  // boundaries are non-decreasing: row counts those at or before cnd
         }
-        bool acc = false;
-        int s = 0;
-        double rx = 0, ry = 0, rz = 0, r2 = 0;
-        if (cnd < ncand) {
-            const int4 rr = r_run[row];
-            const int o = cnd - rr.x;
-            s = (o < rr.z ? rr.y : rr.w) + o;
-            // branch-free: the index and position loads issue together (the local-segment
-            // test no longer gates the position loads)
-            const int32_t j = a.sidx[s];
-            const double sx = a.spos[s], sy = a.spos[Nq + s], sz = a.spos[2 * Nq + s];
-            const bool local = !a.include_local &&
-                               ((unsigned)(j - jr0) < (unsigned)a.nquad || (unsigned)(j - jl0) < (unsigned)a.nquad);
-            rx = fold_sep(x - sx, a.box.L[0], a.box.Lh[0]);
-            ry = fold_sep(y - sy, a.box.L[1], a.box.Lh[1]);
-            rz = fold_sep(z - sz, a.box.L[2], a.box.Lh[2]);
-            r2 = dist2_nocontract(rx, ry, rz);
-            acc = !local && r2 <= a.rcut2;
-            if (STATS && acc) {
-                cnt++;
-                hsum += mix64((uint64_t)j);
-            }
+        const int4 rr = r_run[row];  // row < nrows
+        const int o = cnd - rr.x;
+        fs = cnd < ncand ? (o < rr.z ? rr.y : rr.w) + o : 0;
+        fj = a.sidx[fs];
+        fx = a.spos[fs];
+        fy = a.spos[Nq + fs];
+        fz = a.spos[2 * Nq + fs];
+    };
+    // current batch's source (s, j, position); the next batch's is fetched before the
+    // current one is tested, so its load latency overlaps the test and the evaluation
+    int s = 0;
+    int32_t j = 0;
+    double sx = 0, sy = 0, sz = 0;
+    if (ncand > 0) fetch(0, s, j, sx, sy, sz);
+    for (int base = 0; base < ncand; base += 64) {
+        int s_n;
+        int32_t j_n;
+        double sx_n, sy_n, sz_n;
+        fetch(base + 64, s_n, j_n, sx_n, sy_n, sz_n);
+        // 2) test 64 candidates
+        const int cnd = base + lane;
+        const bool local = !a.include_local &&
+                           ((unsigned)(j - jr0) < (unsigned)a.nquad || (unsigned)(j - jl0) < (unsigned)a.nquad);
+        const double rx = fold_sep(x - sx, a.box.L[0], a.box.Lh[0]);
+        const double ry = fold_sep(y - sy, a.box.L[1], a.box.Lh[1]);
+        const double rz = fold_sep(z - sz, a.box.L[2], a.box.Lh[2]);
+        const double r2 = dist2_nocontract(rx, ry, rz);
+        const bool acc = cnd < ncand && !local && r2 <= a.rcut2;
+        if (STATS && acc) {
+            cnt++;
+            hsum += mix64((uint64_t)j);
         }
         const unsigned long long mask = __ballot(acc);
         if (acc) {
@@ -386,6 +398,11 @@ __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
             nqd = rest;
             __syncthreads();
         }
+        s = s_n;
+        j = j_n;
+        sx = sx_n;
+        sy = sy_n;
+        sz = sz_n;
     }
     if (lane < nqd) eval(lane);
 
