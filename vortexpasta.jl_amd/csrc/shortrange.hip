@@ -240,6 +240,12 @@ __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
     // first quadrature points of the node's right and left segments (source indices are int32)
     const int jr0 = a.nquad * i, jl0 = a.nquad * a.idx_prev[i];
     const int M = a.M, W2 = 2 * M + 1, nrows = W2 * W2;
+    // The minimum-image fold changes no separation of a node whose neighbour cells do not
+    // wrap, when every direction has at least 2M + 2 cells: then |x - s| < (M + 1)·L/Nc ≤ L/2
+    // for every candidate, up to rounding at exactly L/2, which r² ≤ rcut² (rcut < L/2)
+    // rejects either way. Such nodes (most of them) skip the fold.
+    const bool wraps = cx < M || cy < M || cz < M || cx + M >= a.Nc.x || cy + M >= a.Nc.y || cz + M >= a.Nc.z ||
+                       min(min(a.Nc.x, a.Nc.y), a.Nc.z) < 2 * M + 2;
     // row r = (dz, dy) = (r / W2 - M, r % W2 - M); the quotient from a float product with the
     // hardware reciprocal (1 ulp) is exact: (r + 1/2) / W2 stays ≥ 1/18 away from an integer
     // for r < 128, W2 ≤ 9
@@ -358,9 +364,12 @@ __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
         const int cnd = base + lane;
         const bool local = !a.include_local &&
                            ((unsigned)(j - jr0) < (unsigned)a.nquad || (unsigned)(j - jl0) < (unsigned)a.nquad);
-        const double rx = fold_sep(x - sx, a.box.L[0], a.box.Lh[0]);
-        const double ry = fold_sep(y - sy, a.box.L[1], a.box.Lh[1]);
-        const double rz = fold_sep(z - sz, a.box.L[2], a.box.Lh[2]);
+        double rx = x - sx, ry = y - sy, rz = z - sz;
+        if (wraps) {  // wave-uniform
+            rx = fold_sep(rx, a.box.L[0], a.box.Lh[0]);
+            ry = fold_sep(ry, a.box.L[1], a.box.Lh[1]);
+            rz = fold_sep(rz, a.box.L[2], a.box.Lh[2]);
+        }
         const double r2 = dist2_nocontract(rx, ry, rz);
         const bool acc = cnd < ncand && !local && r2 <= a.rcut2;
         if (STATS && acc) {
