@@ -193,6 +193,28 @@ def test_pair_candidate_count(vp, cfg2):
     assert got == want and got >= counts.sum() > 0
 
 
+@pytest.mark.parametrize("M,Ns,cells", [(1, 26, 3), (1, 34, 4), (2, 22, 5), (2, 26, 6), (3, 24, 8)])
+def test_few_cells_min_image(vp, O, cfg2, M, Ns, cells):
+    """Neighbour sets and values with 2M+1 and 2M+2 cells per direction. The pair kernel skips
+    the minimum-image fold for nodes whose neighbour cells do not wrap when there are at
+    least 2M+2 cells; with 2M+1 cells the neighbour range covers the whole box and every node
+    folds. Both must give the brute-force pair sets and the oracle's values."""
+    fs = cfg2[:16]
+    params = _params(vp, Ns=Ns, backend_short=vp.HIPShortRangeBackend(nsubdiv=M))
+    assert math.floor(TWO_PI * M / params.rcut) == cells
+    cache, got = _gpu(vp, params, fs, ("velocity",), pointdata="host", neighbour_stats=True)
+    pd = cache.pointdata
+    counts, hashes = cache.ctx.neighbour_stats()
+    bc, bh = O.neighbour_pairs_bruteforce(pd, params)
+    print(f"  M = {M}, {cells} cells, Np = {pd.Np}, accepted {int(counts.sum())}", flush=True)
+    assert np.array_equal(counts, bc) and np.array_equal(hashes, bh)
+    want = O.compute(pd, params, ("velocity",))
+    err = rel_l2(got["velocity"], want["velocity"])
+    print(f"  velocity: rel L2 vs oracle = {err:.3e}", flush=True)
+    assert err <= TOL64
+    cache.close()
+
+
 @pytest.mark.parametrize("T", [np.float64, np.float32])
 def test_deterministic_repeat(vp, cfg2, T):
     """Two evaluations of the same input are bit-identical (deterministic sorts, no float atomics;
