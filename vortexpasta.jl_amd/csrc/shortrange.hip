@@ -531,7 +531,9 @@ int run_pairs(vpbs_ctx *c, bool vel, bool psi, bool stats, bool include_local) {
     gather_sources<CT><<<nblocks(Nq, 256), 256, 0, s>>>(Nq, c->src_perm, c->points, c->charges, c->src_pos,
                                                           (CT *)c->src_chg, c->src_idx);
     cell_keys<<<nblocks(Np, 256), 256, 0, s>>>(Np, c->nodes, c->box, Nc, c->tgt_key, (int4 *)c->tgt_cell);
-    VPBS_TRY(counting_sort(c, c->sort_tg, c->tgt_key, Np, c->ncells, c->tgt_start, c->tgt_perm, s));
+    // targets only set the pair kernel's processing order (each node's sum is independent of
+    // it): no per-cell index sort
+    VPBS_TRY(counting_sort(c, c->sort_tg, c->tgt_key, Np, c->ncells, c->tgt_start, c->tgt_perm, s, false));
     stage_end(c, ST_CELL, s);
 
     PairArgs<CT> a;

@@ -8,7 +8,8 @@
 //   1. histogram with atomics, remembering each element's arrival rank
 //   2. exclusive scan of the histogram (multi-level, 1024 elements per block)
 //   3. scatter to start[key] + rank
-//   4. one thread per bin sorts its (short) segment by original index
+//   4. one thread per bin sorts its (short) segment by original index (skipped for the
+//      pair kernel's target order, which no result depends on)
 //
 // radix_sort_bins gives the same order with a stable LSD radix sort of the keys
 // (original indices as values) and derives the bin starts from the sorted keys;
@@ -142,14 +143,14 @@ void exclusive_scan(int32_t *a, int64_t n, int32_t *tmp, hipStream_t s) {
 }  // namespace
 
 int counting_sort(vpbs_ctx *c, SortBuffers &sb, const int32_t *keys, int64_t n, int64_t nbins, int32_t *start,
-                  int32_t *perm, hipStream_t s) {
+                  int32_t *perm, hipStream_t s, bool within_bin) {
     VPBS_TRY(grow(c, &sb.rank, &sb.cap_n, n));
     VPBS_TRY(grow(c, &sb.scan_tmp, &sb.cap_tmp, scan_tmp_size(nbins + 1)));
     fill_i32<<<nblocks(nbins + 1, 256), 256, 0, s>>>(start, nbins + 1, 0);
     if (n > 0) histogram_rank<<<nblocks(n, 256), 256, 0, s>>>(keys, n, start, sb.rank);
     exclusive_scan(start, nbins + 1, sb.scan_tmp, s);
     if (n > 0) scatter_ranked<<<nblocks(n, 256), 256, 0, s>>>(keys, sb.rank, n, start, perm);
-    sort_segments<<<nblocks(nbins, 256), 256, 0, s>>>(start, nbins, perm);
+    if (within_bin) sort_segments<<<nblocks(nbins, 256), 256, 0, s>>>(start, nbins, perm);
     VPBS_HIP(c, hipGetLastError());
     return VPBS_OK;
 }

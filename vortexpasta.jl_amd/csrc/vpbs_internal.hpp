@@ -202,8 +202,11 @@ int grow(vpbs_ctx *c, T **ptr, int64_t *cap, int64_t n) {
 }
 
 // sort.hip
+// Order by key; with within_bin = true also by original index inside each bin (deterministic),
+// otherwise the order inside a bin is that of the atomic ranks (for callers to which it does
+// not matter). start[b] = first position of bin b, start[nbins] = n.
 int counting_sort(vpbs_ctx *c, SortBuffers &sb, const int32_t *keys, int64_t n, int64_t nbins, int32_t *start,
-                  int32_t *perm, hipStream_t s);
+                  int32_t *perm, hipStream_t s, bool within_bin = true);
 // The same result (order by key, then original index; start[b] = first position of
 // bin b, start[nbins] = n) by a stable LSD radix sort: no per-bin pass, for large bins.
 int radix_sort_bins(vpbs_ctx *c, SortBuffers &sb, const int32_t *keys, int64_t n, int64_t nbins, int32_t *start,
