@@ -193,12 +193,14 @@ def test_pair_candidate_count(vp, cfg2):
     assert got == want and got >= counts.sum() > 0
 
 
-@pytest.mark.parametrize("M,Ns,cells", [(1, 26, 3), (1, 34, 4), (2, 22, 5), (2, 26, 6), (3, 24, 8)])
+@pytest.mark.parametrize("M,Ns,cells", [(1, 26, 3), (1, 34, 4), (2, 22, 5), (2, 26, 6), (3, 24, 8), (4, 20, 9),
+                                         (4, 22, 10), (4, 26, 12)])
 def test_few_cells_min_image(vp, O, cfg2, M, Ns, cells):
     """Neighbour sets and values with 2M+1 and 2M+2 cells per direction. The pair kernel skips
     the minimum-image fold for nodes whose neighbour cells do not wrap when there are at
     least 2M+2 cells; with 2M+1 cells the neighbour range covers the whole box and every node
-    folds. Both must give the brute-force pair sets and the oracle's values."""
+    folds. Both must give the brute-force pair sets and the oracle's values. M = 4 has 81
+    neighbour cell rows, more than the 64 lanes of the pair kernel's wavefront."""
     fs = cfg2[:16]
     params = _params(vp, Ns=Ns, backend_short=vp.HIPShortRangeBackend(nsubdiv=M))
     assert math.floor(TWO_PI * M / params.rcut) == cells
