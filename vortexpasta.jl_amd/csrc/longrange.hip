@@ -1043,13 +1043,20 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
     lr_tile_keys<<<nblocks(Nq, 256), 256, 0, s>>>(Nq, c->points, g, c->lr_key);
     VPBS_TRY(radix_sort_bins(c, c->sort_lr, c->lr_key, Nq, c->ntiles, c->lr_start, c->lr_perm, s));
     launch_gather<CT>(Nq, c->lr_perm, c->points, c->charges, g, beta, c->lr_i0, (CT *)c->lr_w, s);
-    if (c->slab_interp && Np > 0) {
-        VPBS_TRY(slab_node_buffers(c));
-        slab_keys<<<nblocks(Np, 256), 256, 0, s>>>(Np, c->nodes, g, c->nd_key);
-        VPBS_TRY(radix_sort_bins(c, c->sort_nd, c->nd_key, Np, c->nslabs, c->nd_start, c->nd_perm, s));
-        launch_gather_nodes<CT>(Np, c->nd_perm, c->nodes, g, beta, c->nd_i0, (CT *)c->nd_rec, s);
-    }
     stage_end(c, ST_LRSORT, s);
+    // The slab-sorted node records are read only by the interpolation at the end: they are
+    // sorted and gathered on the short-range stream (before its own work), beside this
+    // stream's spreading, and the interpolation waits for them.
+    const bool nodes_aside = c->slab_interp && Np > 0;
+    if (nodes_aside) {
+        hipStream_t s2 = c->s_sr;
+        VPBS_TRY(slab_node_buffers(c));
+        slab_keys<<<nblocks(Np, 256), 256, 0, s2>>>(Np, c->nodes, g, c->nd_key);
+        VPBS_TRY(radix_sort_bins(c, c->sort_nd, c->nd_key, Np, c->nslabs, c->nd_start, c->nd_perm, s2));
+        launch_gather_nodes<CT>(Np, c->nd_perm, c->nodes, g, beta, c->nd_i0, (CT *)c->nd_rec, s2);
+        VPBS_HIP(c, hipGetLastError());
+        VPBS_HIP(c, hipEventRecord(c->ev_nodes, s2));
+    }
 
     CT *grid = (CT *)c->grid;
     stage_begin(c, ST_SPREAD, s);
@@ -1071,6 +1078,7 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
         stage_begin(c, ST_FFTB, s);
         if (vel) VPBS_TRY(c->slab_interp ? spectral_y_inverse(c, c->zpen) : spectral_inverse(c, c->zpen));
         stage_end(c, ST_FFTB, s);
+        if (nodes_aside) VPBS_HIP(c, hipStreamWaitEvent(s, c->ev_nodes, 0));
         stage_begin(c, ST_INTERP, s);
         if (vel) {
             if (c->slab_interp)

@@ -128,6 +128,7 @@ static void release(vpbs_ctx *c) {
         if (c->ev_end[st]) (void)hipEventDestroy(c->ev_end[st]);
     }
     if (c->ev_lr_done) (void)hipEventDestroy(c->ev_lr_done);
+    if (c->ev_nodes) (void)hipEventDestroy(c->ev_nodes);
     if (c->s_sr) (void)hipStreamDestroy(c->s_sr);
     if (c->s_lr) (void)hipStreamDestroy(c->s_lr);
     delete c;
@@ -313,7 +314,8 @@ int vpbs_create(vpbs_ctx **out, const vpbs_params *p) {
     }
     if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->s_sr, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s_lr, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_lr_done, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_lr_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_nodes, hipEventDisableTiming) != hipSuccess) {
         fail(c, VPBS_ERR_HIP, "stream/event creation failed");
         return bail(VPBS_ERR_HIP);
     }

@@ -65,6 +65,7 @@ struct vpbs_ctx {
     hipEvent_t ev_beg[vpbs::ST_COUNT]{}, ev_end[vpbs::ST_COUNT]{};
     bool stage_ran[vpbs::ST_COUNT]{};
     hipEvent_t ev_lr_done = nullptr;
+    hipEvent_t ev_nodes = nullptr;  // slab-sorted node records ready (recorded on s_sr, waited on by s_lr)
     vpbs_timings timings{};
     vpbs::Box box{};
 
