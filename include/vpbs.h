@@ -109,6 +109,11 @@ typedef struct vpbs_timings {
     int64_t grid[3];     /* oversampled NUFFT grid */
     int64_t cells[3];    /* cell-list cells per direction */
     double point_data;   /* kernels of the last vpbs_set_filaments (0 after vpbs_set_points*) */
+    double node_sort;    /* slab sort and kernel-weight records of the nodes for the interpolation (runs on
+                            the short-range stream ahead of the pair sum, beside spreading) */
+    double features;     /* learned correction: node cell list and per-node features */
+    double mlp;          /* learned correction: network inference and the corrected sum */
+    double check;        /* learned correction: sampled check (converged-setting difference at the sample) */
 } vpbs_timings;
 
 typedef struct vpbs_ctx vpbs_ctx;
@@ -259,6 +264,53 @@ int vpbs_get_timings(vpbs_ctx *ctx, vpbs_timings *t);
 
 /* Oversampled grid and cell counts chosen by the context (for reporting). */
 int vpbs_get_sizes(vpbs_ctx *ctx, int64_t *grid3, int64_t *cells3);
+
+/*
+ * ---------------------------------------------------------------------------
+ * Learned correction. No counterpart in the reference: a small network
+ * predicts, per node, the difference between the velocity of a cheap splitting
+ * (small β: short cut-off, coarse grid) and that of a converged one (β = 4 on
+ * the same grid), from local geometric features. The plain path stays the
+ * judge: every learned evaluation checks the corrected velocity on a sample of
+ * nodes against the converged setting and falls back to it when the error is
+ * above tolerance.
+ *
+ * Network (vpbs_mlp): x (n_in) → [Linear → SiLU] × n_layers (width n_hidden) →
+ * Linear → y (n_out). Parameters are one flat FP32 array: for each layer
+ * l = 0..n_layers the weights W_l[out][in] (row-major) then the bias b_l[out].
+ * Inputs are normalised as (x − in_shift)·in_scale and the output is
+ * multiplied by out_scale. Training (FP32 forward, backward and Adam kernels)
+ * minimises (1/B) Σ_b Σ_c (y_bc − t_bc)²/out_scale_c² over batches of B
+ * samples. Inference runs on BF16 MFMA with FP32 accumulation; it is compiled
+ * for (n_in, n_hidden, n_layers) = (128, 128, 3), (128, 128, 2) and (32, 32, 2).
+ * Sample arrays X (n rows of n_in floats), T and Y (n rows of n_out floats)
+ * are DEVICE pointers; parameter arrays are host pointers.
+ */
+typedef struct vpbs_mlp vpbs_mlp;
+
+/* n_in and n_hidden multiples of 32, 1 ≤ n_layers ≤ 7, 1 ≤ n_out ≤ 16. */
+int vpbs_mlp_create(vpbs_mlp **net, int32_t n_in, int32_t n_hidden, int32_t n_layers, int32_t n_out,
+                    int32_t device);
+void vpbs_mlp_destroy(vpbs_mlp *net);
+const char *vpbs_mlp_last_error(const vpbs_mlp *net);
+int64_t vpbs_mlp_num_params(const vpbs_mlp *net);
+/* Set parameters and normalisation (resets the Adam state); get them back (any may be NULL). */
+int vpbs_mlp_set_params(vpbs_mlp *net, const float *params, const float *in_shift, const float *in_scale,
+                        const float *out_scale);
+int vpbs_mlp_get_params(vpbs_mlp *net, float *params, float *in_shift, float *in_scale, float *out_scale);
+/* FP32 forward pass (the training arithmetic): Y = net(X), n ≤ 2^24. */
+int vpbs_mlp_forward(vpbs_mlp *net, int64_t n, const float *X, float *Y);
+/* BF16-MFMA inference: Y = net(X). */
+int vpbs_mlp_infer(vpbs_mlp *net, int64_t n, const float *X, float *Y);
+/* Loss of all n samples as one batch (mean over samples of the summed squared normalised
+ * errors) and its gradient with respect to the parameters (host array of num_params floats;
+ * either output may be NULL). */
+int vpbs_mlp_gradient(vpbs_mlp *net, int64_t n, const float *X, const float *T, double *loss, float *grad);
+/* steps Adam steps (bias-corrected) on batches of `batch` samples drawn uniformly with
+ * replacement from the n samples by a hash of (seed, step, position); batch ≥ n uses all
+ * samples in order. losses (host, steps values, may be NULL) receives each step's batch loss. */
+int vpbs_mlp_train(vpbs_mlp *net, int64_t n, const float *X, const float *T, int32_t steps, int32_t batch, double lr,
+                   double beta1, double beta2, double eps, uint64_t seed, double *losses);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,13 @@ def vp():
     return ge.load_package()
 
 
+@pytest.fixture(autouse=True)
+def _library_matches_sources(request):
+    """GPU tests run only against a libvpbs.so built from the sources as they are now."""
+    if request.node.get_closest_marker("gpu") is not None:
+        ge.check_built()
+
+
 @pytest.fixture(scope="session")
 def O():
     return ge.load_oracle()

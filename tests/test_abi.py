@@ -9,7 +9,7 @@ ROOT = pathlib.Path(__file__).resolve().parents[1]
 
 def declared_symbols():
     text = (ROOT / "include" / "vpbs.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\**\s*(vpbs_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int64_t|int|void|const char \*)\s*\**\s*(vpbs_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_entry_points():

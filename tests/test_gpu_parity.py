@@ -137,6 +137,33 @@ def test_unclosed_lines_background_vorticity(vp, O):
     cache.close()
 
 
+def test_background_vorticity_alpha_independence(vp, O):
+    """The product's background-vorticity correction (host side of compute_on_nodes_) pinned by the
+    α-independence of test/background_vorticity.jl:58-80: a 4×4 array of unclosed lines, Ns = 32
+    against Ns = 64 (α doubles); per filament v within rtol 2e-6 and ψ within 2e-7; each run also
+    ≤ 1e-12 against the oracle."""
+    from test_oracle import background_lines, background_params
+
+    fs = background_lines(vp)
+    runs = []
+    for Ngrid in (32, 64):
+        p = background_params(vp, Ngrid)
+        cache, got = _gpu(vp, p, fs)
+        want = O.compute(_pd(vp, cache, fs), p, Q)
+        for k in Q:
+            assert rel_l2(got[k], want[k]) <= TOL64, (Ngrid, k)
+        runs.append(got)
+        cache.close()
+    b = vp.FilamentBatch(fs)
+    for k, tol in (("velocity", 2e-6), ("streamfunction", 2e-7)):
+        worst = 0.0
+        for o, n in zip(b.offs, b.Ns):
+            x, y = runs[0][k][:, o : o + n], runs[1][k][:, o : o + n]
+            worst = max(worst, np.linalg.norm(x - y) / max(np.linalg.norm(x), np.linalg.norm(y)))
+        print(f"  {k}: α-independence {worst:.2e} (tolerance {tol})", flush=True)
+        assert worst < tol
+
+
 def test_fp32_bulk_path(vp, O, cfg2):
     """FP32 bulk arithmetic against the FP64 oracle.
 

@@ -386,3 +386,77 @@ def test_band_field(vp, O):
     want = 2 * np.real(c[:, None] * np.exp(1j * ph)[None])
     assert rel_l2(O.band_field(u1, p, x, 3.0, 4.0, "exact"), want) < 1e-13
     assert np.all(O.band_field(u1, p, x, 4.0, 9.0, "exact") == 0)  # |k| = √14 < 4 is outside
+
+
+# ----------------------------------------------------------------------------
+# Background vorticity (unclosed periodic lines with a net end-to-end offset)
+
+
+def background_lines(vp, Np=64, L=TWO_PI):
+    """4×4 array of aligned periodic lines along z with identical helical perturbations
+    r(t) = e^{4πit}/100 (test/background_vorticity.jl:38-56): a net mean vorticity."""
+    xs = [L * (2 * i + 1) / 8 for i in range(4)]  # range(0, L; length = 9)[2:2:end]
+    fs = []
+    for x in xs:
+        for y in xs:
+            S = vp.define_curve(vp.PeriodicLine(r=lambda t: np.exp(4j * np.pi * np.asarray(t)) / 100), scale=L,
+                                translate=(x, y, L / 2))
+            fs.append(vp.filaments.init(S, Np))
+    return fs
+
+
+def background_params(vp, Ngrid):
+    """GaussianSplitting(rtol = 1e-6, Ns) with Γ = 2, a = 1e-8, m = 4 (background_vorticity.jl:18-25,62-64);
+    doubling Ns doubles α."""
+    spl = vp.GaussianSplitting.create(Ls=TWO_PI, rtol=1e-6, Ns=Ngrid)
+    return vp.ParamsBiotSavart.build(np.float64, Γ=2.0, a=1e-8, splitting=spl,
+                                     backend_long=vp.HIPLongRangeBackend(σ=1.5, m=4))
+
+
+def background_alpha_diffs(vp, O, fs):
+    """Per-filament max over filaments of |a - b| / max(|a|, |b|) between the Ns = 32 and
+    Ns = 64 splittings, for v and ψ (Julia's `a ≈ b` on each filament's vector of nodes)."""
+    out = []
+    for Ngrid in (32, 64):
+        p = background_params(vp, Ngrid)
+        pd = vp.add_point_charges(fs, p)
+        r = O.compute(pd, p, ("velocity", "streamfunction"))
+        out.append((pd, r))
+    b = out[0][0].batch
+    diffs = {}
+    for k in ("velocity", "streamfunction"):
+        a1, a2 = out[0][1][k], out[1][1][k]
+        worst = 0.0
+        for o, n in zip(b.offs, b.Ns):
+            x, y = a1[:, o : o + n], a2[:, o : o + n]
+            worst = max(worst, np.linalg.norm(x - y) / max(np.linalg.norm(x), np.linalg.norm(y)))
+        diffs[k] = worst
+    return diffs
+
+
+def test_background_vorticity_alpha_independence(vp, O):
+    """test/background_vorticity.jl:58-80: v within rtol 2e-6 and ψ within 2e-7 between α and 2α."""
+    fs = background_lines(vp)
+    d = background_alpha_diffs(vp, O, fs)
+    print(f"  α-independence with a mean vorticity: v {d['velocity']:.2e}, ψ {d['streamfunction']:.2e}")
+    assert d["velocity"] < 2e-6
+    assert d["streamfunction"] < 2e-7
+    # the correction is what makes ψ independent of α here: it must be large
+    p = background_params(vp, 32)
+    pd = vp.add_point_charges(fs, p)
+    shift = O.background_vorticity_psi(pd, p)
+    assert abs(shift[2]) > 1e-3 and abs(shift[0]) < 1e-12 and abs(shift[1]) < 1e-12
+    # ⟨ω⟩ = 16 Γ L ẑ / L³ and the shift is −⟨ω⟩/(4α²) (gaussian.jl:241)
+    want = -16 * 2.0 * TWO_PI / TWO_PI ** 3 / (4 * p.α ** 2)
+    assert abs(shift[2] - want) < 1e-12 * abs(want)
+
+
+@pytest.mark.parametrize("mistake", ["sign", "double", "missing"])
+def test_planted_mistake_background_vorticity(vp, O, monkeypatch, mistake):
+    """A flipped sign, a doubled factor or a missing correction breaks the α-independence of ψ."""
+    orig = O.background_vorticity_psi
+    f = {"sign": -1.0, "double": 2.0, "missing": 0.0}[mistake]
+    monkeypatch.setattr(O, "background_vorticity_psi", lambda pd, p: f * orig(pd, p))
+    d = background_alpha_diffs(vp, O, background_lines(vp))
+    assert d["streamfunction"] > 1e-4, d
+    assert d["velocity"] < 2e-6  # the velocity does not see it

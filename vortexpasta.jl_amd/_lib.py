@@ -49,6 +49,16 @@ EXPORTED = (
     "vpbs_neighbour_stats",
     "vpbs_get_timings",
     "vpbs_get_sizes",
+    "vpbs_mlp_create",
+    "vpbs_mlp_destroy",
+    "vpbs_mlp_last_error",
+    "vpbs_mlp_num_params",
+    "vpbs_mlp_set_params",
+    "vpbs_mlp_get_params",
+    "vpbs_mlp_forward",
+    "vpbs_mlp_infer",
+    "vpbs_mlp_gradient",
+    "vpbs_mlp_train",
 )
 
 
@@ -92,6 +102,10 @@ class VpbsTimings(ctypes.Structure):
         ("grid", ctypes.c_int64 * 3),
         ("cells", ctypes.c_int64 * 3),
         ("point_data", ctypes.c_double),
+        ("node_sort", ctypes.c_double),
+        ("features", ctypes.c_double),
+        ("mlp", ctypes.c_double),
+        ("check", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -157,6 +171,27 @@ def load() -> ctypes.CDLL:
     L.vpbs_get_timings.restype = ctypes.c_int
     L.vpbs_get_sizes.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.vpbs_get_sizes.restype = ctypes.c_int
+    i32 = ctypes.c_int32
+    L.vpbs_mlp_create.argtypes = [ctypes.POINTER(vp), i32, i32, i32, i32, i32]
+    L.vpbs_mlp_create.restype = ctypes.c_int
+    L.vpbs_mlp_destroy.argtypes = [vp]
+    L.vpbs_mlp_destroy.restype = None
+    L.vpbs_mlp_last_error.argtypes = [vp]
+    L.vpbs_mlp_last_error.restype = ctypes.c_char_p
+    L.vpbs_mlp_num_params.argtypes = [vp]
+    L.vpbs_mlp_num_params.restype = ctypes.c_int64
+    L.vpbs_mlp_set_params.argtypes = [vp, vp, vp, vp, vp]
+    L.vpbs_mlp_set_params.restype = ctypes.c_int
+    L.vpbs_mlp_get_params.argtypes = [vp, vp, vp, vp, vp]
+    L.vpbs_mlp_get_params.restype = ctypes.c_int
+    L.vpbs_mlp_forward.argtypes = [vp, i64, vp, vp]
+    L.vpbs_mlp_forward.restype = ctypes.c_int
+    L.vpbs_mlp_infer.argtypes = [vp, i64, vp, vp]
+    L.vpbs_mlp_infer.restype = ctypes.c_int
+    L.vpbs_mlp_gradient.argtypes = [vp, i64, vp, vp, ctypes.POINTER(d), vp]
+    L.vpbs_mlp_gradient.restype = ctypes.c_int
+    L.vpbs_mlp_train.argtypes = [vp, i64, vp, vp, i32, i32, d, d, d, d, ctypes.c_uint64, vp]
+    L.vpbs_mlp_train.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -355,3 +390,92 @@ class Context:
         c = (ctypes.c_int64 * 3)()
         self._check(self._L.vpbs_get_sizes(self._h, g, c), "vpbs_get_sizes")
         return tuple(g), tuple(c)
+
+
+class MLP:
+    """Owns one vpbs_mlp (network of the learned correction) on one GPU.
+
+    Sample arrays are torch CUDA tensors (float32, contiguous): torch provides
+    the device memory, the library does the arithmetic.
+    """
+
+    def __init__(self, n_in: int, n_hidden: int, n_layers: int, n_out: int, device: int = 0):
+        self._L = load()
+        self._h = ctypes.c_void_p()
+        rc = self._L.vpbs_mlp_create(ctypes.byref(self._h), n_in, n_hidden, n_layers, n_out, device)
+        if rc != 0:
+            raise VpbsError(f"vpbs_mlp_create failed ({rc}): invalid sizes or device")
+        self.n_in, self.H, self.nl, self.n_out = n_in, n_hidden, n_layers, n_out
+        self.nparams = int(self._L.vpbs_mlp_num_params(self._h))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise VpbsError(f"{what} failed ({rc}): {self._L.vpbs_mlp_last_error(self._h).decode()}")
+
+    def close(self):
+        if self._h:
+            self._L.vpbs_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params, in_shift, in_scale, out_scale):
+        a = [np.ascontiguousarray(x, dtype=np.float32) for x in (params, in_shift, in_scale, out_scale)]
+        if a[0].size != self.nparams or a[1].size != self.n_in or a[2].size != self.n_in or a[3].size != self.n_out:
+            raise ValueError("parameter or normalisation sizes do not match the network")
+        self._check(self._L.vpbs_mlp_set_params(self._h, *(_ptr(x) for x in a)), "vpbs_mlp_set_params")
+
+    def get_params(self):
+        out = (np.empty(self.nparams, np.float32), np.empty(self.n_in, np.float32), np.empty(self.n_in, np.float32),
+               np.empty(self.n_out, np.float32))
+        self._check(self._L.vpbs_mlp_get_params(self._h, *(_ptr(x) for x in out)), "vpbs_mlp_get_params")
+        return out
+
+    @staticmethod
+    def _dev(t, cols):
+        import torch
+
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+                and t.dim() == 2 and t.shape[1] == cols):
+            raise ValueError(f"expected a contiguous float32 CUDA tensor of shape (n, {cols})")
+        return ctypes.c_void_p(t.data_ptr())
+
+    def _out(self, n):
+        import torch
+
+        return torch.empty((n, self.n_out), dtype=torch.float32, device="cuda")
+
+    def forward(self, X):
+        Y = self._out(X.shape[0])
+        self._check(self._L.vpbs_mlp_forward(self._h, X.shape[0], self._dev(X, self.n_in), self._dev(Y, self.n_out)),
+                    "vpbs_mlp_forward")
+        return Y
+
+    def infer(self, X):
+        Y = self._out(X.shape[0])
+        self._check(self._L.vpbs_mlp_infer(self._h, X.shape[0], self._dev(X, self.n_in), self._dev(Y, self.n_out)),
+                    "vpbs_mlp_infer")
+        return Y
+
+    def gradient(self, X, T):
+        g = np.empty(self.nparams, np.float32)
+        loss = ctypes.c_double()
+        self._check(self._L.vpbs_mlp_gradient(self._h, X.shape[0], self._dev(X, self.n_in), self._dev(T, self.n_out),
+                                              ctypes.byref(loss), _ptr(g)), "vpbs_mlp_gradient")
+        return loss.value, g
+
+    def train(self, X, T, steps: int, batch: int, lr: float, beta1=0.9, beta2=0.999, eps=1e-8, seed: int = 0,
+              record_losses: bool = True):
+        losses = np.empty(steps) if record_losses and steps > 0 else None
+        self._check(self._L.vpbs_mlp_train(self._h, X.shape[0], self._dev(X, self.n_in), self._dev(T, self.n_out),
+                                           int(steps), int(batch), float(lr), float(beta1), float(beta2), float(eps),
+                                           ctypes.c_uint64(seed), _ptr(losses)), "vpbs_mlp_train")
+        return losses

@@ -734,7 +734,8 @@ __global__ void fourier_kernel(FourierArgs<CT> a, CT *__restrict__ spec, CT *__r
 // z-pencils of the band-filtered field, ready for the inverse transforms: the
 // kept û on kmin² ≤ k² ≤ kmax² times the type-2 deconvolution, zero elsewhere.
 template <typename CT>
-__global__ void band_to_zpencils(FourierArgs<CT> a, double kmin2, double kmax2, CT *__restrict__ zp) {
+__global__ void band_to_zpencils(FourierArgs<CT> a, double kmin2, double kmax2, double cf, double cc,
+                                 CT *__restrict__ zp) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= a.cstride) return;
     const int iz = (int)(idx % a.G[2]);
@@ -752,7 +753,8 @@ __global__ void band_to_zpencils(FourierArgs<CT> a, double kmin2, double kmax2, 
         for (int d = 0; d < 3; d++) o[d * cs] = o[d * cs + 1] = (CT)0;
         return;
     }
-    const double D = a.dx[kx] * a.dy[ky + a.kmax[1]] * a.dz[kz + a.kmax[2]];
+    double D = a.dx[kx] * a.dy[ky + a.kmax[1]] * a.dz[kz + a.kmax[2]];
+    if (cf > 0) D *= exp(-cf * k2) - exp(-cc * k2);  // difference of two Ewald Gaussians
     const int jz = kz >= 0 ? kz : kz + a.nkz;
     const CT *u = a.uhat + 2 * (((int64_t)j * a.nkz + jz) * a.nkx + kx);
 #pragma unroll
@@ -1051,9 +1053,11 @@ int launch_lr(vpbs_ctx *c, uint32_t flags) {
     if (nodes_aside) {
         hipStream_t s2 = c->s_sr;
         VPBS_TRY(slab_node_buffers(c));
+        stage_begin(c, ST_NDSORT, s2);
         slab_keys<<<nblocks(Np, 256), 256, 0, s2>>>(Np, c->nodes, g, c->nd_key);
         VPBS_TRY(radix_sort_bins(c, c->sort_nd, c->nd_key, Np, c->nslabs, c->nd_start, c->nd_perm, s2));
         launch_gather_nodes<CT>(Np, c->nd_perm, c->nodes, g, beta, c->nd_i0, (CT *)c->nd_rec, s2);
+        stage_end(c, ST_NDSORT, s2);
         VPBS_HIP(c, hipGetLastError());
         VPBS_HIP(c, hipEventRecord(c->ev_nodes, s2));
     }
@@ -1265,17 +1269,26 @@ int longrange_compute(vpbs_ctx *c, uint32_t flags) {
 
 namespace {
 
+// Kept field on kmin² ≤ k² ≤ kmax² (weighted by e^{−cf·k²} − e^{−cc·k²} when cf > 0) on the
+// real grid.
 template <typename CT>
-int band_impl(vpbs_ctx *c, double kmin, double kmax, double *v_out) {
+int band_grid(vpbs_ctx *c, double kmin2, double kmax2, double cf, double cc) {
     hipStream_t s = c->s_lr;
     if (c->fft_fused) {
-        VPBS_TRY(spectral_band(c, kmin * kmin, kmax * kmax));
+        VPBS_TRY(spectral_band(c, kmin2, kmax2, cf, cc));
     } else {
         FourierArgs<CT> fa = fourier_args<CT>(c);
-        band_to_zpencils<CT><<<nblocks(fa.cstride, 256), 256, 0, s>>>(fa, kmin * kmin, kmax * kmax, (CT *)c->zpen);
+        band_to_zpencils<CT><<<nblocks(fa.cstride, 256), 256, 0, s>>>(fa, kmin2, kmax2, cf, cc, (CT *)c->zpen);
         VPBS_HIP(c, hipGetLastError());
         VPBS_TRY(inverse_to_grid<CT>(c, (Cx<CT> *)c->zpen));
     }
+    return VPBS_OK;
+}
+
+template <typename CT>
+int band_impl(vpbs_ctx *c, double kmin, double kmax, double *v_out) {
+    hipStream_t s = c->s_lr;
+    VPBS_TRY(band_grid<CT>(c, kmin * kmin, kmax * kmax, 0.0, 0.0));
     launch_interp<CT>(lr_geom(c), c->Np, c->nodes, (CT)c->es_beta, (const CT *)c->grid, c->v_lr, s);
     VPBS_HIP(c, hipGetLastError());
     VPBS_HIP(c, hipMemcpyAsync(v_out, c->v_lr, sizeof(double) * 3 * (size_t)c->Np, hipMemcpyDeviceToHost, s));
@@ -1358,6 +1371,23 @@ int longrange_band(vpbs_ctx *c, double kmin, double kmax, double *v_out) {
     if (!(kmin >= 0) || !(kmax >= kmin)) return fail(c, VPBS_ERR_INVALID, "expected 0 <= kmin <= kmax");
     if (!v_out) return fail(c, VPBS_ERR_INVALID, "null output");
     return c->p.precision == VPBS_F32 ? band_impl<float>(c, kmin, kmax, v_out) : band_impl<double>(c, kmin, kmax, v_out);
+}
+
+int longrange_filtered_grid(vpbs_ctx *c, double cf, double cc) {
+    if (!c->uhat_quantity)
+        return fail(c, VPBS_ERR_STATE, "no Fourier field kept: run vpbs_compute with VPBS_KEEP_UHAT (and the long range)");
+    const double inf = HUGE_VAL;
+    return c->p.precision == VPBS_F32 ? band_grid<float>(c, 0.0, inf, cf, cc) : band_grid<double>(c, 0.0, inf, cf, cc);
+}
+
+int longrange_interp_at(vpbs_ctx *c, int64_t n, const double *pos, double *out) {
+    if (n <= 0) return VPBS_OK;
+    if (c->p.precision == VPBS_F32)
+        launch_interp<float>(lr_geom(c), n, pos, (float)c->es_beta, (const float *)c->grid, out, c->s_lr);
+    else
+        launch_interp<double>(lr_geom(c), n, pos, c->es_beta, (const double *)c->grid, out, c->s_lr);
+    VPBS_HIP(c, hipGetLastError());
+    return VPBS_OK;
 }
 
 void longrange_release(vpbs_ctx *c) {

@@ -835,9 +835,11 @@ __global__ __launch_bounds__(NTZ) void spec_z_fourier(SpecGeom g, SpecFourier f,
 
 // Band-filtered kept field in z space: the kept û on kmin² ≤ k² ≤ kmax² times the
 // type-2 deconvolution, zero elsewhere, inverse FFT_z → x2. Tile = (jy, kx-block).
+// With cf > 0 the band is further weighted by e^{−cf·k²} − e^{−cc·k²} (difference of two
+// Ewald Gaussians, cf = 1/4α₁², cc = 1/4α₂²).
 template <typename CT, int LS>
-__global__ __launch_bounds__(NTZ) void spec_z_band(SpecGeom g, SpecFourier f, double kmin2, double kmax2,
-                                                   C2<CT> *__restrict__ x2, const C2<CT> *__restrict__ tw) {
+__global__ __launch_bounds__(NTZ) void spec_z_band(SpecGeom g, SpecFourier f, double kmin2, double kmax2, double cf,
+                                                   double cc, C2<CT> *__restrict__ x2, const C2<CT> *__restrict__ tw) {
     const int L = LS ? LS : g.G[2];
     const int lgB = LS ? tile_lgb<CT>(LS, KIND_Z) : g.lgB[2];
     const int B = 1 << lgB;
@@ -867,7 +869,8 @@ __global__ __launch_bounds__(NTZ) void spec_z_band(SpecGeom g, SpecFourier f, do
                 const double KX = f.kscale[0] * kx, KZ = f.kscale[2] * kz;
                 const double k2 = KX * KX + KY * KY + KZ * KZ;
                 if (k2 >= kmin2 && k2 <= kmax2) {
-                    const double D = f.dx[kx] * f.dy[ky + K1] * f.dz[kz + K2];
+                    double D = f.dx[kx] * f.dy[ky + K1] * f.dz[kz + K2];
+                    if (cf > 0) D *= exp(-cf * k2) - exp(-cc * k2);
                     const int jz = kz >= 0 ? kz : kz + nkz;
                     const C2<CT> u = uh[c * f.ucstride + ((int64_t)jy * nkz + jz) * nkx + kx];
                     v = C2<CT>{(CT)((double)u.re * D), (CT)((double)u.im * D)};
@@ -1435,13 +1438,13 @@ int inverse_impl(vpbs_ctx *c, void *zp) {
 }
 
 template <typename CT>
-int band_impl(vpbs_ctx *c, double kmin2, double kmax2) {
+int band_impl(vpbs_ctx *c, double kmin2, double kmax2, double cf, double cc) {
     hipStream_t s = c->s_lr;
     const SpecGeom g = spec_geom<CT>(c);
     const SpecFourier f = spec_fourier_args<CT>(c);
     const int64_t nkb = (c->nkx + (1 << g.lgB[2]) - 1) >> g.lgB[2];
     SPEC_LAUNCH(c->G[2], spec_z_band, grid8((int64_t)c->nky * nkb), NTZ, spec_lds<CT>(g, 2),
-                g, f, kmin2, kmax2, (C2<CT> *)c->zpen, twiddles<CT>(c, 2));
+                g, f, kmin2, kmax2, cf, cc, (C2<CT> *)c->zpen, twiddles<CT>(c, 2));
     VPBS_HIP(c, hipGetLastError());
     return inverse_impl<CT>(c, c->zpen);
 }
@@ -1561,8 +1564,9 @@ int spectral_inverse(vpbs_ctx *c, void *zp) {
     return c->p.precision == VPBS_F32 ? inverse_impl<float>(c, zp) : inverse_impl<double>(c, zp);
 }
 
-int spectral_band(vpbs_ctx *c, double kmin2, double kmax2) {
-    return c->p.precision == VPBS_F32 ? band_impl<float>(c, kmin2, kmax2) : band_impl<double>(c, kmin2, kmax2);
+int spectral_band(vpbs_ctx *c, double kmin2, double kmax2, double cf, double cc) {
+    return c->p.precision == VPBS_F32 ? band_impl<float>(c, kmin2, kmax2, cf, cc)
+                                      : band_impl<double>(c, kmin2, kmax2, cf, cc);
 }
 
 void spectral_release(vpbs_ctx *c) {

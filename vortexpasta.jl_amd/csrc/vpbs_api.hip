@@ -41,6 +41,30 @@ void stage_end(vpbs_ctx *c, Stage st, hipStream_t s) { (void)hipEventRecord(c->e
 
 int merge_outputs(vpbs_ctx *c, const double *a, const double *b, double *o, int64_t n, hipStream_t s);
 
+StreamDrain::~StreamDrain() {
+    if (c->s_sr) (void)hipStreamSynchronize(c->s_sr);
+    if (c->s_lr) (void)hipStreamSynchronize(c->s_lr);
+}
+
+// Stage times of the last compute (ms; 0 for stages that did not run).
+void collect_timings(vpbs_ctx *c) {
+    vpbs_timings &T = c->timings;
+    struct {
+        Stage st;
+        double *slot;
+    } tab[] = {{ST_LOCAL, &T.local_term}, {ST_CELL, &T.cell_build}, {ST_PAIR, &T.pair},
+               {ST_SELF, &T.self},        {ST_LRSORT, &T.lr_sort},   {ST_SPREAD, &T.spread},
+               {ST_FFTF, &T.fft_forward}, {ST_FOURIER, &T.fourier},  {ST_FFTB, &T.fft_backward},
+               {ST_INTERP, &T.interp},    {ST_MERGE, &T.merge},      {ST_NDSORT, &T.node_sort},
+               {ST_FEATURES, &T.features}, {ST_MLP, &T.mlp},         {ST_CHECK, &T.check}};
+    for (auto &e : tab) {
+        float ms = 0.f;
+        *e.slot = 0.0;
+        if (c->stage_ran[e.st] && hipEventElapsedTime(&ms, c->ev_beg[e.st], c->ev_end[e.st]) == hipSuccess)
+            *e.slot = ms;
+    }
+}
+
 static int validate(const vpbs_params *p, std::string &msg) {
     char buf[512];
     if (p->abi_version != VPBS_ABI_VERSION) {
@@ -403,8 +427,11 @@ int vpbs_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff, double 
     if (!std::isnan(integration_cutoff) && !(integration_cutoff > 0))
         return fail(c, VPBS_ERR_INVALID, "integration_cutoff must be positive (or NaN for the default)");
     VPBS_HIP(c, hipSetDevice(c->device));
+    // Both streams are drained on every return, the early (error) returns included, so that
+    // no kernel of this call still reads the point data when the caller replaces it.
+    StreamDrain drain{c};
     auto t0 = std::chrono::steady_clock::now();
-    for (int st = 0; st <= ST_MERGE; st++) c->stage_ran[st] = false;
+    for (int st = 0; st < ST_POINTS; st++) c->stage_ran[st] = false;
     const bool with_sr = !(flags & VPBS_NO_SHORTRANGE);
     const bool with_lr = !(flags & VPBS_NO_LONGRANGE) && !(flags & VPBS_LIA_ONLY);
     const int64_t n3 = 3 * c->Np;
@@ -447,17 +474,9 @@ int vpbs_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff, double 
     c->last_psi = psi;
     auto t1 = std::chrono::steady_clock::now();
 
-    // timings
+    collect_timings(c);
+    c->timings.total = std::chrono::duration<double, std::milli>(t1 - t0).count();
     vpbs_timings &T = c->timings;
-    double *slots[ST_MERGE + 1] = {&T.local_term, &T.cell_build, &T.pair,  &T.self,   &T.lr_sort, &T.spread,
-                                   &T.fft_forward, &T.fourier,   &T.fft_backward, &T.interp, &T.merge};
-    for (int st = 0; st <= ST_MERGE; st++) {
-        float ms = 0.f;
-        *slots[st] = 0.0;
-        if (c->stage_ran[st] && hipEventElapsedTime(&ms, c->ev_beg[st], c->ev_end[st]) == hipSuccess)
-            *slots[st] = ms;
-    }
-    T.total = std::chrono::duration<double, std::milli>(t1 - t0).count();
     T.pair_candidates = 0;
     if (with_sr && (flags & VPBS_NEIGHBOUR_STATS) && !(flags & VPBS_LIA_ONLY) && c->Np > 0) {
         std::vector<int32_t> nc((size_t)c->Np);

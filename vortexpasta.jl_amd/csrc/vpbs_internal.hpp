@@ -51,7 +51,11 @@ enum Stage {
     ST_FFTB,
     ST_INTERP,
     ST_MERGE,
-    ST_POINTS,  // vpbs_set_filaments (not part of vpbs_compute)
+    ST_NDSORT,   // slab sort and records of the nodes (short-range stream, read by the interpolation)
+    ST_FEATURES, // learned correction: node cell list and feature kernel
+    ST_MLP,      // learned correction: network inference and the corrected sum
+    ST_CHECK,    // learned correction: sampled check against the converged setting
+    ST_POINTS,   // vpbs_set_filaments (not part of vpbs_compute)
     ST_COUNT
 };
 
@@ -235,10 +239,15 @@ bool spectral_slab_supported(const vpbs_ctx *c);
 int spectral_slab_slots(const vpbs_ctx *c);
 int spectral_y_inverse(vpbs_ctx *c, void *zp);
 int spectral_x_inverse_interp(vpbs_ctx *c, double *out);
-int spectral_band(vpbs_ctx *c, double kmin2, double kmax2);
+int spectral_band(vpbs_ctx *c, double kmin2, double kmax2, double cf = 0, double cc = 0);
 void spectral_release(vpbs_ctx *c);
 int longrange_spectrum(vpbs_ctx *c, int64_t nk, double dk, double *Ek);
 int longrange_band(vpbs_ctx *c, double kmin, double kmax, double *v_out);
+// The kept field filtered by (e^{−cf·k²} − e^{−cc·k²}) on every retained mode, transformed
+// to the real grid (long-range stream); then interpolated at n positions pos[3][n] (device,
+// folded) into out[3][n] (device, FP64).
+int longrange_filtered_grid(vpbs_ctx *c, double cf, double cc);
+int longrange_interp_at(vpbs_ctx *c, int64_t n, const double *pos, double *out);
 void longrange_release(vpbs_ctx *c);
 int oversampled_size(int N, double sigma);
 double es_beta(int m, double sigma);
@@ -259,5 +268,11 @@ int device_check(vpbs_ctx *c);
 // helpers
 void stage_begin(vpbs_ctx *c, Stage st, hipStream_t s);
 void stage_end(vpbs_ctx *c, Stage st, hipStream_t s);
+void collect_timings(vpbs_ctx *c);
+// Synchronises both streams of the context when it goes out of scope.
+struct StreamDrain {
+    vpbs_ctx *c;
+    ~StreamDrain();
+};
 
 }  // namespace vpbs
