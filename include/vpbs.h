@@ -312,6 +312,40 @@ int vpbs_mlp_gradient(vpbs_mlp *net, int64_t n, const float *X, const float *T, 
 int vpbs_mlp_train(vpbs_mlp *net, int64_t n, const float *X, const float *T, int32_t steps, int32_t batch, double lr,
                    double beta1, double beta2, double eps, uint64_t seed, double *losses);
 
+/*
+ * The learned correction on a context. The context's own splitting (alpha, rcut, Ns) is the
+ * cheap one; alpha_conv < alpha and rcut_conv > rcut define the converged splitting on the
+ * same grid (β = alpha_conv·rcut_conv = 4 is converged to ~6e-9). feature_radius: radius of
+ * the strands the features describe. Every vpbs_compute_learned checks the corrected
+ * velocity on ⌈sample_fraction·Np⌉ nodes (a stratified random sample, seeded by seed and the
+ * call count) against the converged splitting and, when the relative L2 error exceeds tol,
+ * rejects the correction and evaluates the converged splitting on every node instead. net
+ * (128 inputs, 3 outputs) is borrowed and must outlive its use; it may be NULL for the
+ * training helpers below. Requires filaments from vpbs_set_filaments.
+ */
+typedef struct vpbs_learned_report {
+    double err_corrected;   /* sampled relative L2 error of the corrected velocity vs the converged splitting */
+    double err_uncorrected; /* the same for the cheap velocity alone */
+    int64_t nsample;        /* nodes in the sample (0: no check) */
+    int32_t fallback;       /* 1: correction rejected, the converged splitting was evaluated on every node */
+    double total_ms;        /* wall time of the call */
+} vpbs_learned_report;
+
+int vpbs_learned_setup(vpbs_ctx *ctx, vpbs_mlp *net, double alpha_conv, double rcut_conv, double feature_radius,
+                       double sample_fraction, double tol, uint64_t seed);
+/* Velocity of the cheap splitting plus the network's correction, checked as above. v_out:
+ * host array of 3*Np doubles or NULL (result left on the device, see vpbs_output_device_ptrs). */
+int vpbs_compute_learned(vpbs_ctx *ctx, double *v_out, vpbs_learned_report *report);
+/* Training helpers (after a vpbs_compute of the current points; pointers host or device).
+ * Features of every node: X[Np][128] (6 + 6·16 used: curvature vector/α and tangent in the node's
+ * frame, then for the 16 nearest strands the foot of the perpendicular on the strand's tangent
+ * line (×α) and the strand's tangent, in the frame); frame[Np][9] = (t, n, b). */
+int vpbs_learned_features(vpbs_ctx *ctx, float *X, float *frame);
+/* Converged − cheap velocity on every node (needs VPBS_VELOCITY | VPBS_KEEP_UHAT on the last
+ * vpbs_compute): dv[3*Np] (FP64) and/or T[Np][3] = its components along (t, n, b) in units of
+ * Γ·alpha/4π (the network's target). */
+int vpbs_learned_difference(vpbs_ctx *ctx, double *dv, float *T);
+
 #ifdef __cplusplus
 }
 #endif

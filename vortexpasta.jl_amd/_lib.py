@@ -59,6 +59,10 @@ EXPORTED = (
     "vpbs_mlp_infer",
     "vpbs_mlp_gradient",
     "vpbs_mlp_train",
+    "vpbs_learned_setup",
+    "vpbs_compute_learned",
+    "vpbs_learned_features",
+    "vpbs_learned_difference",
 )
 
 
@@ -107,6 +111,19 @@ class VpbsTimings(ctypes.Structure):
         ("mlp", ctypes.c_double),
         ("check", ctypes.c_double),
     ]
+
+
+class VpbsLearnedReport(ctypes.Structure):
+    _fields_ = [
+        ("err_corrected", ctypes.c_double),
+        ("err_uncorrected", ctypes.c_double),
+        ("nsample", ctypes.c_int64),
+        ("fallback", ctypes.c_int32),
+        ("total_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
@@ -192,6 +209,14 @@ def load() -> ctypes.CDLL:
     L.vpbs_mlp_gradient.restype = ctypes.c_int
     L.vpbs_mlp_train.argtypes = [vp, i64, vp, vp, i32, i32, d, d, d, d, ctypes.c_uint64, vp]
     L.vpbs_mlp_train.restype = ctypes.c_int
+    L.vpbs_learned_setup.argtypes = [vp, vp, d, d, d, d, d, ctypes.c_uint64]
+    L.vpbs_learned_setup.restype = ctypes.c_int
+    L.vpbs_compute_learned.argtypes = [vp, vp, ctypes.POINTER(VpbsLearnedReport)]
+    L.vpbs_compute_learned.restype = ctypes.c_int
+    L.vpbs_learned_features.argtypes = [vp, vp, vp]
+    L.vpbs_learned_features.restype = ctypes.c_int
+    L.vpbs_learned_difference.argtypes = [vp, vp, vp]
+    L.vpbs_learned_difference.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -384,6 +409,35 @@ class Context:
         t = VpbsTimings()
         self._check(self._L.vpbs_get_timings(self._h, ctypes.byref(t)), "vpbs_get_timings")
         return t.as_dict()
+
+    # learned correction
+    def learned_setup(self, net, alpha_conv: float, rcut_conv: float, feature_radius: float,
+                      sample_fraction: float = 0.01, tol: float = 1e-4, seed: int = 0):
+        h = net.handle if net is not None else None
+        self._check(self._L.vpbs_learned_setup(self._h, h, float(alpha_conv), float(rcut_conv), float(feature_radius),
+                                               float(sample_fraction), float(tol), ctypes.c_uint64(seed)),
+                    "vpbs_learned_setup")
+
+    def compute_learned(self, fetch=True):
+        """Returns (v (3, Np) or None, report dict)."""
+        v = np.empty((3, self.Np)) if fetch else None
+        rep = VpbsLearnedReport()
+        self._check(self._L.vpbs_compute_learned(self._h, _ptr(v), ctypes.byref(rep)), "vpbs_compute_learned")
+        return v, rep.as_dict()
+
+    def learned_features(self, X=None, frame=None):
+        """Features into X (numpy (Np, 128) float32, or a CUDA tensor) and frames (Np, 9)."""
+        px = ctypes.c_void_p(X.data_ptr()) if hasattr(X, "data_ptr") else _ptr(X)
+        pf = ctypes.c_void_p(frame.data_ptr()) if hasattr(frame, "data_ptr") else _ptr(frame)
+        self._check(self._L.vpbs_learned_features(self._h, px, pf), "vpbs_learned_features")
+        return X, frame
+
+    def learned_difference(self, dv=None, T=None):
+        """Converged − cheap velocity: dv (3, Np) float64 and/or targets T (Np, 3) float32 (numpy or CUDA)."""
+        pd_ = ctypes.c_void_p(dv.data_ptr()) if hasattr(dv, "data_ptr") else _ptr(dv)
+        pt = ctypes.c_void_p(T.data_ptr()) if hasattr(T, "data_ptr") else _ptr(T)
+        self._check(self._L.vpbs_learned_difference(self._h, pd_, pt), "vpbs_learned_difference")
+        return dv, T
 
     def sizes(self):
         g = (ctypes.c_int64 * 3)()

@@ -1,0 +1,47 @@
+// Device helpers shared by the short-range and learned-correction kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace vpbs {
+
+// Minimum-image separation along one period (Filaments/min_distance.jl:54-66): two
+// selects, ≥ L/2 → −L, < −L/2 → +L.
+__device__ inline double fold_sep(double r, double L, double Lh) {
+    r = (r >= Lh) ? r - L : r;
+    r = (r < -Lh) ? r + L : r;
+    return r;
+}
+
+// r² rounded exactly as (x² + y²) + z² (no contraction), as in the reference.
+__device__ inline double dist2_nocontract(double a, double b, double c) {
+#pragma clang fp contract(off)
+    double s = a * a + b * b;
+    return s + c * c;
+}
+
+__device__ inline int cell_index(double x, double L, int N) { return (int)((x / L) * (double)N); }
+
+// Inclusive prefix sum over the 64 lanes of a wavefront with DPP row operations: shifts by
+// 1, 2, 4 and 8 lanes within each row of 16, then the last lane of row 0 (row 2) added to
+// row 1 (row 3), then lane 31 added to rows 2 and 3. Lanes a shift does not reach add 0.
+__device__ inline int wave_incl_scan(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return v;
+}
+
+__device__ inline uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+}  // namespace vpbs

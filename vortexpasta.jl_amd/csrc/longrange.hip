@@ -1262,8 +1262,10 @@ int longrange_compute(vpbs_ctx *c, uint32_t flags) {
     if (keep && !c->uhat) VPBS_HIP(c, hipMalloc(&c->uhat, 2 * es * 3 * (size_t)c->uhat_stride));
     c->uhat_quantity = 0;
     const int rc = c->p.precision == VPBS_F32 ? launch_lr<float>(c, flags) : launch_lr<double>(c, flags);
-    if (rc == VPBS_OK && keep)
+    if (rc == VPBS_OK && keep) {
         c->uhat_quantity = (flags & VPBS_VELOCITY) ? VPBS_UHAT_VELOCITY : VPBS_UHAT_STREAMFUNCTION;
+        c->uhat_version = c->points_version;
+    }
     return rc;
 }
 

@@ -508,6 +508,7 @@ int point_data_from_state(vpbs_ctx *c) {
     VPBS_HIP(c, hipEventElapsedTime(&ms, c->ev_beg[ST_POINTS], c->ev_end[ST_POINTS]));
     c->timings.point_data = ms;
     c->have_points = true;
+    c->points_version++;
     c->have_stats = false;
     c->last_vel = c->last_psi = false;
     return VPBS_OK;

@@ -17,6 +17,7 @@
 // semantics, params.jl:289-298), which avoids the FP32 cancellation of the
 // singular local integrals.
 #include "vpbs_internal.hpp"
+#include "device_util.hpp"
 
 #include <cmath>
 
@@ -26,20 +27,6 @@ namespace {
 
 constexpr double TWO_OVER_SQRT_PI = 1.1283791670955126;
 
-__device__ inline double fold_sep(double r, double L, double Lh) {
-    r = (r >= Lh) ? r - L : r;
-    r = (r < -Lh) ? r + L : r;
-    return r;
-}
-
-// r² rounded exactly as (x² + y²) + z² (no contraction), as in the reference.
-__device__ inline double dist2_nocontract(double a, double b, double c) {
-#pragma clang fp contract(off)
-    double s = a * a + b * b;
-    return s + c * c;
-}
-
-__device__ inline int cell_index(double x, double L, int N) { return (int)((x / L) * (double)N); }
 
 struct Kahan {
     float s = 0.f, c = 0.f;
@@ -150,12 +137,6 @@ struct PairArgs {
     int32_t *cands;  // stats: candidate sources tested per node
 };
 
-__device__ inline uint64_t mix64(uint64_t x) {
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-}
 
 template <typename CT>
 struct Acc3;
@@ -190,18 +171,6 @@ __device__ inline void rinv_r(float r2, float &rinv, float &r) {
     r = r2 * rinv;
 }
 
-// Inclusive prefix sum over the 64 lanes of a wavefront with DPP row operations: shifts by
-// 1, 2, 4 and 8 lanes within each row of 16, then the last lane of row 0 (row 2) added to
-// row 1 (row 3), then lane 31 added to rows 2 and 3. Lanes a shift does not reach add 0.
-__device__ inline int wave_incl_scan(int v) {
-    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
-    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
-    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
-    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
-    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
-    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
-    return v;
-}
 
 // One wavefront per target node (cell-sorted order; XCD-aware block order):
 //  1) lanes build the source runs of the (2M+1)² neighbour cell rows (≤ 2 contiguous
@@ -530,11 +499,9 @@ int run_pairs(vpbs_ctx *c, bool vel, bool psi, bool stats, bool include_local) {
     VPBS_TRY(counting_sort(c, c->sort_sr, c->src_key, Nq, c->ncells, c->src_start, c->src_perm, s));
     gather_sources<CT><<<nblocks(Nq, 256), 256, 0, s>>>(Nq, c->src_perm, c->points, c->charges, c->src_pos,
                                                           (CT *)c->src_chg, c->src_idx);
-    cell_keys<<<nblocks(Np, 256), 256, 0, s>>>(Np, c->nodes, c->box, Nc, c->tgt_key, (int4 *)c->tgt_cell);
-    // targets only set the pair kernel's processing order (each node's sum is independent of
-    // it): no per-cell index sort
-    VPBS_TRY(counting_sort(c, c->sort_tg, c->tgt_key, Np, c->ncells, c->tgt_start, c->tgt_perm, s, false));
+    VPBS_TRY(build_target_cells(c, s));
     stage_end(c, ST_CELL, s);
+    c->cells_version = c->points_version;
 
     PairArgs<CT> a;
     a.Np = Np;
@@ -573,6 +540,17 @@ int run_pairs(vpbs_ctx *c, bool vel, bool psi, bool stats, bool include_local) {
 }
 
 }  // namespace
+
+// Nodes by cell of the short-range cell list. The order inside a cell is that of the atomic
+// ranks: the pair kernel's sums do not depend on it (it only sets the processing order), and
+// the feature kernel breaks distance ties by node index.
+int build_target_cells(vpbs_ctx *c, hipStream_t s) {
+    const int64_t Np = c->Np;
+    const int3 Nc = make_int3(c->Nc[0], c->Nc[1], c->Nc[2]);
+    cell_keys<<<nblocks(Np, 256), 256, 0, s>>>(Np, c->nodes, c->box, Nc, c->tgt_key, (int4 *)c->tgt_cell);
+    VPBS_TRY(counting_sort(c, c->sort_tg, c->tgt_key, Np, c->ncells, c->tgt_start, c->tgt_perm, s, false));
+    return VPBS_OK;
+}
 
 int shortrange_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff) {
     const int64_t Np = c->Np;

@@ -134,6 +134,7 @@ static void release(vpbs_ctx *c) {
     if (c->s_sr) (void)hipStreamSynchronize(c->s_sr);
     if (c->s_lr) (void)hipStreamSynchronize(c->s_lr);
     longrange_release(c);
+    learned_release(c);
     void *bufs[] = {c->nodes,     c->sp,        c->spp,      c->dlm,       c->dlp,        c->idx_prev,
                     c->points,    c->charges,   c->v_sr,     c->psi_sr,    c->v_lr,       c->psi_lr,
                     c->v_out,     c->psi_out,   c->src_key,  c->src_start, c->src_perm,   c->tgt_key,   c->tgt_cell,
@@ -291,6 +292,7 @@ static int set_points_impl(vpbs_ctx *c, int64_t Np, const double *nodes, const i
     c->have_filaments = false;
     if (kind == hipMemcpyDeviceToDevice) VPBS_TRY(device_check(c));
     c->have_points = true;
+    c->points_version++;
     c->have_stats = false;
     c->last_vel = c->last_psi = false;
     return VPBS_OK;
@@ -444,6 +446,8 @@ int vpbs_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff, double 
         VPBS_TRY(shortrange_compute(c, flags, integration_cutoff));
         // BiotSavart.jl:570-576: with the short range, unless LIA = Val(:only)
         if (!(flags & VPBS_LIA_ONLY)) VPBS_TRY(add_local_integrals(c, flags, c->s_sr));
+        // learned correction (vpbs_compute_learned): features and inference beside the long range
+        if (c->lc_active) VPBS_TRY(learned_after_shortrange(c));
     }
     if (with_lr) VPBS_HIP(c, hipEventRecord(c->ev_lr_done, c->s_lr));
 
@@ -467,6 +471,7 @@ int vpbs_compute(vpbs_ctx *c, uint32_t flags, double integration_cutoff, double 
             VPBS_HIP(c, hipMemsetAsync(f.out, 0, sizeof(double) * n3, s));
     }
     stage_end(c, ST_MERGE, s);
+    if (c->lc_active && vel && with_sr) VPBS_TRY(learned_after_merge(c));
     VPBS_HIP(c, hipStreamSynchronize(s));
     VPBS_HIP(c, hipStreamSynchronize(c->s_lr));
     VPBS_HIP(c, hipGetLastError());

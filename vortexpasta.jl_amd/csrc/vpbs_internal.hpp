@@ -172,6 +172,28 @@ struct vpbs_ctx {
     size_t spec_lds_max = 0;    // LDS per block available to the fused kernels
     void *spec_tw = nullptr;
     double *spec_fac = nullptr;
+
+    // point-data generation (incremented whenever the point data changes) and the generation
+    // the cell lists and the kept Fourier field were last computed for
+    uint64_t points_version = 0, cells_version = ~0ull, uhat_version = ~0ull;
+
+    // learned correction (learned.hip)
+    vpbs_mlp *lc_net = nullptr;       // borrowed from the caller
+    bool lc_ready = false, lc_active = false;
+    double lc_alpha_c = 0, lc_alpha_f = 0, lc_rcut_f = 0, lc_rfeat = 0, lc_frac = 0, lc_tol = 0;
+    uint64_t lc_seed = 0, lc_calls = 0;
+    int64_t lc_cap = 0, lc_cap_s = 0;
+    double *lc_npos = nullptr;        // cell-sorted node positions [3][Np]
+    float *lc_feat = nullptr;         // features [Np][128]
+    float *lc_frame = nullptr;        // local frames (t, n, b) [Np][9]
+    float *lc_y = nullptr;            // network outputs [Np][3]
+    double *lc_dvp = nullptr;         // applied correction [3][Np]
+    double *lc_dsr = nullptr;         // converged − cheap short-range difference [3][n] (sample or all nodes)
+    double *lc_dlr = nullptr;         // converged − cheap long-range difference [3][n]
+    int32_t *lc_sidx = nullptr;       // sample nodes
+    double *lc_spos = nullptr;        // their positions [3][ns]
+    double *lc_stats = nullptr;       // check sums (3 doubles)
+    hipEvent_t lc_ev = nullptr;
 };
 
 namespace vpbs {
@@ -264,6 +286,20 @@ int advance_filaments(vpbs_ctx *c, double dt, int nsteps);
 // vpbs_api.hip
 int ensure_capacity(vpbs_ctx *c, int64_t Np, int64_t Nq);
 int device_check(vpbs_ctx *c);
+
+// shortrange.hip: target (node) cell list of the current point data (tgt_key/start/perm/cell)
+int build_target_cells(vpbs_ctx *c, hipStream_t s);
+
+// mlp.hip
+int mlp_infer(vpbs_mlp *m, int64_t n, const float *X, float *Y, hipStream_t s);
+const char *mlp_error(const vpbs_mlp *m);
+int mlp_n_in(const vpbs_mlp *m);
+int mlp_n_out(const vpbs_mlp *m);
+
+// learned.hip: stages of the learned evaluation inside vpbs_compute (short-range stream)
+int learned_after_shortrange(vpbs_ctx *c);
+int learned_after_merge(vpbs_ctx *c);
+void learned_release(vpbs_ctx *c);
 
 // helpers
 void stage_begin(vpbs_ctx *c, Stage st, hipStream_t s);
