@@ -1,0 +1,152 @@
+"""Learned correction: features, the converged − cheap difference, the sampled check and its
+fallback, training through the library's kernels, and the gain of the committed network.
+
+The correction has no counterpart in the reference. Its pieces are pinned as follows:
+* the converged − cheap difference against two full oracle evaluations (the oracle is pinned
+  by the reference's own tests), FP64, ≤ 1e-12 of the velocity;
+* the features against a brute-force CPU restatement (oracle/learned_oracle.py);
+* the check/fallback logic by a network that predicts zero: rejected → the converged
+  splitting's velocity, accepted → the cheap one, both ≤ 1e-12 against the oracle.
+"""
+import importlib.util
+import math
+import pathlib
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+TWO_PI = 2 * math.pi
+
+
+@pytest.fixture(scope="module")
+def LO():
+    spec = importlib.util.spec_from_file_location("learned_oracle", ROOT / "oracle" / "learned_oracle.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="module")
+def small(vp):
+    """8 random rings (≈1.6k nodes) on a 64³ cheap grid: β = 2 (α = 7.75, rcut = 0.258), converged β = 4."""
+    fs = vp.tangles.random_rings(8, 0.5, 1.0, TWO_PI / 64, seed=11)
+    st = vp.learned.LearnedSettings.create(TWO_PI, 64)
+    return fs, st
+
+
+def _oracle_pair(vp, O, fs, st):
+    p_c = st.params(np.float64)
+    conv = st.converged
+    spl_f = vp.GaussianSplitting.create(Ls=st.Ls, α=conv.α, rcut=conv.rcut, Ns=st.Ns)
+    p_f = vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl_f, avoid_explicit_erf=False)
+    pd = vp.add_point_charges(fs, p_c)
+    v_c = O.compute(pd, p_c, ("velocity",))["velocity"]
+    v_f = O.compute(pd, p_f, ("velocity",))["velocity"]
+    return pd, p_c, v_c, v_f
+
+
+def test_difference_vs_oracle(vp, O, small):
+    fs, st = small
+    pd, p_c, v_c, v_f = _oracle_pair(vp, O, fs, st)
+    L = vp._lib
+    ctx = L.Context(p_c)
+    ctx.set_filaments(vp.FilamentBatch(fs), p_c)
+    ctx.learned_setup(None, st.converged.α, st.converged.rcut, st.feature_radius, 0.0, 1.0)
+    v, _ = ctx.compute(L.VELOCITY | L.KEEP_UHAT)
+    dv = np.empty((3, pd.Np))
+    ctx.learned_difference(dv, None)
+    ctx.close()
+    err = np.linalg.norm(dv - (v_f - v_c)) / np.linalg.norm(v_f)
+    rel = np.linalg.norm(v_f - v_c) / np.linalg.norm(v_f)
+    print(f"  converged − cheap: |Δv|/|v| = {rel:.2e}; error vs oracle {err:.2e} of |v|")
+    assert rel_l2(v, v_c) <= 1e-12
+    assert err <= 1e-12
+    assert 1e-5 < rel < 1e-1
+
+
+def test_features_vs_restatement(vp, small, LO):
+    fs, st = small
+    p_c = st.params(np.float64)
+    batch = vp.FilamentBatch(fs)
+    L = vp._lib
+    ctx = L.Context(p_c)
+    ctx.set_filaments(batch, p_c)
+    ctx.learned_setup(None, st.converged.α, st.converged.rcut, st.feature_radius, 0.0, 1.0)
+    ctx.compute(L.VELOCITY)
+    X = np.empty((batch.Np, 128), np.float32)
+    Fr = np.empty((batch.Np, 9), np.float32)
+    ctx.learned_features(X, Fr)
+    pts = ctx.get_points()
+    ctx.close()
+    Xw, Fw = LO.features(pts["nodes"], pts["sp"], pts["spp"], batch.offs, batch.Ns, st.Ls, st.cheap.α,
+                         st.feature_radius)
+    nslots = (np.abs(Xw[:, 6::6][:, :16]) + np.abs(Xw[:, 9::6][:, :16]) > 0).sum(axis=1)
+    print(f"  strands per node: mean {nslots.mean():.2f}, max {nslots.max()}")
+    assert nslots.max() > 0
+    assert np.abs(Fr - Fw).max() < 1e-6
+    assert np.abs(X - Xw).max() < 2e-5, np.abs(X - Xw).max()
+
+
+def _zero_net(vp):
+    n = vp.learned.CorrectionNetwork.num_params()
+    return vp.learned.CorrectionNetwork(np.zeros(n), np.zeros(128), np.ones(128), np.ones(3))
+
+
+@pytest.mark.parametrize("tol", [1e-12, 1.0])
+def test_check_rejects_or_accepts(vp, O, small, tol):
+    """A network that predicts zero: its error equals the cheap splitting's. Below tolerance the
+    cheap velocity is returned; above it the converged splitting's on every node."""
+    fs, st = small
+    pd, p_c, v_c, v_f = _oracle_pair(vp, O, fs, st)
+    lb = vp.learned.LearnedBiotSavart(st, _zero_net(vp), T=np.float64, sample_fraction=0.05, tol=tol, seed=3)
+    v, rep = lb.velocity_on_nodes(fs)
+    lb.close()
+    print(f"  tol {tol}: {rep}")
+    assert rep["nsample"] == math.ceil(0.05 * pd.Np)
+    assert rep["err_corrected"] == pytest.approx(rep["err_uncorrected"], rel=1e-12)
+    whole = np.linalg.norm(v_f - v_c) / np.linalg.norm(v_f)
+    assert whole / 3 < rep["err_uncorrected"] < 3 * whole
+    if tol < 1e-6:
+        assert rep["fallback"] == 1
+        assert rel_l2(v.T, v_f) <= 1e-12
+    else:
+        assert rep["fallback"] == 0
+        assert rel_l2(v.T, v_c) <= 1e-12
+
+
+def test_errors_and_limits(vp, small):
+    fs, st = small
+    L = vp._lib
+    ctx = L.Context(st.params(np.float32))
+    with pytest.raises(L.VpbsError, match="not set up"):
+        ctx.compute_learned()
+    with pytest.raises(L.VpbsError, match="alpha_conv < alpha"):
+        ctx.learned_setup(None, 2 * st.cheap.α, st.converged.rcut, st.feature_radius)
+    with pytest.raises(L.VpbsError, match="feature_radius"):
+        ctx.learned_setup(None, st.converged.α, st.converged.rcut, 5 * st.feature_radius)
+    ctx.learned_setup(None, st.converged.α, st.converged.rcut, st.feature_radius)
+    with pytest.raises(L.VpbsError, match="vpbs_set_filaments"):
+        ctx.learned_features(np.empty((1, 128), np.float32))
+    ctx.close()
+
+
+def test_training_pipeline_gain(vp, small):
+    """Pairs from 6 small tangles, Adam through vpbs_mlp_train, then a held-out tangle: the
+    corrected velocity is closer to the converged splitting than the cheap one (whole sample)."""
+    _, st = small
+    tangles = [vp.tangles.random_rings(10, 0.5, 1.0, TWO_PI / 64, seed=100 + k) for k in range(6)]
+    X, T = vp.learned.training_pairs(tangles, st)
+    net = vp.learned.CorrectionNetwork.initial(seed=1)
+    vp.learned.fit(net, X, T, steps=3000, batch=2048, lr=2e-3, lr_final=1e-4, chunks=6)
+    held = vp.tangles.random_rings(10, 0.5, 1.0, TWO_PI / 64, seed=999)
+    lb = vp.learned.LearnedBiotSavart(st, net, T=np.float64, sample_fraction=1.0, tol=1.0)
+    _, rep = lb.velocity_on_nodes(held)
+    lb.close()
+    gain = rep["err_uncorrected"] / rep["err_corrected"]
+    print(f"  held-out: cheap {rep['err_uncorrected']:.3e}, corrected {rep['err_corrected']:.3e}, gain {gain:.1f}x;"
+          f" losses {net.meta['loss_history'][0]:.3e} -> {net.meta['loss_history'][-1]:.3e}")
+    assert gain > 2

@@ -5,7 +5,7 @@ ABI (include/vpbs.h). The directory name contains a dot, so it is loaded with
 ``__graft_entry__.load_package()`` (module name ``vortexpasta_jl_amd``).
 """
 
-from . import diagnostics, filaments, forcing, params, pointdata, tangles, timestepping  # noqa: F401
+from . import diagnostics, filaments, forcing, learned, params, pointdata, tangles, timestepping  # noqa: F401
 from .biotsavart import (  # noqa: F401
     BiotSavartCache,
     background_vorticity_correction,
