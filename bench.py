@@ -22,6 +22,7 @@ import argparse
 import json
 import math
 import os
+import pathlib
 import sys
 import time
 
@@ -75,27 +76,34 @@ def time_path(vp, params, pd, steps: int, warmup: int):
     return ms, stages, v, grid, cells
 
 
-def errors_per_quantity(vp, p32, p64, pd):
-    """Relative L2 error of the FP32-bulk path against the FP64 path for v, ψ and E.
+def gpu_fields(vp, params, pd):
+    """Velocity and streamfunction of one (untimed) evaluation in the precision of ``params``."""
+    ctx = vp._lib.Context(params)
+    ctx.set_points(pd)
+    out = ctx.compute(vp._lib.VELOCITY | vp._lib.STREAMFUNCTION, fetch=True)
+    ctx.close()
+    return out
 
-    One evaluation of velocity and streamfunction in each precision (untimed);
-    the energy is ``kinetic_energy_from_streamfunction`` on the nodes
-    (``src/Diagnostics/energy.jl:104-119``). The FP64 path itself matches the
-    CPU oracle to ≤1e-12 in the parity tests, which run at sizes the oracle
-    finishes in seconds.
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def errors_vs_oracle(vp, fields, ref, pd, p64):
+    """Relative L2 errors of velocity, streamfunction and energy against the oracle's fields.
+
+    The energy is ``kinetic_energy_from_streamfunction`` on the nodes
+    (``src/Diagnostics/energy.jl:104-119``) of each streamfunction.
     """
-    out = {}
-    for name, params in (("f32", p32), ("f64", p64)):
-        ctx = vp._lib.Context(params)
-        ctx.set_points(pd)
-        out[name] = ctx.compute(vp._lib.VELOCITY | vp._lib.STREAMFUNCTION, fetch=True)
-        ctx.close()
-    (v32, psi32), (v64, psi64) = out["f32"], out["f64"]
-    E = {k: vp.diagnostics.kinetic_energy_from_streamfunction(pd.batch, out[k][1], p64.Γ, p64.Ls, sp=pd.sp)
-         for k in out}
-    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
-    return {"velocity": rel(v32, v64), "streamfunction": rel(psi32, psi64),
-            "energy": abs(E["f32"] - E["f64"]) / abs(E["f64"]), "energy_f64": E["f64"]}
+    v, psi = fields
+    out = {"velocity": rel_l2(v, ref["velocity"])}
+    if psi is not None and "streamfunction" in ref:
+        E = vp.diagnostics.kinetic_energy_from_streamfunction(pd.batch, psi, p64.Γ, p64.Ls, sp=pd.sp)
+        E0 = vp.diagnostics.kinetic_energy_from_streamfunction(pd.batch, ref["streamfunction"], p64.Γ, p64.Ls,
+                                                              sp=pd.sp)
+        out.update({"streamfunction": rel_l2(psi, ref["streamfunction"]), "energy": abs(E - E0) / abs(E0),
+                    "energy_oracle": E0})
+    return out
 
 
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 5):
@@ -245,8 +253,33 @@ def pair_kernel_report(vp, params, pd, stage_ms: float, reps: int = 5):
             "achieved_tflops": round(tf, 3), "peak_tflops": VALU_PEAK_TFS, "frac": round(tf / VALU_PEAK_TFS, 4)}
 
 
-def cpu_baseline(O, pd, params, target_s: float = 12.0):
-    """Time the FP64 CPU oracle on a bounded sample and scale to node-velocities/s.
+def oracle_run(O, pd, params):
+    """One full FP64 oracle evaluation of the velocity (timed: the CPU baseline) and one of the
+    streamfunction (untimed), on all host cores the process may use (at most 16)."""
+    import resource
+
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    t = time.perf_counter()
+    ref = O.compute(pd, params, ("velocity",), workers=cores)
+    t_v = time.perf_counter() - t
+    log(f"oracle velocity: {t_v:.1f} s")
+    t = time.perf_counter()
+    ref["streamfunction"] = O.compute(pd, params, ("streamfunction",), workers=cores)["streamfunction"]
+    log(f"oracle streamfunction: {time.perf_counter() - t:.1f} s")
+    peak_gb = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1e6
+    cb = {"value": pd.Np / t_v, "unit": "node-velocities/s", "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
+          "kind": "port",
+          "sample": f"one full velocity evaluation of the same {pd.Np}-node workload by the FP64 C/numpy oracle "
+                    f"(cell-list pair sum, LIA, self terms, ES-kernel NUFFT with scipy FFTs), {t_v:.1f} s",
+          "seconds_per_eval": t_v, "peak_host_memory_GB": round(peak_gb, 1),
+          "reference": "reference CPU path not timed: Julia unavailable"}
+    return ref, cb
+
+
+def cpu_baseline_sampled(O, pd, params, target_s: float = 12.0):
+    """Time the FP64 CPU oracle on a bounded sample and scale to node-velocities/s (for
+    configurations whose full oracle evaluation does not fit in the bench's time).
 
     Sample: short-range pair sum for a contiguous block of nodes (whole rings),
     spreading/interpolation for a contiguous block of points/nodes, one 3D r2c
@@ -313,9 +346,47 @@ def cpu_baseline(O, pd, params, target_s: float = 12.0):
               f"interp {np_s} nodes, one {G[0]}^3 r2c FFT (x6), LIA+self on all nodes; scaled to one evaluation "
               f"of {total:.1f} s")
     return {"value": Np / total, "unit": "node-velocities/s", "cores": cores, "kind": "port", "sample": sample,
-            "seconds_per_eval": total,
+            "seconds_per_eval": total, "reference": "reference CPU path not timed: Julia unavailable",
             "breakdown_s": {"pairs": t_sr_full, "local_self": t_local, "spread": t_spread, "fft": t_fft,
                             "interp": t_interp}}
+
+
+def time_learned(vp, pd, spl, steps: int, warmup: int):
+    """The cheap splitting (β = 2 on the half grid, FP32 bulk) plus the learned correction with
+    its 1 % sampled check (vpbs_compute_learned), timed per call (wall clock, the check and any
+    fallback included). Returns None when no trained network is available."""
+    L = vp.learned
+    if not L.DEFAULT_WEIGHTS.exists():
+        log(f"learned correction: no weights at {L.DEFAULT_WEIGHTS}")
+        return None
+    net = L.CorrectionNetwork.load(L.DEFAULT_WEIGHTS)
+    st = L.LearnedSettings.create(spl.Ls, tuple(n // 2 for n in spl.Ns))
+    lb = L.LearnedBiotSavart(st, net, T=np.float32, sample_fraction=0.01, tol=1e-4)
+    lb.ctx.set_filaments(pd.batch, lb.params)  # point data of the same filaments, built on the device
+    for _ in range(warmup):
+        lb.ctx.compute_learned(fetch=False)
+    walls, reps, stages = [], [], {}
+    for _ in range(steps):
+        t = time.perf_counter()
+        _, rep = lb.ctx.compute_learned(fetch=False)
+        walls.append(time.perf_counter() - t)
+        reps.append(rep)
+        for k, v in lb.ctx.timings().items():
+            if isinstance(v, float) and k != "total":
+                stages[k] = stages.get(k, 0.0) + v / steps
+    v, rep = lb.ctx.compute_learned(fetch=True)
+    lb.close()
+    ms = float(np.mean(walls)) * 1e3
+    return {"ms_per_step": ms, "value": pd.Np / (ms * 1e-3), "velocity": v,
+            "cheap": {"Ns": list(st.Ns), "beta": st.beta_cheap, "alpha": st.cheap.α, "rcut": st.cheap.rcut},
+            "converged": {"beta": st.beta_conv, "alpha": st.converged.α, "rcut": st.converged.rcut},
+            "sampled_check": {"fraction": 0.01, "tol": 1e-4,
+                              "err_corrected_mean": float(np.mean([r["err_corrected"] for r in reps])),
+                              "err_uncorrected_mean": float(np.mean([r["err_uncorrected"] for r in reps])),
+                              "fallbacks": int(sum(r["fallback"] for r in reps))},
+            "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+            "network": {"file": str(L.DEFAULT_WEIGHTS.relative_to(pathlib.Path(ge.ROOT))),
+                        "held_out_mean_gain": net.meta.get("held_out", {}).get("mean_gain")}}
 
 
 def main():
@@ -329,8 +400,11 @@ def main():
     ap.add_argument("--fft", default="fused", choices=["fused", "separate", "rocfft"],
                     help="long-range transforms: fused hand-written kernels (default), the same with the "
                          "interpolation in its own kernel, or rocFFT")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 path (and the error figures)")
+    ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle (CPU baseline and errors)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 path")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction path")
+    ap.add_argument("--only", choices=["fp32", "fp64", "learned"],
+                    help="time one path only and skip every other measurement (for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB copy kernel (counter calibration for rocprofv3 --pmc) and exit")
     args = ap.parse_args()
@@ -348,16 +422,29 @@ def main():
     log(f"workload: {len(fs)} filaments, Np = {pd.Np}, Nq = {pd.Nq}, Ns = {spl.Ns}, α = {spl.α:.4f}, "
         f"rcut = {spl.rcut:.5f} (host prep {time.perf_counter() - t:.1f} s)")
 
+    if args.only:
+        if args.only == "learned":
+            r = time_learned(vp, pd, spl, args.steps, args.warmup)
+            log(f"learned path: {r['ms_per_step']:.3f} ms/step" if r else "learned path: no weights")
+        else:
+            ms, st, _, _, _ = time_path(vp, p32 if args.only == "fp32" else p64, pd, args.steps, args.warmup)
+            log(f"{args.only} path: {ms:.3f} ms/step; stages {json.dumps({k: round(v, 3) for k, v in st.items()})}")
+        return
+
     ms32, st32, v32, grid, cells = time_path(vp, p32, pd, args.steps, args.warmup)
     log(f"FP32 path: {ms32:.3f} ms/step; stages {json.dumps({k: round(v, 3) for k, v in st32.items()})}")
     res64 = None
     if not args.no_fp64:
         ms64, st64, v64, _, _ = time_path(vp, p64, pd, args.steps, args.warmup)
         log(f"FP64 path: {ms64:.3f} ms/step; stages {json.dumps({k: round(v, 3) for k, v in st64.items()})}")
-        errs = errors_per_quantity(vp, p32, p64, pd)
-        log(f"FP32 vs FP64: {json.dumps(errs)}")
         res64 = {"value": pd.Np / (ms64 * 1e-3), "ms_per_step": ms64, "dtype": "f64",
                  "stages_ms": {k: round(v, 4) for k, v in st64.items()}}
+    learned = None
+    if not args.no_learned:
+        learned = time_learned(vp, pd, spl, args.steps, args.warmup)
+        if learned:
+            log(f"learned path: {learned['ms_per_step']:.3f} ms/step; {json.dumps(learned['sampled_check'])}; "
+                f"stages {json.dumps(learned['stages_ms'])}")
 
     pdt = time_point_data(vp, p64, pd)
     log(f"point data: {json.dumps(pdt)}")
@@ -430,16 +517,26 @@ def main():
     }
     if res64 is not None:
         result["fp64_path"] = res64
-        result["rel_l2_error"] = {
-            "velocity": errs["velocity"], "streamfunction": errs["streamfunction"], "energy": errs["energy"],
-            "against": "FP64 HIP path on the same inputs; the FP64 path matches the CPU oracle to <=1e-12 "
-                       "(tests/test_gpu_parity.py)"}
+    if learned is not None:
+        vl = learned.pop("velocity")
+        learned["speedup_vs_fp64"] = (res64["ms_per_step"] / learned["ms_per_step"]) if res64 else None
+        learned["dtype"] = "f32 (bf16 network)"
+        result["learned_correction"] = learned
     if not args.no_cpu_baseline:
         O = ge.load_oracle()
-        log("timing the CPU oracle sample ...")
-        cb = cpu_baseline(O, pd, p64)
+        log("running the FP64 oracle on the whole workload ...")
+        ref, cb = oracle_run(O, pd, p64)
         log(f"CPU baseline: {cb['value']:.3e} node-velocities/s ({cb['sample']})")
         result["cpu_baseline"] = cb
+        errs = {"fp32": errors_vs_oracle(vp, gpu_fields(vp, p32, pd), ref, pd, p64)}
+        if res64 is not None:
+            errs["fp64"] = errors_vs_oracle(vp, gpu_fields(vp, p64, pd), ref, pd, p64)
+        if learned is not None:
+            errs["learned"] = {"velocity": rel_l2(vl, ref["velocity"])}
+            learned["rel_l2_velocity_vs_oracle"] = errs["learned"]["velocity"]
+        errs["against"] = "one full FP64 oracle evaluation (oracle/) of the same point data"
+        log(f"errors vs oracle: {json.dumps(errs)}")
+        result["rel_l2_error"] = errs
     print(json.dumps(result), flush=True)
 
 

@@ -739,6 +739,30 @@ def test_fft_paths_agree_static_lengths(vp, Ns):
                 assert err <= tol
 
 
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("T,tol", [(np.float32, 2e-6), (np.float64, 1e-12)], ids=["f32", "f64"])
+def test_fft_paths_agree_1536(vp, T, tol):
+    """1024³ Fourier grid (1536³ NUFFT grid, the compile-time 1536-point plans of the 4M-node
+    configuration): fused transforms with the slab interpolation (FP32) or the separate
+    interpolation (FP64) against rocFFT, long-range velocity of a few rings. One context at a time
+    (FP64 needs ~185 GB)."""
+    fs = vp.tangles.random_rings(6, 0.5, 1.0, TWO_PI / 256, seed=13)
+    batch = vp.FilamentBatch(fs)
+    L = vp._lib
+    outs = {}
+    for fft in ("fused", "rocfft"):
+        params = _params(vp, T=T, Ns=1024, backend_long=vp.HIPLongRangeBackend(fft=fft))
+        ctx = L.Context(params)
+        ctx.set_filaments(batch, params)
+        outs[fft], _ = ctx.compute(L.VELOCITY | L.NO_SHORTRANGE)
+        grid, _ = ctx.sizes()
+        ctx.close()
+    assert tuple(grid) == (1536, 1536, 1536)
+    err = rel_l2(outs["fused"], outs["rocfft"])
+    print(f"  1536^3 {T.__name__}: fused vs rocFFT {err:.2e}", flush=True)
+    assert err <= tol
+
+
 @pytest.mark.parametrize("Ns", [127, (128, 120, 100)], ids=["odd_nkx", "noncubic"])
 @pytest.mark.parametrize("T", [np.float64, np.float32])
 def test_slab_interpolation_shapes(vp, O, T, Ns):

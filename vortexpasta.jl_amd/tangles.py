@@ -95,7 +95,8 @@ def kelvin_lines(nlines: int, ds: float, L: float = TWO_PI, modes=range(1, 9), a
 
 def config_tangle(cfg: int, seed: int = 42) -> List[ClosedFilament]:
     """Filaments of the benchmark configurations (1: one ring; 2: 64 rings ≈16k nodes;
-    3: ≈3000 rings ≈1M nodes; 4: ≈256k nodes)."""
+    3: ≈3000 rings ≈1M nodes; 4: ≈256k nodes; 5: 8192 rings with radii U[0.2, 0.8] plus 256
+    Kelvin-wave lines along x, y and z, node spacing 2π/1024, ≈4.4M nodes)."""
     if cfg == 1:
         return single_ring(math.pi / 3, 64)
     if cfg == 2:
@@ -104,4 +105,6 @@ def config_tangle(cfg: int, seed: int = 42) -> List[ClosedFilament]:
         return random_rings(3000, 0.3, 1.0, TWO_PI / 512, seed=seed)
     if cfg == 4:
         return random_rings(1000, 0.5, 1.5, TWO_PI / 256, seed=seed)
+    if cfg == 5:
+        return random_rings(8192, 0.2, 0.8, TWO_PI / 1024, seed=seed) + kelvin_lines(256, TWO_PI / 1024, seed=seed + 1)
     raise ValueError(f"unknown configuration {cfg}")
