@@ -192,7 +192,9 @@ int vpbs_set_filaments(vpbs_ctx *ctx, int64_t Nf, const int64_t *counts, const d
  * step the knots are kept and only the spline coefficients are solved again;
  * after the step the knots are recomputed from the new nodes (chordal). No
  * forcing, reconnections or adaptive time step. The point data of the final
- * state is left in the context.
+ * state is left in the context. When a network is set up (vpbs_learned_setup
+ * with a non-NULL net), every evaluation is the learned one (vpbs_compute_learned:
+ * cheap splitting, correction, sampled check and fallback); see vpbs_learned_stats.
  */
 int vpbs_advance(vpbs_ctx *ctx, double dt, int32_t nsteps);
 
@@ -336,6 +338,11 @@ int vpbs_learned_setup(vpbs_ctx *ctx, vpbs_mlp *net, double alpha_conv, double r
 /* Velocity of the cheap splitting plus the network's correction, checked as above. v_out:
  * host array of 3*Np doubles or NULL (result left on the device, see vpbs_output_device_ptrs). */
 int vpbs_compute_learned(vpbs_ctx *ctx, double *v_out, vpbs_learned_report *report);
+/* Learned evaluations since vpbs_learned_setup (vpbs_compute_learned calls, and the
+ * evaluations of vpbs_advance, which uses the learned evaluation whenever a network is set up):
+ * their number, how many were rejected by the sampled check, and the largest sampled error of
+ * a corrected velocity. Any pointer may be NULL. */
+int vpbs_learned_stats(vpbs_ctx *ctx, int64_t *calls, int64_t *fallbacks, double *max_err);
 /* Training helpers (after a vpbs_compute of the current points; pointers host or device).
  * Features of every node: X[Np][128] (6 + 6·16 used: curvature vector/α and tangent in the node's
  * frame, then for the 16 nearest strands the foot of the perpendicular on the strand's tangent

@@ -40,8 +40,22 @@ def _c_layout(struct, fields, tmp_path):
 
 
 def test_struct_layouts_match_header(vp, tmp_path):
-    for cstruct, pystruct in (("vpbs_params", vp._lib.VpbsParams), ("vpbs_timings", vp._lib.VpbsTimings)):
+    for cstruct, pystruct in (("vpbs_params", vp._lib.VpbsParams), ("vpbs_timings", vp._lib.VpbsTimings),
+                              ("vpbs_learned_report", vp._lib.VpbsLearnedReport)):
         names = [n for n, _ in pystruct._fields_]
         got = _c_layout(cstruct, names, tmp_path)
         want = [getattr(pystruct, n).offset for n in names] + [ctypes.sizeof(pystruct)]
         assert got == want, (cstruct, got, want)
+
+
+def test_report_structs_convert_to_dicts(vp):
+    """The ctypes mirrors of vpbs_timings and vpbs_learned_report turn into plain dicts."""
+    t = vp._lib.VpbsTimings()
+    t.pair = 1.5
+    t.grid[:] = (768, 768, 640)
+    d = t.as_dict()
+    assert d["pair"] == 1.5 and d["grid"] == (768, 768, 640) and d["cells"] == (0, 0, 0)
+    assert set(d) == {n for n, _ in vp._lib.VpbsTimings._fields_}
+    r = vp._lib.VpbsLearnedReport(err_corrected=1e-5, fallback=1)
+    assert r.as_dict() == {"err_corrected": 1e-5, "err_uncorrected": 0.0, "nsample": 0, "fallback": 1,
+                           "total_ms": 0.0}

@@ -69,7 +69,9 @@ def test_difference_vs_oracle(vp, O, small):
 
 
 def test_features_vs_restatement(vp, small, LO):
-    fs, st = small
+    """60 rings (≈2.9k nodes): ~1 strand per node on average within the feature radius, up to 6."""
+    _, st = small
+    fs = vp.tangles.random_rings(60, 0.5, 1.0, TWO_PI / 64, seed=11)
     p_c = st.params(np.float64)
     batch = vp.FilamentBatch(fs)
     L = vp._lib
@@ -86,7 +88,7 @@ def test_features_vs_restatement(vp, small, LO):
                          st.feature_radius)
     nslots = (np.abs(Xw[:, 6::6][:, :16]) + np.abs(Xw[:, 9::6][:, :16]) > 0).sum(axis=1)
     print(f"  strands per node: mean {nslots.mean():.2f}, max {nslots.max()}")
-    assert nslots.max() > 0
+    assert nslots.mean() > 0.5 and nslots.max() >= 4
     assert np.abs(Fr - Fw).max() < 1e-6
     assert np.abs(X - Xw).max() < 2e-5, np.abs(X - Xw).max()
 
@@ -130,7 +132,7 @@ def test_errors_and_limits(vp, small):
         ctx.learned_setup(None, st.converged.α, st.converged.rcut, 5 * st.feature_radius)
     ctx.learned_setup(None, st.converged.α, st.converged.rcut, st.feature_radius)
     with pytest.raises(L.VpbsError, match="vpbs_set_filaments"):
-        ctx.learned_features(np.empty((1, 128), np.float32))
+        ctx.learned_features(np.empty((0, 128), np.float32))
     ctx.close()
 
 
@@ -139,9 +141,9 @@ def test_training_pipeline_gain(vp, small):
     corrected velocity is closer to the converged splitting than the cheap one (whole sample)."""
     _, st = small
     tangles = [vp.tangles.random_rings(10, 0.5, 1.0, TWO_PI / 64, seed=100 + k) for k in range(6)]
-    X, T = vp.learned.training_pairs(tangles, st)
+    data = vp.learned.training_pairs(tangles, st)
     net = vp.learned.CorrectionNetwork.initial(seed=1)
-    vp.learned.fit(net, X, T, steps=3000, batch=2048, lr=2e-3, lr_final=1e-4, chunks=6)
+    vp.learned.fit(net, data, steps=3000, batch=2048, lr=2e-3, lr_final=1e-4, chunks=6)
     held = vp.tangles.random_rings(10, 0.5, 1.0, TWO_PI / 64, seed=999)
     lb = vp.learned.LearnedBiotSavart(st, net, T=np.float64, sample_fraction=1.0, tol=1.0)
     _, rep = lb.velocity_on_nodes(held)

@@ -89,8 +89,6 @@ def test_bf16_rounding(M):
 
 
 def _gpu_problem(vp, M, n_in, H, nl, n_out, B, seed=3, n_used=None):
-    import torch
-
     dims, p, X, T, norm = _problem(M, n_in, H, nl, n_out, B, seed)
     if n_used is not None:
         X[:, n_used:] = 0.0
@@ -100,15 +98,15 @@ def _gpu_problem(vp, M, n_in, H, nl, n_out, B, seed=3, n_used=None):
     normf = tuple(np.float32(a).astype(np.float64) for a in norm)
     Xf = X.astype(np.float32)
     Tf = T.astype(np.float32)
-    Xd = torch.from_numpy(Xf).cuda()
-    Td = torch.from_numpy(Tf).cuda()
+    Xd = vp._lib.DeviceArray.from_numpy(Xf)
+    Td = vp._lib.DeviceArray.from_numpy(Tf)
     return dims, pf, Xf.astype(np.float64), Tf.astype(np.float64), normf, net, Xd, Td
 
 
 @pytest.mark.gpu
 def test_forward_and_gradient_vs_restatement(vp, M):
     dims, p, X, T, norm, net, Xd, Td = _gpu_problem(vp, M, 128, 128, 3, 3, 1500)
-    y = net.forward(Xd).cpu().numpy()
+    y = net.forward(Xd).numpy()
     want = M.forward(p, dims, X, *norm)[0]
     ef = np.linalg.norm(y - want) / np.linalg.norm(want)
     loss, g = net.gradient(Xd, Td)
@@ -153,7 +151,7 @@ def test_adam_step_vs_restatement(vp, M):
 def test_bf16_inference_vs_restatement(vp, M, shape, n):
     n_in, H, nl = shape
     dims, p, X, T, norm, net, Xd, Td = _gpu_problem(vp, M, n_in, H, nl, 3, n, n_used=n_in - 26)
-    y = net.infer(Xd).cpu().numpy()
+    y = net.infer(Xd).numpy()
     emu = M.infer_bf16(p, dims, X, *norm)
     f64 = M.forward(p, dims, X, *norm)[0]
     e_emu = np.linalg.norm(y - emu) / np.linalg.norm(emu)
@@ -167,8 +165,6 @@ def test_bf16_inference_vs_restatement(vp, M, shape, n):
 @pytest.mark.gpu
 def test_training_reduces_loss(vp, M):
     """Adam on random batches fits a smooth target: the loss drops by more than 20×."""
-    import torch
-
     rng = np.random.default_rng(5)
     n, n_in = 20000, 32
     X = rng.normal(size=(n, n_in)).astype(np.float32)
@@ -176,7 +172,7 @@ def test_training_reduces_loss(vp, M):
     T = (np.tanh(X @ Wt.T) + 0.3 * np.sin(X[:, :3])).astype(np.float32)
     net = vp._lib.MLP(n_in, 32, 2, 3)
     net.set_params(M.init_params(n_in, 32, 2, 3, seed=2), np.zeros(n_in), np.ones(n_in), np.ones(3))
-    Xd, Td = torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()
+    Xd, Td = vp._lib.DeviceArray.from_numpy(X), vp._lib.DeviceArray.from_numpy(T)
     l0, _ = net.gradient(Xd, Td)
     net.train(Xd, Td, steps=1500, batch=1024, lr=3e-3, seed=11, record_losses=False)
     l1, _ = net.gradient(Xd, Td)

@@ -61,6 +61,7 @@ EXPORTED = (
     "vpbs_mlp_train",
     "vpbs_learned_setup",
     "vpbs_compute_learned",
+    "vpbs_learned_stats",
     "vpbs_learned_features",
     "vpbs_learned_difference",
 )
@@ -112,6 +113,12 @@ class VpbsTimings(ctypes.Structure):
         ("check", ctypes.c_double),
     ]
 
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["grid"] = tuple(self.grid)
+        d["cells"] = tuple(self.cells)
+        return d
+
 
 class VpbsLearnedReport(ctypes.Structure):
     _fields_ = [
@@ -124,12 +131,6 @@ class VpbsLearnedReport(ctypes.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
-
-    def as_dict(self):
-        d = {name: getattr(self, name) for name, _ in self._fields_}
-        d["grid"] = tuple(self.grid)
-        d["cells"] = tuple(self.cells)
-        return d
 
 
 class VpbsError(RuntimeError):
@@ -213,6 +214,8 @@ def load() -> ctypes.CDLL:
     L.vpbs_learned_setup.restype = ctypes.c_int
     L.vpbs_compute_learned.argtypes = [vp, vp, ctypes.POINTER(VpbsLearnedReport)]
     L.vpbs_compute_learned.restype = ctypes.c_int
+    L.vpbs_learned_stats.argtypes = [vp, vp, vp, vp]
+    L.vpbs_learned_stats.restype = ctypes.c_int
     L.vpbs_learned_features.argtypes = [vp, vp, vp]
     L.vpbs_learned_features.restype = ctypes.c_int
     L.vpbs_learned_difference.argtypes = [vp, vp, vp]
@@ -246,7 +249,137 @@ def make_params(params, device: int = 0) -> VpbsParams:
 
 
 def _ptr(a):
-    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return ctypes.c_void_p(a.ptr)
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _expect(a, shape, dtype, name):
+    """A host array or DeviceArray of exactly this shape and type (C-contiguous)."""
+    ok = tuple(a.shape) == tuple(shape) and np.dtype(a.dtype) == np.dtype(dtype)
+    if isinstance(a, np.ndarray):
+        ok = ok and a.flags.c_contiguous
+    elif not isinstance(a, DeviceArray):
+        ok = False
+    if not ok:
+        raise ValueError(f"{name}: expected a C-contiguous {np.dtype(dtype).name} array of shape {tuple(shape)}")
+
+
+_hip = None
+
+
+def hip():
+    """The HIP runtime libvpbs is linked against (already loaded with it: found by its soname), for
+    device buffers of the caller (training samples, device-side outputs)."""
+    global _hip
+    if _hip is None:
+        load()
+        H = ctypes.CDLL("libamdhip64.so.7")
+        sz, vp = ctypes.c_size_t, ctypes.c_void_p
+        H.hipSetDevice.argtypes = [ctypes.c_int]
+        H.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
+        H.hipFree.argtypes = [vp]
+        H.hipMemcpy.argtypes = [vp, vp, sz, ctypes.c_int]
+        H.hipMemcpy2D.argtypes = [vp, sz, vp, sz, sz, sz, ctypes.c_int]
+        H.hipGetErrorString.argtypes = [ctypes.c_int]
+        H.hipGetErrorString.restype = ctypes.c_char_p
+        for f in (H.hipSetDevice, H.hipMalloc, H.hipFree, H.hipMemcpy, H.hipMemcpy2D):
+            f.restype = ctypes.c_int
+        _hip = H
+    return _hip
+
+
+def _hip_check(rc, what):
+    if rc != 0:
+        raise VpbsError(f"{what} failed ({rc}): {hip().hipGetErrorString(rc).decode()}")
+
+
+class DeviceArray:
+    """A C-contiguous array in device memory (hipMalloc), freed with the object.
+
+    Plumbing for the caller's side of the C ABI: training samples and device-resident
+    outputs; every computation on it runs in libvpbs.
+    """
+
+    def __init__(self, shape, dtype=np.float32, device: int = 0):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.device = device
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        H = hip()
+        p = ctypes.c_void_p()
+        _hip_check(H.hipSetDevice(device), "hipSetDevice")
+        _hip_check(H.hipMalloc(ctypes.byref(p), max(self.nbytes, 1)), f"hipMalloc({self.nbytes})")
+        self.ptr = p.value
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def __len__(self):
+        return self.shape[0]
+
+    @staticmethod
+    def from_numpy(a, device: int = 0) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(a.shape, a.dtype, device)
+        d.copy_from(a)
+        return d
+
+    def copy_from(self, a):
+        """Copy a host array (or a DeviceArray) of the same size and type into this one."""
+        if isinstance(a, DeviceArray):
+            if a.nbytes != self.nbytes:
+                raise ValueError("size mismatch")
+            src = ctypes.c_void_p(a.ptr)
+        else:
+            a = np.ascontiguousarray(a, dtype=self.dtype)
+            if a.nbytes != self.nbytes:
+                raise ValueError("size mismatch")
+            src = ctypes.c_void_p(a.ctypes.data)
+        _hip_check(hip().hipMemcpy(ctypes.c_void_p(self.ptr), src, self.nbytes, 4), "hipMemcpy")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _hip_check(hip().hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(self.ptr), self.nbytes, 4),
+                   "hipMemcpy")
+        return out
+
+    def rows(self, step: int) -> np.ndarray:
+        """Every ``step``-th row (first axis) copied to the host (hipMemcpy2D)."""
+        row = self.nbytes // self.shape[0]
+        n = (self.shape[0] + step - 1) // step
+        out = np.empty((n,) + self.shape[1:], self.dtype)
+        _hip_check(hip().hipMemcpy2D(ctypes.c_void_p(out.ctypes.data), row, ctypes.c_void_p(self.ptr), row * step,
+                                     row, n, 4), "hipMemcpy2D")
+        return out
+
+    @staticmethod
+    def concatenate(parts, device: int = 0) -> "DeviceArray":
+        """Rows of the parts (same trailing shape and type) in one new array."""
+        parts = list(parts)
+        tail, dt = parts[0].shape[1:], parts[0].dtype
+        if any(p.shape[1:] != tail or p.dtype != dt for p in parts):
+            raise ValueError("parts differ in trailing shape or type")
+        out = DeviceArray((sum(p.shape[0] for p in parts),) + tail, dt, device)
+        off = 0
+        for p in parts:
+            _hip_check(hip().hipMemcpy(ctypes.c_void_p(out.ptr + off), ctypes.c_void_p(p.ptr), p.nbytes, 4),
+                       "hipMemcpy")
+            off += p.nbytes
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            hip().hipFree(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Context:
@@ -425,17 +558,30 @@ class Context:
         self._check(self._L.vpbs_compute_learned(self._h, _ptr(v), ctypes.byref(rep)), "vpbs_compute_learned")
         return v, rep.as_dict()
 
+    def learned_stats(self):
+        """(calls, fallbacks, largest sampled error) of the learned evaluations since learned_setup."""
+        calls, fb, err = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+        self._check(self._L.vpbs_learned_stats(self._h, ctypes.byref(calls), ctypes.byref(fb), ctypes.byref(err)),
+                    "vpbs_learned_stats")
+        return {"calls": calls.value, "fallbacks": fb.value, "max_err_corrected": err.value}
+
     def learned_features(self, X=None, frame=None):
-        """Features into X (numpy (Np, 128) float32, or a CUDA tensor) and frames (Np, 9)."""
-        px = ctypes.c_void_p(X.data_ptr()) if hasattr(X, "data_ptr") else _ptr(X)
-        pf = ctypes.c_void_p(frame.data_ptr()) if hasattr(frame, "data_ptr") else _ptr(frame)
+        """Features into X ((Np, 128) float32, numpy or DeviceArray) and frames (Np, 9)."""
+        if X is not None:
+            _expect(X, (self.Np, 128), np.float32, "X")
+        if frame is not None:
+            _expect(frame, (self.Np, 9), np.float32, "frame")
+        px, pf = _ptr(X), _ptr(frame)
         self._check(self._L.vpbs_learned_features(self._h, px, pf), "vpbs_learned_features")
         return X, frame
 
     def learned_difference(self, dv=None, T=None):
-        """Converged − cheap velocity: dv (3, Np) float64 and/or targets T (Np, 3) float32 (numpy or CUDA)."""
-        pd_ = ctypes.c_void_p(dv.data_ptr()) if hasattr(dv, "data_ptr") else _ptr(dv)
-        pt = ctypes.c_void_p(T.data_ptr()) if hasattr(T, "data_ptr") else _ptr(T)
+        """Converged − cheap velocity: dv (3, Np) float64 and/or targets T (Np, 3) float32 (numpy or DeviceArray)."""
+        if dv is not None:
+            _expect(dv, (3, self.Np), np.float64, "dv")
+        if T is not None:
+            _expect(T, (self.Np, 3), np.float32, "T")
+        pd_, pt = _ptr(dv), _ptr(T)
         self._check(self._L.vpbs_learned_difference(self._h, pd_, pt), "vpbs_learned_difference")
         return dv, T
 
@@ -449,8 +595,7 @@ class Context:
 class MLP:
     """Owns one vpbs_mlp (network of the learned correction) on one GPU.
 
-    Sample arrays are torch CUDA tensors (float32, contiguous): torch provides
-    the device memory, the library does the arithmetic.
+    Sample arrays are DeviceArrays of float32 rows (n, n_in) / (n, n_out).
     """
 
     def __init__(self, n_in: int, n_hidden: int, n_layers: int, n_out: int, device: int = 0):
@@ -495,17 +640,12 @@ class MLP:
 
     @staticmethod
     def _dev(t, cols):
-        import torch
-
-        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
-                and t.dim() == 2 and t.shape[1] == cols):
-            raise ValueError(f"expected a contiguous float32 CUDA tensor of shape (n, {cols})")
-        return ctypes.c_void_p(t.data_ptr())
+        if not (isinstance(t, DeviceArray) and t.dtype == np.float32 and len(t.shape) == 2 and t.shape[1] == cols):
+            raise ValueError(f"expected a float32 DeviceArray of shape (n, {cols})")
+        return ctypes.c_void_p(t.ptr)
 
     def _out(self, n):
-        import torch
-
-        return torch.empty((n, self.n_out), dtype=torch.float32, device="cuda")
+        return DeviceArray((n, self.n_out), np.float32)
 
     def forward(self, X):
         Y = self._out(X.shape[0])

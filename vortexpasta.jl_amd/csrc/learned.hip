@@ -675,7 +675,17 @@ int vpbs_learned_setup(vpbs_ctx *c, vpbs_mlp *net, double alpha_conv, double rcu
     c->lc_tol = tol;
     c->lc_seed = seed;
     c->lc_calls = 0;
+    c->lc_fallbacks = 0;
+    c->lc_max_err = 0.0;
     c->lc_ready = true;
+    return VPBS_OK;
+}
+
+int vpbs_learned_stats(vpbs_ctx *c, int64_t *calls, int64_t *fallbacks, double *max_err) {
+    if (!c) return VPBS_ERR_INVALID;
+    if (calls) *calls = (int64_t)c->lc_calls;
+    if (fallbacks) *fallbacks = c->lc_fallbacks;
+    if (max_err) *max_err = c->lc_max_err;
     return VPBS_OK;
 }
 
@@ -773,6 +783,8 @@ int vpbs_compute_learned(vpbs_ctx *c, double *v_out, vpbs_learned_report *rep) {
         }
     }
     c->lc_calls++;
+    c->lc_fallbacks += R.fallback;
+    if (R.nsample > 0) c->lc_max_err = std::max(c->lc_max_err, R.err_corrected);
     VPBS_HIP(c, hipStreamSynchronize(c->s_sr));
     VPBS_HIP(c, hipStreamSynchronize(c->s_lr));
     auto t1 = std::chrono::steady_clock::now();

@@ -1298,18 +1298,24 @@ int band_impl(vpbs_ctx *c, double kmin, double kmax, double *v_out) {
     return VPBS_OK;
 }
 
+// Partial spectra of nb blocks into partial [nb][nk], their sum into out [nk].
 template <typename CT>
-int spectrum_impl(vpbs_ctx *c, int nk, double dk, double *Ek) {
-    hipStream_t s = c->s_lr;
-    const int nb = 1024, bs = 256;
-    double *partial = nullptr, *out = nullptr;
-    VPBS_HIP(c, hipMallocAsync((void **)&partial, sizeof(double) * ((size_t)nb * nk + nk), s));
-    out = partial + (size_t)nb * nk;
-    spectrum_partial<CT><<<nb, bs, sizeof(double) * nk, s>>>(
+void launch_spectrum(vpbs_ctx *c, int nk, double dk, int nb, double *partial, double *out, hipStream_t s) {
+    spectrum_partial<CT><<<nb, 256, sizeof(double) * nk, s>>>(
         (const CT *)c->uhat, c->uhat_stride, c->nkx, c->nkz, c->kmax_mode[1], c->kmax_mode[2], 2 * M_PI / c->p.Ls[0],
         2 * M_PI / c->p.Ls[1], 2 * M_PI / c->p.Ls[2], 1.0 / dk, nk, c->uhat_quantity == VPBS_UHAT_STREAMFUNCTION,
         partial);
     spectrum_reduce<<<nblocks(nk, 256), 256, 0, s>>>(partial, nb, nk, out);
+}
+
+template <typename CT>
+int spectrum_impl(vpbs_ctx *c, int nk, double dk, double *Ek) {
+    hipStream_t s = c->s_lr;
+    const int nb = 1024;
+    double *partial = nullptr, *out = nullptr;
+    VPBS_HIP(c, hipMallocAsync((void **)&partial, sizeof(double) * ((size_t)nb * nk + nk), s));
+    out = partial + (size_t)nb * nk;
+    launch_spectrum<CT>(c, nk, dk, nb, partial, out, s);
     VPBS_HIP(c, hipGetLastError());
     VPBS_HIP(c, hipMemcpyAsync(Ek, out, sizeof(double) * nk, hipMemcpyDeviceToHost, s));
     VPBS_HIP(c, hipFreeAsync(partial, s));

@@ -657,8 +657,12 @@ int advance_filaments(vpbs_ctx *c, double dt, int nsteps) {
     hipStream_t s = c->s_sr;
     const Fil f = filament_view(c);
     const unsigned nbn = nblocks(n3, 256), nbf = nblocks(c->pd_nf, 64);
-    auto velocity = [&]() -> int {  // node velocities into c->v_out from the current state
+    // node velocities into c->v_out from the current state: the learned evaluation (cheap
+    // splitting, correction, sampled check) when a network is set up, else the plain one
+    const bool learned = c->lc_ready && c->lc_net;
+    auto velocity = [&]() -> int {
         VPBS_TRY(point_data_from_state(c));
+        if (learned) return vpbs_compute_learned(c, nullptr, nullptr);
         return vpbs_compute(c, VPBS_VELOCITY, __builtin_nan(""), nullptr, nullptr);
     };
     auto stage = [&](double a, const double *v) -> int {  // X = X0 + a·v, coefficients with the same knots

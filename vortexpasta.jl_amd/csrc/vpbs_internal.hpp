@@ -182,6 +182,8 @@ struct vpbs_ctx {
     bool lc_ready = false, lc_active = false;
     double lc_alpha_c = 0, lc_alpha_f = 0, lc_rcut_f = 0, lc_rfeat = 0, lc_frac = 0, lc_tol = 0;
     uint64_t lc_seed = 0, lc_calls = 0;
+    int64_t lc_fallbacks = 0;         // rejected corrections since vpbs_learned_setup
+    double lc_max_err = 0.0;          // largest sampled error of the corrected velocity since then
     int64_t lc_cap = 0, lc_cap_s = 0;
     double *lc_npos = nullptr;        // cell-sorted node positions [3][Np]
     float *lc_feat = nullptr;         // features [Np][128]

@@ -23,8 +23,8 @@ puts β = 2 at ~1e-3) from local geometry:
   back to the converged splitting on every node when the error exceeds the
   tolerance.
 
-Device memory for the training samples comes from torch (plumbing only); every
-computation runs in libvpbs.
+Training samples live in device buffers (``_lib.DeviceArray``, plain hipMalloc);
+every computation on them runs in libvpbs.
 """
 
 from __future__ import annotations
@@ -48,6 +48,8 @@ __all__ = [
     "LearnedSettings",
     "CorrectionNetwork",
     "training_pairs",
+    "TrainingSet",
+    "SampleStats",
     "fit",
     "LearnedBiotSavart",
     "DEFAULT_WEIGHTS",
@@ -64,8 +66,11 @@ class LearnedSettings:
     """Cheap and converged splittings on the same grid, and the feature radius.
 
     ``Ns``: the cheap (coarse) Fourier grid; α = kmax/(2β), rcut = β/α for both
-    β (``gaussian.jl:155-162``). ``feature_factor``: feature radius in units
-    of the cheap cut-off.
+    β (``gaussian.jl:155-162``). ``beta_short``, when given, sets the cheap cut-off
+    rcut = beta_short/α independently of the α of ``beta_cheap`` (the long-range
+    truncation stays that of ``beta_cheap``, the short-range truncation becomes
+    ~erfc(beta_short)). ``feature_factor``: feature radius in units of the cheap
+    cut-off.
     """
 
     Ls: Tuple[float, float, float]
@@ -73,16 +78,32 @@ class LearnedSettings:
     beta_cheap: float = 2.0
     beta_conv: float = 4.0
     feature_factor: float = 2.0
+    beta_short: Optional[float] = None
+    nufft_m: int = 4
+    nufft_sigma: float = 1.5
 
     @staticmethod
-    def create(Ls, Ns, beta_cheap=2.0, beta_conv=4.0, feature_factor=2.0) -> "LearnedSettings":
+    def create(Ls, Ns, beta_cheap=2.0, beta_conv=4.0, feature_factor=2.0, beta_short=None, nufft_m=4,
+               nufft_sigma=1.5) -> "LearnedSettings":
         Ls = tuple(float(x) for x in np.broadcast_to(np.asarray(Ls, dtype=np.float64), (3,)))
         Ns = tuple(int(x) for x in np.broadcast_to(np.asarray(Ns), (3,)))
-        return LearnedSettings(Ls, Ns, float(beta_cheap), float(beta_conv), float(feature_factor))
+        bs = None if beta_short is None else float(beta_short)
+        return LearnedSettings(Ls, Ns, float(beta_cheap), float(beta_conv), float(feature_factor), bs, int(nufft_m),
+                               float(nufft_sigma))
+
+    def to_meta(self) -> dict:
+        return asdict(self)
+
+    @staticmethod
+    def from_meta(d: dict) -> "LearnedSettings":
+        return LearnedSettings.create(**d)
 
     @property
     def cheap(self) -> GaussianSplitting:
-        return GaussianSplitting.create(Ls=self.Ls, β=self.beta_cheap, Ns=self.Ns)
+        g = GaussianSplitting.create(Ls=self.Ls, β=self.beta_cheap, Ns=self.Ns)
+        if self.beta_short is None:
+            return g
+        return GaussianSplitting.create(Ls=self.Ls, α=g.α, rcut=self.beta_short / g.α, Ns=self.Ns)
 
     @property
     def converged(self) -> GaussianSplitting:
@@ -95,6 +116,7 @@ class LearnedSettings:
     def params(self, T=np.float32, Γ=1.0, a=1e-6, Δ=0.25, **kws) -> ParamsBiotSavart:
         """Parameters of the cheap context. Local segments are excluded from the pair sum and
         their long-range part removed (avoid_explicit_erf = false), in both precisions."""
+        kws.setdefault("backend_long", HIPLongRangeBackend(σ=self.nufft_sigma, m=self.nufft_m))
         return ParamsBiotSavart.build(T, Γ=Γ, a=a, Δ=Δ, splitting=self.cheap, avoid_explicit_erf=False, **kws)
 
 
@@ -132,17 +154,12 @@ class CorrectionNetwork:
             i = fan = HIDDEN
         return CorrectionNetwork(np.concatenate(parts), np.zeros(N_IN), np.ones(N_IN), np.ones(N_OUT))
 
-    def set_normalisation(self, X, T):
-        """Per-feature mean and 1/std (constant features: shift only), per-output std."""
-        import torch
-
-        X = X[:, :N_IN]
-        mean = X.mean(dim=0).double().cpu().numpy()
-        std = X.std(dim=0).double().cpu().numpy()
+    def set_normalisation(self, stats: "SampleStats"):
+        """Per-feature mean and 1/std (constant features: shift only), per-output rms."""
+        mean, std = stats.x_mean, stats.x_std
         self.in_shift = mean.astype(np.float32)
         self.in_scale = np.where(std > 1e-6, 1.0 / np.maximum(std, 1e-30), 1.0).astype(np.float32)
-        self.out_scale = T.double().pow(2).mean(dim=0).sqrt().cpu().numpy().astype(np.float32)
-        del torch
+        self.out_scale = stats.t_rms.astype(np.float32)
 
     def to_device(self, device: int = 0) -> "_lib.MLP":
         net = _lib.MLP(N_IN, HIDDEN, LAYERS, N_OUT, device=device)
@@ -169,40 +186,96 @@ def _context(settings: LearnedSettings, T, Γ=1.0, device=0, **kws):
     return p, ctx, conv
 
 
+class SampleStats:
+    """Running per-column sums of features and targets (FP64, on the host)."""
+
+    def __init__(self):
+        self.n = 0
+        self.sx = np.zeros(N_IN)
+        self.sxx = np.zeros(N_IN)
+        self.stt = np.zeros(N_OUT)
+
+    def add(self, X: np.ndarray, T: np.ndarray):
+        X = X.astype(np.float64)
+        self.n += X.shape[0]
+        self.sx += X.sum(axis=0)
+        self.sxx += (X * X).sum(axis=0)
+        self.stt += (T.astype(np.float64) ** 2).sum(axis=0)
+
+    @property
+    def x_mean(self):
+        return self.sx / self.n
+
+    @property
+    def x_std(self):
+        m = self.x_mean
+        return np.sqrt(np.maximum(self.sxx / self.n - m * m, 0.0) * self.n / max(self.n - 1, 1))
+
+    @property
+    def t_rms(self):
+        return np.sqrt(self.stt / self.n)
+
+
+class TrainingSet:
+    """Features X (n, 128) and targets T (n, 3), float32 DeviceArrays, with their statistics."""
+
+    def __init__(self, X, T, stats: SampleStats):
+        self.X, self.T, self.stats = X, T, stats
+
+    def __len__(self):
+        return self.X.shape[0]
+
+    @staticmethod
+    def concatenate(sets: Sequence["TrainingSet"]) -> "TrainingSet":
+        st = SampleStats()
+        for s in sets:
+            st.n += s.stats.n
+            st.sx += s.stats.sx
+            st.sxx += s.stats.sxx
+            st.stt += s.stats.stt
+        return TrainingSet(_lib.DeviceArray.concatenate([s.X for s in sets]),
+                           _lib.DeviceArray.concatenate([s.T for s in sets]), st)
+
+
 def training_pairs(tangles: Iterable[Sequence], settings: LearnedSettings, Γ: float = 1.0, device: int = 0,
-                   log=None):
-    """Features and targets of every node of every tangle, as CUDA float32 tensors (n, 128), (n, 3).
+                   log=None) -> TrainingSet:
+    """Features and targets of every node of every tangle (device arrays (n, 128), (n, 3)).
 
     Each tangle is evaluated once with the cheap splitting in FP64 (point data built on the
-    device), then the converged − cheap difference is computed exactly on every node.
+    device), then the converged − cheap difference is computed exactly on every node. The
+    per-column statistics (for the normalisation) come from every 8th row.
     """
-    import torch
-
     p, ctx, conv = _context(settings, np.float64, Γ=Γ, device=device)
     ctx.learned_setup(None, conv.α, conv.rcut, settings.feature_radius, 0.0, 1.0)
     Xs, Ts = [], []
+    stats = SampleStats()
     for k, fs in enumerate(tangles):
         t0 = time.perf_counter()
         batch = fs if isinstance(fs, FilamentBatch) else FilamentBatch(fs)
         ctx.set_filaments(batch, p)
         ctx.compute(_lib.VELOCITY | _lib.KEEP_UHAT, fetch=False)
-        X = torch.empty((batch.Np, N_IN), dtype=torch.float32, device="cuda")
-        T = torch.empty((batch.Np, N_OUT), dtype=torch.float32, device="cuda")
+        X = _lib.DeviceArray((batch.Np, N_IN), np.float32, device)
+        T = _lib.DeviceArray((batch.Np, N_OUT), np.float32, device)
         ctx.learned_features(X, None)
         ctx.learned_difference(None, T)
+        stats.add(X.rows(8), T.rows(8))
         Xs.append(X)
         Ts.append(T)
         if log:
             log(f"pairs: tangle {k}: {batch.Np} nodes ({time.perf_counter() - t0:.1f} s)")
     ctx.close()
-    return torch.cat(Xs), torch.cat(Ts)
+    out = TrainingSet(_lib.DeviceArray.concatenate(Xs, device), _lib.DeviceArray.concatenate(Ts, device), stats)
+    for a in Xs + Ts:
+        a.free()
+    return out
 
 
-def fit(net: CorrectionNetwork, X, T, steps: int = 20000, batch: int = 8192, lr: float = 2e-3,
+def fit(net: CorrectionNetwork, data: TrainingSet, steps: int = 20000, batch: int = 8192, lr: float = 2e-3,
         lr_final: float = 2e-5, chunks: int = 20, seed: int = 0, device: int = 0, log=None) -> CorrectionNetwork:
     """Adam on the GPU (``vpbs_mlp_train``) with a learning rate decaying geometrically from
-    ``lr`` to ``lr_final`` over ``chunks`` equal parts. Normalisation from (X, T)."""
-    net.set_normalisation(X, T)
+    ``lr`` to ``lr_final`` over ``chunks`` equal parts. Normalisation from the set's statistics."""
+    X, T = data.X, data.T
+    net.set_normalisation(data.stats)
     dev = net.to_device(device)
     per = max(1, steps // chunks)
     hist = []

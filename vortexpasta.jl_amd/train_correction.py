@@ -45,12 +45,18 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--lr-final", type=float, default=2e-5)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--Ns", type=int, default=256, help="cheap Fourier grid per direction")
+    ap.add_argument("--beta-cheap", type=float, default=2.5, help="β of the cheap splitting (α = kmax/2β)")
+    ap.add_argument("--beta-short", type=float, default=None, help="cheap cut-off rcut = beta_short/α")
+    ap.add_argument("--feature-factor", type=float, default=2.0, help="feature radius / cheap cut-off")
+    ap.add_argument("--nufft-m", type=int, default=4, help="NUFFT kernel half-support of the cheap evaluation")
     args = ap.parse_args()
 
     vp = ge.load_package()
     L = vp.learned
     two_pi = 2 * math.pi
-    st = L.LearnedSettings.create(two_pi, 256)
+    st = L.LearnedSettings.create(two_pi, args.Ns, beta_cheap=args.beta_cheap, beta_short=args.beta_short,
+                                  feature_factor=args.feature_factor, nufft_m=args.nufft_m)
     log(f"cheap {st.cheap}\nconverged {st.converged}\nfeature radius {st.feature_radius:.5f}")
 
     def small(seed):
@@ -60,22 +66,18 @@ def main():
         return vp.tangles.random_rings(3000, 0.3, 1.0, two_pi / 512, seed=seed)
 
     t0 = time.perf_counter()
-    gen = (small(1000 + k) for k in range(args.tangles))
-    X1, T1 = L.training_pairs(gen, st, log=log)
-    gen = (big(2000 + k) for k in range(args.big))
-    X2, T2 = L.training_pairs(gen, st, log=log)
-    import torch
-
-    X = torch.cat([X1, X2])
-    T = torch.cat([T1, T2])
-    del X1, X2, T1, T2
-    log(f"pairs: {X.shape[0]} samples in {time.perf_counter() - t0:.1f} s; target rms {T.pow(2).mean().sqrt().item():.3e}")
+    sets = [L.training_pairs((small(1000 + k) for k in range(args.tangles)), st, log=log)]
+    if args.big:
+        sets.append(L.training_pairs((big(2000 + k) for k in range(args.big)), st, log=log))
+    data = L.TrainingSet.concatenate(sets) if len(sets) > 1 else sets[0]
+    del sets
+    log(f"pairs: {len(data)} samples in {time.perf_counter() - t0:.1f} s; target rms {data.stats.t_rms}")
 
     net = L.CorrectionNetwork.initial(seed=args.seed)
     t0 = time.perf_counter()
-    L.fit(net, X, T, steps=args.steps, batch=args.batch, lr=args.lr, lr_final=args.lr_final, seed=args.seed, log=log)
+    L.fit(net, data, steps=args.steps, batch=args.batch, lr=args.lr, lr_final=args.lr_final, seed=args.seed, log=log)
     log(f"fit: {time.perf_counter() - t0:.1f} s")
-    del X, T
+    del data
 
     # held-out evaluation (FP32 bulk, BF16 inference; the check over every node gives the error
     # of the corrected and of the cheap velocity against the converged splitting)
@@ -90,8 +92,7 @@ def main():
     lb.close()
     e = np.array(errs)
     net.meta.update({
-        "settings": {"Ls": st.Ls, "Ns": st.Ns, "beta_cheap": st.beta_cheap, "beta_conv": st.beta_conv,
-                     "feature_factor": st.feature_factor},
+        "settings": st.to_meta(),
         "training": {"small_tangles": [1000, 1000 + args.tangles], "big_tangles": [2000, 2000 + args.big]},
         "held_out": {"seeds": [5000, 5000 + args.held_out], "err_cheap": e[:, 0].tolist(),
                      "err_corrected": e[:, 1].tolist(), "mean_gain": float(np.mean(e[:, 0] / e[:, 1]))},
