@@ -253,6 +253,18 @@ int vpbs_get_uhat(vpbs_ctx *ctx, int32_t *quantity, int64_t *dims3, double *out)
 int vpbs_energy_spectrum(vpbs_ctx *ctx, int64_t nk, double dk, double *Ek);
 
 /*
+ * vpbs_energy_spectrum without any allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * a time-stepping or serving loop that forbids hipMalloc). The per-block partial
+ * spectra live in a fixed device array of 256 x 1024 + 1024 doubles (2.1 MB)
+ * that each GPU holds until the process ends; nk must be at most 1024 (larger
+ * nk: VPBS_ERR_INVALID). Ek: nk doubles, host or device memory. Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU (several
+ * contexts, several host threads) race and must be serialised by the caller.
+ */
+int vpbs_energy_spectrum_noalloc(vpbs_ctx *ctx, int64_t nk, double dk, double *Ek);
+
+/*
  * Kept field restricted to kmin² ≤ k² ≤ kmax², interpolated at the current
  * nodes (type-2 NUFFT): the band velocity behind Forcing.FourierBandForcingBS
  * (fourier_band_bs.jl; band as SyntheticFields.set_wavevector_band!) and, with

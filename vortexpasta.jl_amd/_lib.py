@@ -42,6 +42,7 @@ EXPORTED = (
     "vpbs_get_filaments",
     "vpbs_get_uhat",
     "vpbs_energy_spectrum",
+    "vpbs_energy_spectrum_noalloc",
     "vpbs_interpolate_band",
     "vpbs_compute",
     "vpbs_get_outputs",
@@ -175,6 +176,8 @@ def load() -> ctypes.CDLL:
     L.vpbs_get_uhat.restype = ctypes.c_int
     L.vpbs_energy_spectrum.argtypes = [vp, i64, d, vp]
     L.vpbs_energy_spectrum.restype = ctypes.c_int
+    L.vpbs_energy_spectrum_noalloc.argtypes = [vp, i64, d, vp]
+    L.vpbs_energy_spectrum_noalloc.restype = ctypes.c_int
     L.vpbs_interpolate_band.argtypes = [vp, d, d, vp]
     L.vpbs_interpolate_band.restype = ctypes.c_int
     L.vpbs_compute.argtypes = [vp, u32, d, vp, vp]
@@ -524,6 +527,15 @@ class Context:
     def energy_spectrum(self, nk: int, dk: float) -> np.ndarray:
         Ek = np.empty(int(nk))
         self._check(self._L.vpbs_energy_spectrum(self._h, int(nk), float(dk), _ptr(Ek)), "vpbs_energy_spectrum")
+        return Ek
+
+    def energy_spectrum_noalloc(self, nk: int, dk: float, out=None):
+        """vpbs_energy_spectrum_noalloc: the same spectrum without device allocation (nk ≤ 1024);
+        ``out``: nk float64 values, a numpy array (default) or a DeviceArray."""
+        Ek = np.empty(int(nk)) if out is None else out
+        _expect(Ek, (int(nk),), np.float64, "out")
+        self._check(self._L.vpbs_energy_spectrum_noalloc(self._h, int(nk), float(dk), _ptr(Ek)),
+                    "vpbs_energy_spectrum_noalloc")
         return Ek
 
     def interpolate_band(self, kmin: float, kmax: float) -> np.ndarray:

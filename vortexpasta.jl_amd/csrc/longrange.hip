@@ -37,12 +37,26 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 #include <vector>
+
+// Scratch of the allocation-free energy spectrum (vpbs_energy_spectrum_noalloc): the
+// per-block partial spectra followed by the summed spectrum. Each GPU the library runs on
+// holds its own copy until the process ends; host code reaches it only through the
+// per-device address taken with hipGetSymbolAddress at context creation.
+constexpr int SPEC_NA_BLOCKS = 256;
+constexpr int SPEC_NA_MAXBINS = 1024;
+static __device__ __align__(256) double scratch[(size_t)SPEC_NA_BLOCKS * SPEC_NA_MAXBINS + SPEC_NA_MAXBINS];
 
 namespace vpbs {
 
 namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[MAX_DEVICES] = {};  // device address of `scratch` on each GPU (null: not taken yet)
+
 
 // Spreading tile: 16 × 8 × 8 cells. A pair row of a tile (rows y and y+1
 // interleaved) is 32 values, 128 B in FP32; 16 x-cells keep the per-lane
@@ -1170,8 +1184,23 @@ int oversampled_size(int N, double sigma) {
     }
 }
 
+// Device address of `scratch` on the context's GPU (current here), taken once per device.
+int scratch_address(vpbs_ctx *c) {
+    if (c->device < 0 || c->device >= MAX_DEVICES)
+        return fail(c, VPBS_ERR_INVALID, "device %d: at most %d devices", c->device, MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+        void *ptr = nullptr;
+        VPBS_HIP(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[c->device] = (double *)ptr;
+    }
+    c->spec_scratch = g_scratch_ptr[c->device];
+    return VPBS_OK;
+}
+
 int longrange_setup(vpbs_ctx *c) {
     const int m = c->p.nufft_m;
+    VPBS_TRY(scratch_address(c));
     for (int d = 0; d < 3; d++) {
         c->G[d] = oversampled_size(c->p.Ns[d], c->p.nufft_sigma);
         if (c->G[d] < 4 * m) c->G[d] = ((4 * m + 31) / 32) * 32;
@@ -1323,6 +1352,20 @@ int spectrum_impl(vpbs_ctx *c, int nk, double dk, double *Ek) {
     return VPBS_OK;
 }
 
+// The same with the partial spectra and the sum in the GPU's `scratch` array (nk ≤
+// SPEC_NA_MAXBINS): no allocation. Every partial bin and every output bin is written by the
+// kernels before it is read, so nothing left by an earlier call is used.
+template <typename CT>
+int spectrum_noalloc_impl(vpbs_ctx *c, int nk, double dk, double *Ek) {
+    hipStream_t s = c->s_lr;
+    double *partial = c->spec_scratch, *out = partial + (size_t)SPEC_NA_BLOCKS * nk;
+    launch_spectrum<CT>(c, nk, dk, SPEC_NA_BLOCKS, partial, out, s);
+    VPBS_HIP(c, hipGetLastError());
+    VPBS_HIP(c, hipMemcpyAsync(Ek, out, sizeof(double) * nk, hipMemcpyDefault, s));
+    VPBS_HIP(c, hipStreamSynchronize(s));
+    return VPBS_OK;
+}
+
 template <typename CT>
 int get_uhat_impl(vpbs_ctx *c, double *out) {
     const int64_t n = 3 * c->uhat_stride;
@@ -1371,6 +1414,19 @@ int longrange_spectrum(vpbs_ctx *c, int64_t nk, double dk, double *Ek) {
     if (!(dk > 0) || !std::isfinite(dk)) return fail(c, VPBS_ERR_INVALID, "wavenumber step must be positive");
     if (!Ek) return fail(c, VPBS_ERR_INVALID, "null output");
     return c->p.precision == VPBS_F32 ? spectrum_impl<float>(c, (int)nk, dk, Ek) : spectrum_impl<double>(c, (int)nk, dk, Ek);
+}
+
+int longrange_spectrum_noalloc(vpbs_ctx *c, int64_t nk, double dk, double *Ek) {
+    if (c->uhat_quantity != VPBS_UHAT_VELOCITY && c->uhat_quantity != VPBS_UHAT_STREAMFUNCTION)
+        return fail(c, VPBS_ERR_STATE, "no Fourier field kept: run vpbs_compute with VPBS_KEEP_UHAT (and the long range)");
+    if (nk < 1 || nk > SPEC_NA_MAXBINS)
+        return fail(c, VPBS_ERR_INVALID, "spectrum length must be in [1, %d] without allocation (got %lld)",
+                    SPEC_NA_MAXBINS, (long long)nk);
+    if (!(dk > 0) || !std::isfinite(dk)) return fail(c, VPBS_ERR_INVALID, "wavenumber step must be positive");
+    if (!Ek) return fail(c, VPBS_ERR_INVALID, "null output");
+    if (!c->spec_scratch) return fail(c, VPBS_ERR_STATE, "scratch address of this GPU not taken");
+    return c->p.precision == VPBS_F32 ? spectrum_noalloc_impl<float>(c, (int)nk, dk, Ek)
+                                      : spectrum_noalloc_impl<double>(c, (int)nk, dk, Ek);
 }
 
 int longrange_band(vpbs_ctx *c, double kmin, double kmax, double *v_out) {

@@ -562,6 +562,12 @@ int vpbs_energy_spectrum(vpbs_ctx *c, int64_t nk, double dk, double *Ek) {
     return longrange_spectrum(c, nk, dk, Ek);
 }
 
+int vpbs_energy_spectrum_noalloc(vpbs_ctx *c, int64_t nk, double dk, double *Ek) {
+    if (!c) return VPBS_ERR_INVALID;
+    VPBS_HIP(c, hipSetDevice(c->device));
+    return longrange_spectrum_noalloc(c, nk, dk, Ek);
+}
+
 int vpbs_interpolate_band(vpbs_ctx *c, double kmin, double kmax, double *v_out) {
     if (!c) return VPBS_ERR_INVALID;
     if (!c->have_points) return fail(c, VPBS_ERR_STATE, "no point data: call vpbs_set_points or vpbs_set_filaments");

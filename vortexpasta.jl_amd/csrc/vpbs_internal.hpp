@@ -153,6 +153,7 @@ struct vpbs_ctx {
     int64_t uhat_stride = 0;   // complex values per component
     void *uhat = nullptr;
     int uhat_quantity = 0;     // 0: none, VPBS_UHAT_VELOCITY or VPBS_UHAT_STREAMFUNCTION
+    double *spec_scratch = nullptr;  // this GPU's `scratch` array (allocation-free spectrum)
     int32_t *lr_key = nullptr, *lr_start = nullptr, *lr_perm = nullptr;
     uint8_t *lr_active = nullptr;  // per tile: 1 if any neighbour tile holds a point
     int32_t *lr_i0 = nullptr;   // tile-sorted points: first stencil index per direction, 3*Nq
@@ -266,6 +267,7 @@ int spectral_x_inverse_interp(vpbs_ctx *c, double *out);
 int spectral_band(vpbs_ctx *c, double kmin2, double kmax2, double cf = 0, double cc = 0);
 void spectral_release(vpbs_ctx *c);
 int longrange_spectrum(vpbs_ctx *c, int64_t nk, double dk, double *Ek);
+int longrange_spectrum_noalloc(vpbs_ctx *c, int64_t nk, double dk, double *Ek);
 int longrange_band(vpbs_ctx *c, double kmin, double kmax, double *v_out);
 // The kept field filtered by (e^{−cf·k²} − e^{−cc·k²}) on every retained mode, transformed
 // to the real grid (long-range stream); then interpolated at n positions pos[3][n] (device,
