@@ -41,7 +41,7 @@ def log(*a):
 
 def build_workload(vp, cfg: int, sigma: float = 1.5, m: int = 4, fft: str = "fused"):
     fs = vp.tangles.config_tangle(cfg)
-    Ns = {1: 64, 2: 128, 3: 512, 4: 256}[cfg]
+    Ns = {1: 64, 2: 128, 3: 512, 4: 256, 5: 1024}[cfg]
     spl = vp.GaussianSplitting.create(Ls=2 * math.pi, β=3.5, Ns=Ns)
     bl = vp.HIPLongRangeBackend(σ=sigma, m=m, fft=fft)
     p32 = vp.ParamsBiotSavart.build(np.float32, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl, backend_long=bl)
@@ -351,16 +351,27 @@ def cpu_baseline_sampled(O, pd, params, target_s: float = 12.0):
                             "interp": t_interp}}
 
 
-def time_learned(vp, pd, spl, steps: int, warmup: int):
-    """The cheap splitting (β = 2 on the half grid, FP32 bulk) plus the learned correction with
-    its 1 % sampled check (vpbs_compute_learned), timed per call (wall clock, the check and any
-    fallback included). Returns None when no trained network is available."""
+def learned_settings(vp, net, spl):
+    """Cheap/converged settings of a trained network for this box: the betas, feature factor and
+    NUFFT settings it was trained with, on half the workload's Fourier grid."""
     L = vp.learned
-    if not L.DEFAULT_WEIGHTS.exists():
-        log(f"learned correction: no weights at {L.DEFAULT_WEIGHTS}")
+    meta = dict(net.meta.get("settings", {}))
+    meta.update({"Ls": spl.Ls, "Ns": tuple(n // 2 for n in spl.Ns)})
+    return L.LearnedSettings.from_meta(meta)
+
+
+def time_learned(vp, pd, spl, steps: int, warmup: int, weights=None):
+    """The cheap splitting (coarse grid, FP32 bulk) plus the learned correction with its 1 %
+    sampled check (vpbs_compute_learned), timed per call (wall clock, the check and any fallback
+    included), and the same cheap splitting without the network. Returns None when no trained
+    network is available."""
+    L = vp.learned
+    path = pathlib.Path(weights) if weights else L.DEFAULT_WEIGHTS
+    if not path.exists():
+        log(f"learned correction: no weights at {path}")
         return None
-    net = L.CorrectionNetwork.load(L.DEFAULT_WEIGHTS)
-    st = L.LearnedSettings.create(spl.Ls, tuple(n // 2 for n in spl.Ns))
+    net = L.CorrectionNetwork.load(path)
+    st = learned_settings(vp, net, spl)
     lb = L.LearnedBiotSavart(st, net, T=np.float32, sample_fraction=0.01, tol=1e-4)
     lb.ctx.set_filaments(pd.batch, lb.params)  # point data of the same filaments, built on the device
     for _ in range(warmup):
@@ -375,18 +386,54 @@ def time_learned(vp, pd, spl, steps: int, warmup: int):
             if isinstance(v, float) and k != "total":
                 stages[k] = stages.get(k, 0.0) + v / steps
     v, rep = lb.ctx.compute_learned(fetch=True)
+    stats = lb.ctx.learned_stats()
     lb.close()
+    # the cheap splitting alone (same context settings, no network, no check)
+    ms_c, st_c, v_c, _, _ = time_path(vp, st.params(np.float32), pd, steps, warmup)
     ms = float(np.mean(walls)) * 1e3
-    return {"ms_per_step": ms, "value": pd.Np / (ms * 1e-3), "velocity": v,
-            "cheap": {"Ns": list(st.Ns), "beta": st.beta_cheap, "alpha": st.cheap.α, "rcut": st.cheap.rcut},
+    cheap = st.cheap
+    return {"ms_per_step": ms, "value": pd.Np / (ms * 1e-3), "velocity": v, "velocity_cheap": v_c,
+            "cheap": {"Ns": list(st.Ns), "beta_long": cheap.β_longrange, "beta_short": cheap.β_shortrange,
+                      "alpha": cheap.α, "rcut": cheap.rcut, "nufft_m": st.nufft_m, "nufft_sigma": st.nufft_sigma,
+                      "ms_per_step_without_network": ms_c,
+                      "stages_ms_without_network": {k: round(v, 4) for k, v in st_c.items()}},
             "converged": {"beta": st.beta_conv, "alpha": st.converged.α, "rcut": st.converged.rcut},
             "sampled_check": {"fraction": 0.01, "tol": 1e-4,
                               "err_corrected_mean": float(np.mean([r["err_corrected"] for r in reps])),
                               "err_uncorrected_mean": float(np.mean([r["err_uncorrected"] for r in reps])),
-                              "fallbacks": int(sum(r["fallback"] for r in reps))},
+                              "fallbacks": int(sum(r["fallback"] for r in reps)), "calls": stats["calls"]},
             "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-            "network": {"file": str(L.DEFAULT_WEIGHTS.relative_to(pathlib.Path(ge.ROOT))),
-                        "held_out_mean_gain": net.meta.get("held_out", {}).get("mean_gain")}}
+            "network": {"file": os.path.relpath(path, ge.ROOT),
+                        "held_out": net.meta.get("held_out", {})}}
+
+
+def time_advance(vp, pd, spl, steps: int, warmup: int, dt: float, weights=None):
+    """SSPRK33 steps with the filaments on the device (vpbs_advance, 3 velocity evaluations per
+    step), FP32 bulk; with a trained network every evaluation is the learned one."""
+    L = vp.learned
+    path = pathlib.Path(weights) if weights else L.DEFAULT_WEIGHTS
+    net = L.CorrectionNetwork.load(path) if path.exists() else None
+    if net is not None:
+        st = learned_settings(vp, net, spl)
+        lb = L.LearnedBiotSavart(st, net, T=np.float32, sample_fraction=0.01, tol=1e-4)
+        ctx, params = lb.ctx, lb.params
+    else:
+        params = vp.ParamsBiotSavart.build(np.float32, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl)
+        ctx, lb = vp._lib.Context(params), None
+    ctx.set_filaments(pd.batch, params)
+    if warmup:
+        ctx.advance(dt, warmup)
+    t0 = time.perf_counter()
+    ctx.advance(dt, steps)
+    wall = time.perf_counter() - t0
+    out = {"steps": steps, "evaluations": 3 * steps, "dt": dt, "seconds": wall,
+           "ms_per_evaluation": wall * 1e3 / (3 * steps), "learned": net is not None}
+    if lb is not None:
+        out["learned_stats"] = ctx.learned_stats()
+        lb.close()
+    else:
+        ctx.close()
+    return out
 
 
 def main():
@@ -394,7 +441,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3, 4])
+    ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--sigma", type=float, default=1.5, help="NUFFT oversampling factor")
     ap.add_argument("--m", type=int, default=4, help="NUFFT kernel half-support")
     ap.add_argument("--fft", default="fused", choices=["fused", "separate", "rocfft"],
@@ -405,6 +452,12 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction path")
     ap.add_argument("--only", choices=["fp32", "fp64", "learned"],
                     help="time one path only and skip every other measurement (for profiler runs)")
+    ap.add_argument("--weights", default=None, help="trained network of the learned correction (.npz; default: "
+                                                     "the package's)")
+    ap.add_argument("--advance", type=int, default=0, metavar="STEPS",
+                    help="time STEPS SSPRK33 steps with the filaments on the device (vpbs_advance, 3 evaluations "
+                         "per step; learned evaluations when a network is available) and exit")
+    ap.add_argument("--dt", type=float, default=1e-5, help="time step of --advance")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB copy kernel (counter calibration for rocprofv3 --pmc) and exit")
     args = ap.parse_args()
@@ -422,9 +475,15 @@ def main():
     log(f"workload: {len(fs)} filaments, Np = {pd.Np}, Nq = {pd.Nq}, Ns = {spl.Ns}, α = {spl.α:.4f}, "
         f"rcut = {spl.rcut:.5f} (host prep {time.perf_counter() - t:.1f} s)")
 
+    if args.advance:
+        r = time_advance(vp, pd, spl, args.advance, args.warmup, args.dt, args.weights)
+        r.update({"workload": f"config {args.config}: {len(fs)} filaments, {pd.Np} nodes, Fourier grid {spl.Ns}",
+                  "node_velocities_per_s": pd.Np / (r["ms_per_evaluation"] * 1e-3)})
+        print(json.dumps(r), flush=True)
+        return
     if args.only:
         if args.only == "learned":
-            r = time_learned(vp, pd, spl, args.steps, args.warmup)
+            r = time_learned(vp, pd, spl, args.steps, args.warmup, args.weights)
             log(f"learned path: {r['ms_per_step']:.3f} ms/step" if r else "learned path: no weights")
         else:
             ms, st, _, _, _ = time_path(vp, p32 if args.only == "fp32" else p64, pd, args.steps, args.warmup)
@@ -441,7 +500,7 @@ def main():
                  "stages_ms": {k: round(v, 4) for k, v in st64.items()}}
     learned = None
     if not args.no_learned:
-        learned = time_learned(vp, pd, spl, args.steps, args.warmup)
+        learned = time_learned(vp, pd, spl, args.steps, args.warmup, args.weights)
         if learned:
             log(f"learned path: {learned['ms_per_step']:.3f} ms/step; {json.dumps(learned['sampled_check'])}; "
                 f"stages {json.dumps(learned['stages_ms'])}")
@@ -519,6 +578,7 @@ def main():
         result["fp64_path"] = res64
     if learned is not None:
         vl = learned.pop("velocity")
+        vl_cheap = learned.pop("velocity_cheap")
         learned["speedup_vs_fp64"] = (res64["ms_per_step"] / learned["ms_per_step"]) if res64 else None
         learned["dtype"] = "f32 (bf16 network)"
         result["learned_correction"] = learned
@@ -533,7 +593,9 @@ def main():
             errs["fp64"] = errors_vs_oracle(vp, gpu_fields(vp, p64, pd), ref, pd, p64)
         if learned is not None:
             errs["learned"] = {"velocity": rel_l2(vl, ref["velocity"])}
+            errs["cheap_without_network"] = {"velocity": rel_l2(vl_cheap, ref["velocity"])}
             learned["rel_l2_velocity_vs_oracle"] = errs["learned"]["velocity"]
+            learned["cheap"]["rel_l2_velocity_vs_oracle"] = errs["cheap_without_network"]["velocity"]
         errs["against"] = "one full FP64 oracle evaluation (oracle/) of the same point data"
         log(f"errors vs oracle: {json.dumps(errs)}")
         result["rel_l2_error"] = errs

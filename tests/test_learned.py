@@ -136,19 +136,23 @@ def test_errors_and_limits(vp, small):
     ctx.close()
 
 
-def test_training_pipeline_gain(vp, small):
-    """Pairs from 6 small tangles, Adam through vpbs_mlp_train, then a held-out tangle: the
-    corrected velocity is closer to the converged splitting than the cheap one (whole sample)."""
-    _, st = small
-    tangles = [vp.tangles.random_rings(10, 0.5, 1.0, TWO_PI / 64, seed=100 + k) for k in range(6)]
+def test_training_pipeline_gain(vp):
+    """Pairs from 8 tangles of 64 rings (≈135k nodes) on a 128³ grid, with a cheap splitting whose
+    error is dominated by the short-range truncation (β = 3 for α, cut-off 2/α: ~1.6e-3), Adam
+    through vpbs_mlp_train, then a held-out tangle: the corrected velocity is closer to the
+    converged splitting than the cheap one (every node checked). A CPU model of the same setting
+    (numpy/torch features and targets from the oracle) reaches 2.2x at this size."""
+    st = vp.learned.LearnedSettings.create(TWO_PI, 128, beta_cheap=3.0, beta_short=2.0)
+    tangles = [vp.tangles.random_rings(64, 0.5, 1.5, TWO_PI / 256, seed=100 + k) for k in range(8)]
     data = vp.learned.training_pairs(tangles, st)
     net = vp.learned.CorrectionNetwork.initial(seed=1)
-    vp.learned.fit(net, data, steps=3000, batch=2048, lr=2e-3, lr_final=1e-4, chunks=6)
-    held = vp.tangles.random_rings(10, 0.5, 1.0, TWO_PI / 64, seed=999)
+    vp.learned.fit(net, data, steps=4000, batch=4096, lr=2e-3, lr_final=2e-5, chunks=8)
+    held = vp.tangles.random_rings(64, 0.5, 1.5, TWO_PI / 256, seed=999)
     lb = vp.learned.LearnedBiotSavart(st, net, T=np.float64, sample_fraction=1.0, tol=1.0)
     _, rep = lb.velocity_on_nodes(held)
     lb.close()
     gain = rep["err_uncorrected"] / rep["err_corrected"]
-    print(f"  held-out: cheap {rep['err_uncorrected']:.3e}, corrected {rep['err_corrected']:.3e}, gain {gain:.1f}x;"
+    print(f"  held-out: cheap {rep['err_uncorrected']:.3e}, corrected {rep['err_corrected']:.3e}, gain {gain:.2f}x;"
           f" losses {net.meta['loss_history'][0]:.3e} -> {net.meta['loss_history'][-1]:.3e}")
-    assert gain > 2
+    assert 1e-3 < rep["err_uncorrected"] < 3e-3
+    assert gain > 1.5

@@ -58,7 +58,7 @@ __all__ = [
 N_IN = 128   # feature row of the library
 N_USED = 102  # 6 + 6·16 filled entries
 HIDDEN, LAYERS, N_OUT = 128, 3, 3
-DEFAULT_WEIGHTS = pathlib.Path(__file__).resolve().parent / "data" / "correction_beta2.npz"
+DEFAULT_WEIGHTS = pathlib.Path(__file__).resolve().parent / "data" / "correction.npz"
 
 
 @dataclass(frozen=True)
