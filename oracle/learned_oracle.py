@@ -7,12 +7,16 @@ defined by ``include/vpbs.h`` (vpbs_learned_features):
 * frame of node i: t = s′/|s′|, κ = (s″ − (s″·t)t)/|s′|², n = κ/|κ|, b = t × n
   (|κ| ≤ 1e-9·α: n = t × e/|t × e|, e the axis least aligned with t);
 * row[1] = |κ|/α, row[3] = 1 (curvature vector and tangent in the frame);
-* strands: nodes j ≠ i within the radius R (minimum image, r² = (x² + y²) + z²),
-  outside i's own stretch (the runs of consecutive neighbours of i along its
-  filament that lie within R, at most 32 each way), where the distance to i has
-  a local minimum along j's filament (r²_j ≤ r²_prev, r²_j < r²_next); for each,
-  r_p = r − (r·t_j)t_j; the 16 with the smallest |r_p|² (ties: smaller j) fill
-  slots 6 + 6k .. 6 + 6k + 5 with (α r_p, t_j) in i's frame.
+* strands: nodes j ≠ i within the radius R (minimum image, r² = (x² + y²) + z²
+  in FP64 with r = s_j − x), outside i's own stretch (the runs of consecutive
+  neighbours of i along its filament that lie within R, at most 32 each way),
+  where the distance to i has a local minimum along j's filament. In FP32 with
+  r' = float(r), w∓ = float(s_{j∓1} − s_j) (minimum image):
+  2 r'·w⁻ + |w⁻|² ≥ 0 and 2 r'·w⁺ + |w⁺|² > 0 (that is d²(j−1) ≥ d²(j) < d²(j+1)).
+  For each, with t_j = float(s′_j/|s′_j|): r_p = r' − (r'·t_j)t_j (FP32); the 16
+  with the smallest |r_p|² (FP32; ties: smaller j) fill slots 6 + 6k .. 6 + 6k + 5
+  with (α r_p, t_j) projected on i's frame in FP64.
+  Dot products are summed as (a₀b₀ + a₁b₁) + a₂b₂ without fused multiply-adds.
 
 The check against the GPU is to FP32 rounding of the stored values.
 """
@@ -65,12 +69,19 @@ def features(nodes, sp, spp, offs, counts, Ls, alpha, radius, K=16, n_in=128):
     pos = np.arange(Np) - off
     prev = off + (pos + N - 1) % N
     nxt = off + (pos + 1) % N
-    tu = sp / np.sqrt(np.sum(np.asarray(sp) ** 2, axis=0))
+    tu = (np.asarray(sp) / np.sqrt(np.sum(np.asarray(sp, np.float64) ** 2, axis=0))).astype(np.float32)
+    wm = _sep(nodes, nodes[:, prev], L).astype(np.float32)  # s_{j-1} - s_j
+    wp = _sep(nodes, nodes[:, nxt], L).astype(np.float32)   # s_{j+1} - s_j
+    f32 = np.float32
+
+    def dot(a, b):  # (a0 b0 + a1 b1) + a2 b2, rounded at each step in the arrays' type
+        return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]
+
     R2 = radius * radius
     X = np.zeros((Np, n_in))
     F = np.zeros((Np, 9))
     for i in range(Np):
-        r = _sep(nodes[:, i : i + 1], nodes, L)
+        r = _sep(nodes[:, i : i + 1], nodes, L)  # s_j - x_i
         d2 = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]
         Ni, pi, oi = N[i], pos[i], off[i]
         # own stretch
@@ -88,18 +99,25 @@ def features(nodes, sp, spp, offs, counts, Ls, alpha, radius, K=16, n_in=128):
         same = fid == fid[i]
         fo = (pos - pi) % Ni
         ok &= ~(same & ((fo <= dp) | (Ni - fo <= dm)))
-        ok &= (d2 <= d2[prev]) & (d2 < d2[nxt])
+        rf = r.astype(np.float32)
+        gm = f32(2) * dot(rf, wm) + dot(wm, wm)
+        gp = f32(2) * dot(rf, wp) + dot(wp, wp)
+        ok &= (gm >= 0) & (gp > 0)
         js = np.nonzero(ok)[0]
-        rv = r[:, js]
+        rj = rf[:, js]
         tj = tu[:, js]
-        rp = rv - np.sum(rv * tj, axis=0) * tj
-        key = np.sum(rp * rp, axis=0).astype(np.float32)
+        rt = dot(rj, tj)
+        rp = rj - rt * tj
+        key = dot(rp, rp)
         order = np.lexsort((js, key))[:K]
-        E = np.stack([t[:, i], n[:, i], b[:, i]])  # rows: frame vectors
+        E = np.stack([t[:, i], n[:, i], b[:, i]])  # rows: frame vectors (FP64)
         X[i, 1] = kn[i] / alpha
         X[i, 3] = 1.0
         for slot, q in enumerate(order):
-            X[i, 6 + 6 * slot : 9 + 6 * slot] = alpha * (E @ rp[:, q])
-            X[i, 9 + 6 * slot : 12 + 6 * slot] = E @ tj[:, q]
+            pq = rp[:, q].astype(np.float64)
+            tq = tj[:, q].astype(np.float64)
+            for e in range(3):
+                X[i, 6 + 6 * slot + e] = alpha * ((pq[0] * E[e, 0] + pq[1] * E[e, 1]) + pq[2] * E[e, 2])
+                X[i, 9 + 6 * slot + e] = (tq[0] * E[e, 0] + tq[1] * E[e, 1]) + tq[2] * E[e, 2]
         F[i] = np.concatenate([t[:, i], n[:, i], b[:, i]])
     return X.astype(np.float32), F.astype(np.float32)

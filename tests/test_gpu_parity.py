@@ -244,6 +244,28 @@ def test_few_cells_min_image(vp, O, cfg2, M, Ns, cells):
     cache.close()
 
 
+@pytest.mark.parametrize("M,Ns", [(2, 128), (3, 128), (2, 24)], ids=["M2", "M3", "M2_few_cells"])
+def test_tiled_pair_kernel(vp, O, cfg2, monkeypatch, M, Ns):
+    """The tiled pair kernel (VPBS_PAIR=tile at context creation): brute-force pair sets and the
+    oracle's values, FP64; with fewer than 2 + 2M cells per direction the library falls back to
+    one wavefront per target."""
+    fs = cfg2[:16]
+    monkeypatch.setenv("VPBS_PAIR", "tile")
+    params = _params(vp, Ns=Ns, backend_short=vp.HIPShortRangeBackend(nsubdiv=M))
+    cache, got = _gpu(vp, params, fs, Q, pointdata="host", neighbour_stats=True)
+    monkeypatch.delenv("VPBS_PAIR")
+    pd = cache.pointdata
+    counts, hashes = cache.ctx.neighbour_stats()
+    bc, bh = O.neighbour_pairs_bruteforce(pd, params)
+    assert np.array_equal(counts, bc) and np.array_equal(hashes, bh)
+    want = O.compute(pd, params, Q)
+    for k in Q:
+        err = rel_l2(got[k], want[k])
+        print(f"  M = {M}, Ns = {Ns}, cells {cache.ctx.sizes()[1]}: {k} rel L2 vs oracle = {err:.3e}", flush=True)
+        assert err <= TOL64
+    cache.close()
+
+
 @pytest.mark.parametrize("T", [np.float64, np.float32])
 def test_deterministic_repeat(vp, cfg2, T):
     """Two evaluations of the same input are bit-identical (deterministic sorts, no float atomics;

@@ -11,7 +11,8 @@
 // (FP64 pair sums, the kept Fourier field v̂ with the difference filter) to make training
 // targets and to judge the network at run time:
 //
-//   node_gather       cell-sorted node positions (the node cell list of the pair kernel)
+//   strand_records    cell-sorted node positions (the node cell list of the pair kernel), unit
+//                     tangents and the offsets to the two neighbours along each filament
 //   feature_kernel    one wavefront per node: local frame (t, n, b) from s′, s″; the K
 //                     nearest strands within the feature radius, one point per strand (the
 //                     node where the distance along that strand has a local minimum, moved
@@ -81,13 +82,37 @@ __device__ inline Frame node_frame(const double *sp, const double *spp, int64_t 
     return f;
 }
 
-__global__ void node_gather(int64_t Np, const int32_t *__restrict__ perm, const double *__restrict__ nodes,
-                            double *__restrict__ npos) {
+// Per node, in cell-sorted order (slot k of the node cell list, node i = perm[k]): its
+// position (FP64) and a record of float4 × 3: (t, i) with t the unit tangent s′/|s′| and the
+// node index as bits, then w⁻ = s_{i−1} − s_i and w⁺ = s_{i+1} − s_i (minimum image, along
+// i's filament). With r = s_i − x for a target x, |x − s_{i∓1}|² − |x − s_i|² = 2 r·w∓ + |w∓|²,
+// so the local-minimum test of the features needs no neighbour loads.
+__global__ void strand_records(int64_t Np, const int32_t *__restrict__ perm, const double *__restrict__ nodes,
+                               const double *__restrict__ sp, const int32_t *__restrict__ fid,
+                               const int64_t *__restrict__ foff, const int32_t *__restrict__ fn, Box box,
+                               double *__restrict__ npos, float4 *__restrict__ rec) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= Np) return;
     const int32_t i = perm[k];
-#pragma unroll
-    for (int d = 0; d < 3; d++) npos[d * Np + k] = nodes[d * Np + i];
+    const double x = nodes[i], y = nodes[Np + i], z = nodes[2 * Np + i];
+    npos[k] = x;
+    npos[Np + k] = y;
+    npos[2 * Np + k] = z;
+    const int f = fid[i];
+    const int64_t o = foff[f];
+    const int n = fn[f], p = (int)(i - o);
+    const int64_t im = o + (p + n - 1) % n, ip = o + (p + 1) % n;
+    const float wmx = (float)fold_sep(nodes[im] - x, box.L[0], box.Lh[0]);
+    const float wmy = (float)fold_sep(nodes[Np + im] - y, box.L[1], box.Lh[1]);
+    const float wmz = (float)fold_sep(nodes[2 * Np + im] - z, box.L[2], box.Lh[2]);
+    const float wpx = (float)fold_sep(nodes[ip] - x, box.L[0], box.Lh[0]);
+    const float wpy = (float)fold_sep(nodes[Np + ip] - y, box.L[1], box.Lh[1]);
+    const float wpz = (float)fold_sep(nodes[2 * Np + ip] - z, box.L[2], box.Lh[2]);
+    const double s1x = sp[i], s1y = sp[Np + i], s1z = sp[2 * Np + i];
+    const double nrm = sqrt(dist2_nocontract(s1x, s1y, s1z));
+    rec[3 * k] = make_float4((float)(s1x / nrm), (float)(s1y / nrm), (float)(s1z / nrm), __int_as_float(i));
+    rec[3 * k + 1] = make_float4(wmx, wmy, wmz, wpx);
+    rec[3 * k + 2] = make_float4(wpy, wpz, 0.f, 0.f);
 }
 
 struct FeatArgs {
@@ -96,6 +121,7 @@ struct FeatArgs {
     const int4 *tcell;      // per node: (ix, iy, iz, key)
     const int32_t *tstart;  // node cell starts
     const double *npos;     // cell-sorted node positions [3][Np]
+    const float4 *rec;      // cell-sorted strand records [Np][3] (strand_records)
     const double *nodes, *sp, *spp;
     const int32_t *fid;
     const int64_t *foff;
@@ -199,45 +225,35 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
             }
             const int o = cnd - r_pre[lo];
             const int s = o < r_n0[lo] ? r_s0[lo] + o : r_s1[lo] + (o - r_n0[lo]);
-            j = a.tperm[s];
+            const float4 ra = a.rec[3 * s], rb = a.rec[3 * s + 1], rc = a.rec[3 * s + 2];
+            j = __float_as_int(ra.w);
             const double r0 = fold_sep(a.npos[s] - x[0], a.box.L[0], a.box.Lh[0]);
             const double r1 = fold_sep(a.npos[Np + s] - x[1], a.box.L[1], a.box.Lh[1]);
             const double r2v = fold_sep(a.npos[2 * Np + s] - x[2], a.box.L[2], a.box.Lh[2]);
             const double d2 = dist2_nocontract(r0, r1, r2v);
             bool ok = j != i && d2 <= a.rf2;
-            const int fj = a.fid[j];
-            const int64_t oj = a.foff[fj];
-            const int Nj = a.fn[fj];
-            const int pj = (int)(j - oj);
-            if (ok && fj == fi) {
-                const int fo = ((pj - pi) % Ni + Ni) % Ni;  // steps forward from i to j
+            if (ok && (unsigned)(j - oi) < (unsigned)Ni) {  // i's own filament
+                const int fo = ((j - i) % Ni + Ni) % Ni;  // steps forward from i to j
                 if (fo <= dplus || Ni - fo <= dminus) ok = false;
             }
+            const float rx = (float)r0, ry = (float)r1, rz = (float)r2v;
             if (ok) {
-                // local minimum of the distance along j's filament
-                const int64_t jp = oj + (pj + Nj - 1) % Nj, jn = oj + (pj + 1) % Nj;
-                const double p0 = fold_sep(a.nodes[jp] - x[0], a.box.L[0], a.box.Lh[0]);
-                const double p1 = fold_sep(a.nodes[Np + jp] - x[1], a.box.L[1], a.box.Lh[1]);
-                const double p2 = fold_sep(a.nodes[2 * Np + jp] - x[2], a.box.L[2], a.box.Lh[2]);
-                const double n0 = fold_sep(a.nodes[jn] - x[0], a.box.L[0], a.box.Lh[0]);
-                const double n1 = fold_sep(a.nodes[Np + jn] - x[1], a.box.L[1], a.box.Lh[1]);
-                const double n2 = fold_sep(a.nodes[2 * Np + jn] - x[2], a.box.L[2], a.box.Lh[2]);
-                ok = d2 <= dist2_nocontract(p0, p1, p2) && d2 < dist2_nocontract(n0, n1, n2);
+                // local minimum of the distance along j's filament: d²(j−1) ≥ d²(j) < d²(j+1)
+#pragma clang fp contract(off)
+                const float gm = 2.f * ((rx * rb.x + ry * rb.y) + rz * rb.z) + ((rb.x * rb.x + rb.y * rb.y) + rb.z * rb.z);
+                const float gp = 2.f * ((rx * rb.w + ry * rc.x) + rz * rc.y) + ((rb.w * rb.w + rc.x * rc.x) + rc.y * rc.y);
+                ok = gm >= 0.f && gp > 0.f;
             }
             if (ok) {
-                const double s1[3] = {a.sp[j], a.sp[Np + j], a.sp[2 * Np + j]};
-                const double sm = 1.0 / sqrt(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]);
-                const double tj[3] = {s1[0] * sm, s1[1] * sm, s1[2] * sm};
-                const double rv[3] = {r0, r1, r2v};
-                const double rt = rv[0] * tj[0] + rv[1] * tj[1] + rv[2] * tj[2];
-                double rp[3];
-                for (int d = 0; d < 3; d++) rp[d] = rv[d] - rt * tj[d];
+#pragma clang fp contract(off)
+                const float rt = (rx * ra.x + ry * ra.y) + rz * ra.z;
+                const float px = rx - rt * ra.x, py = ry - rt * ra.y, pz = rz - rt * ra.z;
                 const double *E[3] = {F.t, F.n, F.b};
                 for (int e = 0; e < 3; e++) {
-                    fv[e] = (float)(a.alpha * (rp[0] * E[e][0] + rp[1] * E[e][1] + rp[2] * E[e][2]));
-                    fv[3 + e] = (float)(tj[0] * E[e][0] + tj[1] * E[e][1] + tj[2] * E[e][2]);
+                    fv[e] = (float)(a.alpha * ((px * E[e][0] + py * E[e][1]) + pz * E[e][2]));
+                    fv[3 + e] = (float)((ra.x * E[e][0] + ra.y * E[e][1]) + ra.z * E[e][2]);
                 }
-                key = (float)(rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2]);
+                key = (px * px + py * py) + pz * pz;
                 acc = true;
             }
         }
@@ -497,6 +513,7 @@ int lc_buffers(vpbs_ctx *c) {
     const int64_t Np = c->Np;
     if (Np > c->lc_cap) {
         VPBS_TRY(regrow(c, &c->lc_npos, 3 * Np));
+        VPBS_TRY(regrow(c, &c->lc_rec, 12 * Np));
         VPBS_TRY(regrow(c, &c->lc_feat, LC_NIN * Np));
         VPBS_TRY(regrow(c, &c->lc_frame, 9 * Np));
         VPBS_TRY(regrow(c, &c->lc_y, 3 * Np));
@@ -507,6 +524,7 @@ int lc_buffers(vpbs_ctx *c) {
     }
     if (!c->lc_stats) VPBS_TRY(regrow(c, &c->lc_stats, 4));
     if (!c->lc_ev) VPBS_HIP(c, hipEventCreateWithFlags(&c->lc_ev, hipEventDisableTiming));
+    if (!c->lc_ev_cells) VPBS_HIP(c, hipEventCreateWithFlags(&c->lc_ev_cells, hipEventDisableTiming));
     return VPBS_OK;
 }
 
@@ -523,13 +541,15 @@ int sample_buffers(vpbs_ctx *c, int64_t ns) {
 int run_features(vpbs_ctx *c, hipStream_t s) {
     const int64_t Np = c->Np;
     VPBS_TRY(lc_buffers(c));
-    node_gather<<<nblk(Np, 256), 256, 0, s>>>(Np, c->tgt_perm, c->nodes, c->lc_npos);
+    strand_records<<<nblk(Np, 256), 256, 0, s>>>(Np, c->tgt_perm, c->nodes, c->sp, c->fil_fid, c->fil_off, c->fil_n,
+                                                 c->box, c->lc_npos, (float4 *)c->lc_rec);
     FeatArgs a;
     a.Np = Np;
     a.tperm = c->tgt_perm;
     a.tcell = (const int4 *)c->tgt_cell;
     a.tstart = c->tgt_start;
     a.npos = c->lc_npos;
+    a.rec = (const float4 *)c->lc_rec;
     a.nodes = c->nodes;
     a.sp = c->sp;
     a.spp = c->spp;
@@ -619,6 +639,24 @@ int learned_after_shortrange(vpbs_ctx *c) {
     return VPBS_OK;
 }
 
+// Inside vpbs_compute (learned evaluation with a sample): the long-range part of the sampled
+// check on the long-range stream, right after the long range (the kept field is ready there).
+int learned_check_long(vpbs_ctx *c) {
+    if (c->lc_ns <= 0) return VPBS_OK;
+    stage_begin(c, ST_CHECK, c->s_lr);
+    return run_check_lr(c, c->lc_ns, c->lc_spos, c->lc_dlr);
+}
+
+// ... and its short-range part, also on the long-range stream, once the short-range stream
+// has built the cell lists (event recorded after the cell build).
+int learned_check_short(vpbs_ctx *c) {
+    if (c->lc_ns <= 0) return VPBS_OK;
+    VPBS_HIP(c, hipStreamWaitEvent(c->s_lr, c->lc_ev_cells, 0));
+    VPBS_TRY(run_check_sr(c, c->lc_ns, c->lc_sidx, c->lc_dsr, c->s_lr));
+    stage_end(c, ST_CHECK, c->s_lr);
+    return VPBS_OK;
+}
+
 // After the merge: the predicted correction added to the velocity.
 int learned_after_merge(vpbs_ctx *c) {
     hipStream_t s = c->s_sr;
@@ -629,11 +667,12 @@ int learned_after_merge(vpbs_ctx *c) {
 }
 
 void learned_release(vpbs_ctx *c) {
-    void *bufs[] = {c->lc_npos, c->lc_feat, c->lc_frame, c->lc_y, c->lc_dvp, c->lc_dsr, c->lc_dlr, c->lc_sidx,
+    void *bufs[] = {c->lc_npos, c->lc_rec, c->lc_feat, c->lc_frame, c->lc_y, c->lc_dvp, c->lc_dsr, c->lc_dlr, c->lc_sidx,
                     c->lc_spos, c->lc_stats};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->lc_ev) (void)hipEventDestroy(c->lc_ev);
+    if (c->lc_ev_cells) (void)hipEventDestroy(c->lc_ev_cells);
 }
 
 }  // namespace vpbs
@@ -738,32 +777,30 @@ int vpbs_compute_learned(vpbs_ctx *c, double *v_out, vpbs_learned_report *rep) {
     VPBS_HIP(c, hipSetDevice(c->device));
     VPBS_TRY(lc_buffers(c));
     auto t0 = std::chrono::steady_clock::now();
+    const int64_t Np = c->Np;
+    int64_t ns = c->lc_frac > 0 ? (int64_t)std::ceil(c->lc_frac * (double)Np) : 0;
+    ns = std::min(ns, Np);
+    c->lc_ns = ns;
+    if (ns > 0) {
+        // the sample, drawn first on the long-range stream; its long-range check runs there after
+        // the long range, its short-range check after the cell lists (inside vpbs_compute)
+        VPBS_TRY(sample_buffers(c, ns));
+        draw_sample<<<nblk(ns, 256), 256, 0, c->s_lr>>>(Np, ns, c->lc_seed + c->lc_calls, c->nodes, c->lc_sidx,
+                                                         c->lc_spos);
+        VPBS_HIP(c, hipGetLastError());
+    }
     c->lc_active = true;
     const int rc = vpbs_compute(c, VPBS_VELOCITY | VPBS_KEEP_UHAT, NAN, nullptr, nullptr);
     c->lc_active = false;
     if (rc != VPBS_OK) return rc;
     StreamDrain drain{c};
-    const int64_t Np = c->Np;
     vpbs_learned_report R{};
     R.fallback = 0;
-    int64_t ns = c->lc_frac > 0 ? (int64_t)std::ceil(c->lc_frac * (double)Np) : 0;
-    ns = std::min(ns, Np);
     R.nsample = ns;
     double sums[3] = {0, 0, 0};
-    c->stage_ran[ST_CHECK] = false;
     if (ns > 0) {
-        VPBS_TRY(sample_buffers(c, ns));
         hipStream_t s = c->s_sr;
-        stage_begin(c, ST_CHECK, s);
-        draw_sample<<<nblk(ns, 256), 256, 0, s>>>(Np, ns, c->lc_seed + c->lc_calls, c->nodes, c->lc_sidx, c->lc_spos);
-        VPBS_HIP(c, hipEventRecord(c->lc_ev, s));
-        VPBS_HIP(c, hipStreamWaitEvent(c->s_lr, c->lc_ev, 0));
-        VPBS_TRY(run_check_lr(c, ns, c->lc_spos, c->lc_dlr));
-        VPBS_TRY(run_check_sr(c, ns, c->lc_sidx, c->lc_dsr, s));
-        VPBS_HIP(c, hipEventRecord(c->lc_ev, c->s_lr));
-        VPBS_HIP(c, hipStreamWaitEvent(s, c->lc_ev, 0));
         check_reduce<<<1, 1024, 0, s>>>(Np, ns, c->lc_sidx, c->lc_dsr, c->lc_dlr, c->lc_dvp, c->v_out, c->lc_stats);
-        stage_end(c, ST_CHECK, s);
         VPBS_HIP(c, hipGetLastError());
         VPBS_HIP(c, hipMemcpyAsync(sums, c->lc_stats, sizeof sums, hipMemcpyDeviceToHost, s));
         VPBS_HIP(c, hipStreamSynchronize(s));
@@ -788,12 +825,6 @@ int vpbs_compute_learned(vpbs_ctx *c, double *v_out, vpbs_learned_report *rep) {
     VPBS_HIP(c, hipStreamSynchronize(c->s_sr));
     VPBS_HIP(c, hipStreamSynchronize(c->s_lr));
     auto t1 = std::chrono::steady_clock::now();
-    if (c->stage_ran[ST_CHECK]) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->ev_beg[ST_CHECK], c->ev_end[ST_CHECK]) == hipSuccess) c->timings.check = ms;
-    } else {
-        c->timings.check = 0.0;
-    }
     R.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     if (rep) *rep = R;
     if (v_out) VPBS_HIP(c, hipMemcpy(v_out, c->v_out, sizeof(double) * 3 * (size_t)Np, hipMemcpyDeviceToHost));

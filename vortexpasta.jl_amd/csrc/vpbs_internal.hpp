@@ -154,6 +154,7 @@ struct vpbs_ctx {
     void *uhat = nullptr;
     int uhat_quantity = 0;     // 0: none, VPBS_UHAT_VELOCITY or VPBS_UHAT_STREAMFUNCTION
     double *spec_scratch = nullptr;  // this GPU's `scratch` array (allocation-free spectrum)
+    bool pair_tiled = false;   // tiled pair kernel (VPBS_PAIR=tile at creation; default one wavefront per target)
     int32_t *lr_key = nullptr, *lr_start = nullptr, *lr_perm = nullptr;
     uint8_t *lr_active = nullptr;  // per tile: 1 if any neighbour tile holds a point
     int32_t *lr_i0 = nullptr;   // tile-sorted points: first stencil index per direction, 3*Nq
@@ -187,6 +188,7 @@ struct vpbs_ctx {
     double lc_max_err = 0.0;          // largest sampled error of the corrected velocity since then
     int64_t lc_cap = 0, lc_cap_s = 0;
     double *lc_npos = nullptr;        // cell-sorted node positions [3][Np]
+    float *lc_rec = nullptr;          // cell-sorted strand records [Np][12] (tangent, index, neighbour offsets)
     float *lc_feat = nullptr;         // features [Np][128]
     float *lc_frame = nullptr;        // local frames (t, n, b) [Np][9]
     float *lc_y = nullptr;            // network outputs [Np][3]
@@ -197,6 +199,8 @@ struct vpbs_ctx {
     double *lc_spos = nullptr;        // their positions [3][ns]
     double *lc_stats = nullptr;       // check sums (3 doubles)
     hipEvent_t lc_ev = nullptr;
+    hipEvent_t lc_ev_cells = nullptr;  // short-range cell lists built (learned evaluation)
+    int64_t lc_ns = 0;                // sample size of the current learned evaluation
 };
 
 namespace vpbs {
@@ -303,6 +307,8 @@ int mlp_n_out(const vpbs_mlp *m);
 // learned.hip: stages of the learned evaluation inside vpbs_compute (short-range stream)
 int learned_after_shortrange(vpbs_ctx *c);
 int learned_after_merge(vpbs_ctx *c);
+int learned_check_long(vpbs_ctx *c);
+int learned_check_short(vpbs_ctx *c);
 void learned_release(vpbs_ctx *c);
 
 // helpers
