@@ -316,6 +316,11 @@ int vpbs_mlp_get_params(vpbs_mlp *net, float *params, float *in_shift, float *in
 int vpbs_mlp_forward(vpbs_mlp *net, int64_t n, const float *X, float *Y);
 /* BF16-MFMA inference: Y = net(X). */
 int vpbs_mlp_infer(vpbs_mlp *net, int64_t n, const float *X, float *Y);
+/* Bound of the BF16 inference's outputs: vpbs_mlp_infer (and the learned correction) clamps
+ * output c to [-ybound[c], ybound[c]] (n_out positive values; NULL: no bound, the default).
+ * For a network of the learned correction: the range of the training targets, so that a node
+ * whose features lie outside the training data cannot receive an arbitrary correction. */
+int vpbs_mlp_set_output_bound(vpbs_mlp *net, const float *ybound);
 /* Loss of all n samples as one batch (mean over samples of the summed squared normalised
  * errors) and its gradient with respect to the parameters (host array of num_params floats;
  * either output may be NULL). */

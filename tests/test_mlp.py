@@ -179,3 +179,21 @@ def test_training_reduces_loss(vp, M):
     print(f"  loss {l0:.3e} -> {l1:.3e}")
     assert l1 < l0 / 20
     net.close()
+
+
+@pytest.mark.gpu
+def test_bf16_inference_output_bound(vp, M):
+    """vpbs_mlp_set_output_bound: the BF16 inference clamps each output to ±bound; no bound again
+    gives the unclamped outputs back; a non-positive bound is rejected."""
+    dims, p, X, T, norm, net, Xd, Td = _gpu_problem(vp, M, 128, 128, 3, 3, 500, n_used=102)
+    y0 = net.infer(Xd).numpy()
+    b = np.quantile(np.abs(y0), 0.7, axis=0).astype(np.float32)
+    net.set_output_bound(b)
+    y1 = net.infer(Xd).numpy()
+    assert np.array_equal(y1, np.clip(y0, -b, b))
+    assert (np.abs(y0) > b).mean() > 0.2
+    net.set_output_bound(None)
+    assert np.array_equal(net.infer(Xd).numpy(), y0)
+    with pytest.raises(vp._lib.VpbsError, match="positive"):
+        net.set_output_bound(np.array([1.0, 0.0, 1.0]))
+    net.close()

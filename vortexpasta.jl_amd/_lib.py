@@ -58,6 +58,7 @@ EXPORTED = (
     "vpbs_mlp_get_params",
     "vpbs_mlp_forward",
     "vpbs_mlp_infer",
+    "vpbs_mlp_set_output_bound",
     "vpbs_mlp_gradient",
     "vpbs_mlp_train",
     "vpbs_learned_setup",
@@ -209,6 +210,8 @@ def load() -> ctypes.CDLL:
     L.vpbs_mlp_forward.restype = ctypes.c_int
     L.vpbs_mlp_infer.argtypes = [vp, i64, vp, vp]
     L.vpbs_mlp_infer.restype = ctypes.c_int
+    L.vpbs_mlp_set_output_bound.argtypes = [vp, vp]
+    L.vpbs_mlp_set_output_bound.restype = ctypes.c_int
     L.vpbs_mlp_gradient.argtypes = [vp, i64, vp, vp, ctypes.POINTER(d), vp]
     L.vpbs_mlp_gradient.restype = ctypes.c_int
     L.vpbs_mlp_train.argtypes = [vp, i64, vp, vp, i32, i32, d, d, d, d, ctypes.c_uint64, vp]
@@ -643,6 +646,13 @@ class MLP:
         if a[0].size != self.nparams or a[1].size != self.n_in or a[2].size != self.n_in or a[3].size != self.n_out:
             raise ValueError("parameter or normalisation sizes do not match the network")
         self._check(self._L.vpbs_mlp_set_params(self._h, *(_ptr(x) for x in a)), "vpbs_mlp_set_params")
+
+    def set_output_bound(self, ybound=None):
+        """Clamp of the BF16 inference's outputs to ±ybound (n_out values; None: no bound)."""
+        b = None if ybound is None else np.ascontiguousarray(ybound, dtype=np.float32)
+        if b is not None and b.size != self.n_out:
+            raise ValueError("the output bound needs one value per output")
+        self._check(self._L.vpbs_mlp_set_output_bound(self._h, _ptr(b)), "vpbs_mlp_set_output_bound")
 
     def get_params(self):
         out = (np.empty(self.nparams, np.float32), np.empty(self.n_in, np.float32), np.empty(self.n_in, np.float32),

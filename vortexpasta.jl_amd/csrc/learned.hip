@@ -40,7 +40,7 @@ namespace {
 
 constexpr int LC_K = 16;        // strands per node
 constexpr int LC_NIN = 128;     // feature row (6 + 6·K used)
-constexpr int LC_QMAX = 256;    // strand candidates kept per node
+constexpr int LC_QMAX = 128;    // strand candidates kept per node
 constexpr int LC_MAXROWS = 81;  // (2·4 + 1)² cell rows of the feature walk
 constexpr double TWO_OVER_SQRT_PI = 1.1283791670955126;
 
@@ -137,8 +137,8 @@ struct FeatArgs {
 
 // One wavefront per node (cell-sorted order, XCD-aware block order).
 __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
-    __shared__ int r_pre[LC_MAXROWS + 1];  // candidates before row r
-    __shared__ int r_s0[LC_MAXROWS], r_n0[LC_MAXROWS], r_s1[LC_MAXROWS];
+    // per row: (candidates before it, run 0 start, run 0 length, run 1 start - run 0 length)
+    __shared__ int4 r_run[LC_MAXROWS];
     __shared__ float q_key[LC_QMAX];
     __shared__ int q_j[LC_QMAX];
     __shared__ float q_f[6][LC_QMAX];
@@ -172,66 +172,96 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
         dminus = mb ? __builtin_ctz(mb) : 32;
     }
 
-    // cell rows of the walk
+    // cell rows of the walk: lanes take rows lane and lane + 64 (≤ 2 runs each with the
+    // periodic wrap along x); a wave scan gives each row's first candidate
     const int4 tc = a.tcell[i];
     const int M = a.Mf, W2 = 2 * M + 1, nrows = W2 * W2;
-    for (int r = lane; r < nrows; r += 64) {
-        int zz = tc.z + r / W2 - M, yy = tc.y + r % W2 - M;
-        zz += (zz < 0) ? a.Nc.z : 0;
-        zz -= (zz >= a.Nc.z) ? a.Nc.z : 0;
-        yy += (yy < 0) ? a.Nc.y : 0;
-        yy -= (yy >= a.Nc.y) ? a.Nc.y : 0;
-        const int64_t rb = ((int64_t)zz * a.Nc.y + yy) * a.Nc.x;
-        const int lo = tc.x - M, hi = tc.x + M;
-        int ca0, cb0, ca1 = 0, cb1 = -1;
-        if (lo < 0) {
-            ca0 = lo + a.Nc.x, cb0 = a.Nc.x - 1, cb1 = hi;
-        } else if (hi >= a.Nc.x) {
-            ca0 = lo, cb0 = a.Nc.x - 1, cb1 = hi - a.Nc.x;
-        } else {
-            ca0 = lo, cb0 = hi;
+    int len[2] = {0, 0};
+    int4 run[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int r = lane + 64 * h;
+        if (r < nrows) {
+            const int qz = r / W2;
+            int zz = tc.z + qz - M, yy = tc.y + (r - qz * W2) - M;
+            zz += (zz < 0) ? a.Nc.z : 0;
+            zz -= (zz >= a.Nc.z) ? a.Nc.z : 0;
+            yy += (yy < 0) ? a.Nc.y : 0;
+            yy -= (yy >= a.Nc.y) ? a.Nc.y : 0;
+            const int64_t rb = ((int64_t)zz * a.Nc.y + yy) * a.Nc.x;
+            const int lo = tc.x - M, hi = tc.x + M;
+            int ca0, cb0, ca1 = 0, cb1 = -1;
+            if (lo < 0) {
+                ca0 = lo + a.Nc.x, cb0 = a.Nc.x - 1, cb1 = hi;
+            } else if (hi >= a.Nc.x) {
+                ca0 = lo, cb0 = a.Nc.x - 1, cb1 = hi - a.Nc.x;
+            } else {
+                ca0 = lo, cb0 = hi;
+            }
+            const int s0 = a.tstart[rb + ca0], n0 = a.tstart[rb + cb0 + 1] - s0;
+            int s1 = 0, n1 = 0;
+            if (cb1 >= ca1) {
+                s1 = a.tstart[rb + ca1];
+                n1 = a.tstart[rb + cb1 + 1] - s1;
+            }
+            run[h] = make_int4(0, s0, n0, s1 - n0);
+            len[h] = n0 + n1;
         }
-        const int s0 = a.tstart[rb + ca0], n0 = a.tstart[rb + cb0 + 1] - s0;
-        int s1 = 0, n1 = 0;
-        if (cb1 >= ca1) {
-            s1 = a.tstart[rb + ca1];
-            n1 = a.tstart[rb + cb1 + 1] - s1;
-        }
-        r_s0[r] = s0;
-        r_n0[r] = n0;
-        r_s1[r] = s1;
-        r_pre[r + 1] = n0 + n1;
     }
-    __syncthreads();
-    if (lane == 0) {
-        r_pre[0] = 0;
-        for (int r = 0; r < nrows; r++) r_pre[r + 1] += r_pre[r];
+    const int incl0 = wave_incl_scan(len[0]);
+    const int tot0 = __builtin_amdgcn_readlane(incl0, 63);
+    const int incl1 = nrows > 64 ? tot0 + wave_incl_scan(len[1]) : tot0;
+    if (lane < nrows) {
+        run[0].x = incl0 - len[0];
+        r_run[lane] = run[0];
     }
+    if (lane + 64 < nrows) {
+        run[1].x = incl1 - len[1];
+        r_run[lane + 64] = run[1];
+    }
+    const int ncand = nrows > 64 ? __builtin_amdgcn_readlane(incl1, nrows - 65)
+                                 : __builtin_amdgcn_readlane(incl0, nrows - 1);
     for (int e = lane; e < LC_NIN; e += 64) row[e] = 0.f;
     __syncthreads();
-    const int ncand = r_pre[nrows];
+
+    // Candidate b + lane: its row by a wave-uniform walk over the row boundaries (each passed
+    // once per node), then its slot, record and position, loaded one batch ahead.
+    int rw = 0;   // uniform: last row starting at or before the fetched batch's end
+    int crow = 0;
+    auto fetch = [&](int b, int &fs) {
+        const int cnd = b + lane;
+        crow = rw;
+        while (rw + 1 < nrows) {
+            const int next = rw < 64 ? __builtin_amdgcn_readlane(incl0, rw) : __builtin_amdgcn_readlane(incl1, rw - 64);
+            if (next > b + 63) break;
+            rw++;
+            crow += cnd >= next;
+        }
+        const int4 rr = r_run[crow];
+        const int o = cnd - rr.x;
+        fs = cnd < ncand ? (o < rr.z ? rr.y : rr.w) + o : 0;
+    };
+    int s = 0;
+    if (ncand > 0) fetch(0, s);
+    float4 ra = a.rec[3 * s], rb = a.rec[3 * s + 1], rc = a.rec[3 * s + 2];
+    double px0 = a.npos[s], px1 = a.npos[Np + s], px2 = a.npos[2 * Np + s];
 
     int nq = 0;  // queued strand points (uniform)
     for (int base = 0; base < ncand; base += 64) {
+        int s_n;
+        fetch(base + 64, s_n);
+        const float4 ra_n = a.rec[3 * s_n], rb_n = a.rec[3 * s_n + 1], rc_n = a.rec[3 * s_n + 2];
+        const double px0_n = a.npos[s_n], px1_n = a.npos[Np + s_n], px2_n = a.npos[2 * Np + s_n];
         const int cnd = base + lane;
         bool acc = false;
         float key = 0.f, fv[6] = {0, 0, 0, 0, 0, 0};
-        int32_t j = -1;
-        if (cnd < ncand) {
-            int lo = 0, hi = nrows - 1;  // last row r with r_pre[r] ≤ cnd
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (r_pre[mid] <= cnd) lo = mid; else hi = mid - 1;
-            }
-            const int o = cnd - r_pre[lo];
-            const int s = o < r_n0[lo] ? r_s0[lo] + o : r_s1[lo] + (o - r_n0[lo]);
-            const float4 ra = a.rec[3 * s], rb = a.rec[3 * s + 1], rc = a.rec[3 * s + 2];
-            j = __float_as_int(ra.w);
-            const double r0 = fold_sep(a.npos[s] - x[0], a.box.L[0], a.box.Lh[0]);
-            const double r1 = fold_sep(a.npos[Np + s] - x[1], a.box.L[1], a.box.Lh[1]);
-            const double r2v = fold_sep(a.npos[2 * Np + s] - x[2], a.box.L[2], a.box.Lh[2]);
+        const int32_t j = __float_as_int(ra.w);
+        {
+            const double r0 = fold_sep(px0 - x[0], a.box.L[0], a.box.Lh[0]);
+            const double r1 = fold_sep(px1 - x[1], a.box.L[1], a.box.Lh[1]);
+            const double r2v = fold_sep(px2 - x[2], a.box.L[2], a.box.Lh[2]);
             const double d2 = dist2_nocontract(r0, r1, r2v);
-            bool ok = j != i && d2 <= a.rf2;
+            bool ok = cnd < ncand && j != i && d2 <= a.rf2;
             if (ok && (unsigned)(j - oi) < (unsigned)Ni) {  // i's own filament
                 const int fo = ((j - i) % Ni + Ni) % Ni;  // steps forward from i to j
                 if (fo <= dplus || Ni - fo <= dminus) ok = false;
@@ -247,22 +277,26 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
             if (ok) {
 #pragma clang fp contract(off)
                 const float rt = (rx * ra.x + ry * ra.y) + rz * ra.z;
-                const float px = rx - rt * ra.x, py = ry - rt * ra.y, pz = rz - rt * ra.z;
-                const double *E[3] = {F.t, F.n, F.b};
-                for (int e = 0; e < 3; e++) {
-                    fv[e] = (float)(a.alpha * ((px * E[e][0] + py * E[e][1]) + pz * E[e][2]));
-                    fv[3 + e] = (float)((ra.x * E[e][0] + ra.y * E[e][1]) + ra.z * E[e][2]);
-                }
-                key = (px * px + py * py) + pz * pz;
+                const float qx = rx - rt * ra.x, qy = ry - rt * ra.y, qz = rz - rt * ra.z;
+                fv[0] = (float)(a.alpha * ((qx * F.t[0] + qy * F.t[1]) + qz * F.t[2]));
+                fv[1] = (float)(a.alpha * ((qx * F.n[0] + qy * F.n[1]) + qz * F.n[2]));
+                fv[2] = (float)(a.alpha * ((qx * F.b[0] + qy * F.b[1]) + qz * F.b[2]));
+                fv[3] = (float)((ra.x * F.t[0] + ra.y * F.t[1]) + ra.z * F.t[2]);
+                fv[4] = (float)((ra.x * F.n[0] + ra.y * F.n[1]) + ra.z * F.n[2]);
+                fv[5] = (float)((ra.x * F.b[0] + ra.y * F.b[1]) + ra.z * F.b[2]);
+                key = (qx * qx + qy * qy) + qz * qz;
                 acc = true;
             }
         }
+        ra = ra_n, rb = rb_n, rc = rc_n;
+        px0 = px0_n, px1 = px1_n, px2 = px2_n;
         const unsigned long long mask = __ballot(acc);
         if (acc) {
             const int pos = nq + __popcll(mask & ((1ull << lane) - 1ull));
             if (pos < LC_QMAX) {
                 q_key[pos] = key;
                 q_j[pos] = j;
+#pragma unroll
                 for (int e = 0; e < 6; e++) q_f[e][pos] = fv[e];
             }
         }
@@ -289,9 +323,14 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
     __syncthreads();
     float4 *dst = (float4 *)(a.feat + (int64_t)i * LC_NIN);
     if (lane < LC_NIN / 4) dst[lane] = make_float4(row[4 * lane], row[4 * lane + 1], row[4 * lane + 2], row[4 * lane + 3]);
-    if (lane < 9) {
-        const double *E = lane < 3 ? F.t : (lane < 6 ? F.n : F.b);
-        a.frame[(int64_t)i * 9 + lane] = (float)E[lane % 3];
+    if (lane < 3) {
+        float *fr = a.frame + (int64_t)i * 9;
+        const double t = lane == 0 ? F.t[0] : (lane == 1 ? F.t[1] : F.t[2]);
+        const double n = lane == 0 ? F.n[0] : (lane == 1 ? F.n[1] : F.n[2]);
+        const double b = lane == 0 ? F.b[0] : (lane == 1 ? F.b[1] : F.b[2]);
+        fr[lane] = (float)t;
+        fr[3 + lane] = (float)n;
+        fr[6 + lane] = (float)b;
     }
 }
 

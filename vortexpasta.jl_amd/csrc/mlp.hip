@@ -41,6 +41,7 @@ struct vpbs_mlp {
     float *params = nullptr, *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
     int64_t adam_t = 0;
     float *norm = nullptr;  // in_shift[n_in], in_scale[n_in], out_scale[n_out]
+    float ybound[16];       // |y_c| bound of the BF16 inference (+inf: none)
     bool have_params = false;
     // BF16 inference weights (LDS image: permuted rows with padding, then FP32 biases and the
     // normalisation), rebuilt after every parameter change
@@ -81,6 +82,11 @@ inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); 
 
 __device__ inline float sigmoidf_(float z) { return 1.0f / (1.0f + expf(-z)); }
 __device__ inline float silu_(float z) { return z * sigmoidf_(z); }
+// SiLU of the BF16 inference: hardware base-2 exponential and reciprocal (about 1 ulp each in
+// FP32), the result being rounded to BF16 right after.
+__device__ inline float silu_fast(float z) {
+    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z * 1.44269504088896340736f));
+}
 __device__ inline float dsilu_(float z) {
     const float s = sigmoidf_(z);
     return s * (1.0f + z * (1.0f - s));
@@ -285,8 +291,8 @@ __device__ inline bf16x8 as_bf16x8(uint4 u) { return __builtin_bit_cast(bf16x8, 
 // starts shift by 4 banks, so the 16 rows an A-fragment load touches hit distinct banks).
 // Columns of W_l (l ≥ 1) are permuted within each 32-column chunk: position 8g + 4h + i
 // holds logical input 16h + 4g + i (see the header comment). The output layer is padded to
-// 16 rows. After the weights: the FP32 biases (H per hidden layer, 16 for the output) and
-// the normalisation (in_shift, in_scale: n_in each; out_scale: 16).
+// 16 rows. After the weights: the FP32 biases (H per hidden layer, 16 for the output), the
+// normalisation (in_shift, in_scale: n_in each; out_scale: 16) and the output bound (16).
 struct InferLayout {
     int n_in, H, nl, n_out;
     size_t w_off[8];  // BF16 offsets of each layer's rows
@@ -307,7 +313,7 @@ inline InferLayout infer_layout(int n_in, int H, int nl, int n_out) {
         off += (size_t)out * (in + 8);
     }
     L.b_off = ((off * 2 + 15) / 16) * 16;
-    const size_t nf = (size_t)nl * H + 16 + 2 * (size_t)n_in + 16;
+    const size_t nf = (size_t)nl * H + 16 + 2 * (size_t)n_in + 16 + 16;
     L.bytes = L.b_off + nf * 4;
     return L;
 }
@@ -315,7 +321,8 @@ inline InferLayout infer_layout(int n_in, int H, int nl, int n_out) {
 constexpr int INFER_WAVES = 8;  // wavefronts per block
 constexpr int INFER_TILES = 2;  // 16-sample tiles per wavefront step
 
-// Y[i][c] = out_scale[c]·net(x'_i)[c] for c < n_out (FP32 out), X rows of n_in floats.
+// Y[i][c] = out_scale[c]·net(x'_i)[c] for c < n_out, clamped to ±bound[c] (FP32 out), X rows of
+// n_in floats.
 template <int NIN, int H, int NL>
 __global__ __launch_bounds__(64 * INFER_WAVES) void mlp_infer_bf16(int64_t n, int n_out, const float *__restrict__ X,
                                                                     const uint4 *__restrict__ img, int img_vec,
@@ -332,7 +339,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void mlp_infer_bf16(int64_t n, in
     constexpr size_t BOFF = ((WBYTES + 15) / 16) * 16;
     const float *fb = (const float *)((const char *)lds4 + BOFF);
     const float *bias_out = fb + NL * H;
-    const float *shift = bias_out + 16, *scale = shift + NIN, *oscale = scale + NIN;
+    const float *shift = bias_out + 16, *scale = shift + NIN, *oscale = scale + NIN, *obound = oscale + 16;
 
     const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
     const int64_t wave = (int64_t)blockIdx.x * INFER_WAVES + (threadIdx.x >> 6);
@@ -395,10 +402,10 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void mlp_infer_bf16(int64_t n, in
                 for (int c = 0; c < KCH; c++) {
                     const f32x4 lo = acc[t][2 * c], hi = acc[t][2 * c + 1];
                     uint4 pk;
-                    pk.x = pack_bf16(silu_(lo[0]), silu_(lo[1]));
-                    pk.y = pack_bf16(silu_(lo[2]), silu_(lo[3]));
-                    pk.z = pack_bf16(silu_(hi[0]), silu_(hi[1]));
-                    pk.w = pack_bf16(silu_(hi[2]), silu_(hi[3]));
+                    pk.x = pack_bf16(silu_fast(lo[0]), silu_fast(lo[1]));
+                    pk.y = pack_bf16(silu_fast(lo[2]), silu_fast(lo[3]));
+                    pk.z = pack_bf16(silu_fast(hi[0]), silu_fast(hi[1]));
+                    pk.w = pack_bf16(silu_fast(hi[2]), silu_fast(hi[3]));
                     act[t][c] = as_bf16x8(pk);
                 }
         }
@@ -423,7 +430,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void mlp_infer_bf16(int64_t n, in
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int o = 4 * g + v;
-                if (o < n_out) Y[i * n_out + o] = out[t][v] * oscale[o];
+                if (o < n_out) Y[i * n_out + o] = fminf(fmaxf(out[t][v] * oscale[o], -obound[o]), obound[o]);
             }
         }
     }
@@ -607,6 +614,8 @@ int build_infer_image(vpbs_mlp *m) {
         sc[k] = N[m->n_in + k];
     }
     for (int o = 0; o < m->n_out; o++) os[o] = N[2 * m->n_in + o];
+    float *ob = os + 16;
+    for (int o = 0; o < 16; o++) ob[o] = m->ybound[o];
     if (m->infer_img && m->infer_bytes != L.bytes) {
         MLP_HIP(m, hipFree(m->infer_img));
         m->infer_img = nullptr;
@@ -667,6 +676,7 @@ int vpbs_mlp_create(vpbs_mlp **out, int32_t n_in, int32_t n_hidden, int32_t n_la
         off += layer_out(m, l);
     }
     m->nparams = off;
+    for (int o = 0; o < 16; o++) m->ybound[o] = INFINITY;
     bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipMalloc((void **)&m->params, sizeof(float) * (size_t)off) == hipSuccess;
     ok = ok && hipMalloc((void **)&m->grad, sizeof(float) * (size_t)off) == hipSuccess;
@@ -723,6 +733,17 @@ int vpbs_mlp_get_params(vpbs_mlp *m, float *params, float *in_shift, float *in_s
     if (in_scale) MLP_HIP(m, hipMemcpy(in_scale, m->norm + m->n_in, sizeof(float) * m->n_in, hipMemcpyDeviceToHost));
     if (out_scale)
         MLP_HIP(m, hipMemcpy(out_scale, m->norm + 2 * m->n_in, sizeof(float) * m->n_out, hipMemcpyDeviceToHost));
+    return VPBS_OK;
+}
+
+int vpbs_mlp_set_output_bound(vpbs_mlp *m, const float *ybound) {
+    if (!m) return VPBS_ERR_INVALID;
+    for (int o = 0; o < m->n_out; o++) {
+        const float b = ybound ? ybound[o] : INFINITY;
+        if (!(b > 0)) return mfail(m, VPBS_ERR_INVALID, "output bound %d must be positive (got %g)", o, (double)b);
+        m->ybound[o] = b;
+    }
+    m->infer_dirty = true;
     return VPBS_OK;
 }
 

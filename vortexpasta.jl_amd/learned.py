@@ -121,13 +121,20 @@ class LearnedSettings:
 
 
 class CorrectionNetwork:
-    """Parameters (flat FP32, ``include/vpbs.h`` layout), normalisation and metadata."""
+    """Parameters (flat FP32, ``include/vpbs.h`` layout), normalisation, output bound and metadata.
 
-    def __init__(self, params, in_shift, in_scale, out_scale, meta: Optional[dict] = None):
+    ``out_bound``: per output, the largest |target| of the training set times ``BOUND_MARGIN``;
+    the inference clamps the prediction to it (None: no bound).
+    """
+
+    BOUND_MARGIN = 1.5
+
+    def __init__(self, params, in_shift, in_scale, out_scale, meta: Optional[dict] = None, out_bound=None):
         self.params = np.asarray(params, dtype=np.float32)
         self.in_shift = np.asarray(in_shift, dtype=np.float32)
         self.in_scale = np.asarray(in_scale, dtype=np.float32)
         self.out_scale = np.asarray(out_scale, dtype=np.float32)
+        self.out_bound = None if out_bound is None else np.asarray(out_bound, dtype=np.float32)
         self.meta = dict(meta or {})
 
     @staticmethod
@@ -160,23 +167,27 @@ class CorrectionNetwork:
         self.in_shift = mean.astype(np.float32)
         self.in_scale = np.where(std > 1e-6, 1.0 / np.maximum(std, 1e-30), 1.0).astype(np.float32)
         self.out_scale = stats.t_rms.astype(np.float32)
+        self.out_bound = (self.BOUND_MARGIN * stats.t_max).astype(np.float32)
 
     def to_device(self, device: int = 0) -> "_lib.MLP":
         net = _lib.MLP(N_IN, HIDDEN, LAYERS, N_OUT, device=device)
         net.set_params(self.params, self.in_shift, self.in_scale, self.out_scale)
+        net.set_output_bound(self.out_bound)
         return net
 
     def save(self, path):
         path = pathlib.Path(path)
         path.parent.mkdir(parents=True, exist_ok=True)
+        extra = {} if self.out_bound is None else {"out_bound": self.out_bound}
         np.savez(path, params=self.params, in_shift=self.in_shift, in_scale=self.in_scale, out_scale=self.out_scale,
-                 meta=np.frombuffer(json.dumps(self.meta).encode(), dtype=np.uint8))
+                 meta=np.frombuffer(json.dumps(self.meta).encode(), dtype=np.uint8), **extra)
 
     @staticmethod
     def load(path=DEFAULT_WEIGHTS) -> "CorrectionNetwork":
         with np.load(path, allow_pickle=False) as z:
             meta = json.loads(bytes(z["meta"]).decode())
-            return CorrectionNetwork(z["params"], z["in_shift"], z["in_scale"], z["out_scale"], meta)
+            bound = z["out_bound"] if "out_bound" in z.files else None
+            return CorrectionNetwork(z["params"], z["in_shift"], z["in_scale"], z["out_scale"], meta, bound)
 
 
 def _context(settings: LearnedSettings, T, Γ=1.0, device=0, **kws):
@@ -194,6 +205,7 @@ class SampleStats:
         self.sx = np.zeros(N_IN)
         self.sxx = np.zeros(N_IN)
         self.stt = np.zeros(N_OUT)
+        self.tmax = np.zeros(N_OUT)
 
     def add(self, X: np.ndarray, T: np.ndarray):
         X = X.astype(np.float64)
@@ -201,6 +213,15 @@ class SampleStats:
         self.sx += X.sum(axis=0)
         self.sxx += (X * X).sum(axis=0)
         self.stt += (T.astype(np.float64) ** 2).sum(axis=0)
+        self.bound(T)
+
+    def bound(self, T: np.ndarray):
+        """Largest |target| per output (every row of T)."""
+        self.tmax = np.maximum(self.tmax, np.abs(T.astype(np.float64)).max(axis=0, initial=0.0))
+
+    @property
+    def t_max(self):
+        return self.tmax
 
     @property
     def x_mean(self):
@@ -233,6 +254,7 @@ class TrainingSet:
             st.sx += s.stats.sx
             st.sxx += s.stats.sxx
             st.stt += s.stats.stt
+            st.tmax = np.maximum(st.tmax, s.stats.tmax)
         return TrainingSet(_lib.DeviceArray.concatenate([s.X for s in sets]),
                            _lib.DeviceArray.concatenate([s.T for s in sets]), st)
 
@@ -243,7 +265,8 @@ def training_pairs(tangles: Iterable[Sequence], settings: LearnedSettings, Γ: f
 
     Each tangle is evaluated once with the cheap splitting in FP64 (point data built on the
     device), then the converged − cheap difference is computed exactly on every node. The
-    per-column statistics (for the normalisation) come from every 8th row.
+    per-column statistics (for the normalisation) come from every 8th row, the output bound from
+    every row.
     """
     p, ctx, conv = _context(settings, np.float64, Γ=Γ, device=device)
     ctx.learned_setup(None, conv.α, conv.rcut, settings.feature_radius, 0.0, 1.0)
@@ -259,6 +282,7 @@ def training_pairs(tangles: Iterable[Sequence], settings: LearnedSettings, Γ: f
         ctx.learned_features(X, None)
         ctx.learned_difference(None, T)
         stats.add(X.rows(8), T.rows(8))
+        stats.bound(T.numpy())
         Xs.append(X)
         Ts.append(T)
         if log:
