@@ -39,7 +39,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(vp, cfg: int, sigma: float = 1.5, m: int = 4, fft: str = "fused"):
+class DeviceWorkload:
+    """The filaments only (point data built on the device by vpbs_set_filaments)."""
+
+    def __init__(self, batch):
+        self.batch = batch
+        self.Np = batch.Np
+        self.Nq = 0
+
+
+def build_workload(vp, cfg: int, sigma: float = 1.5, m: int = 4, fft: str = "fused", host_points: bool = True):
     fs = vp.tangles.config_tangle(cfg)
     Ns = {1: 64, 2: 128, 3: 512, 4: 256, 5: 1024}[cfg]
     spl = vp.GaussianSplitting.create(Ls=2 * math.pi, β=3.5, Ns=Ns)
@@ -47,6 +56,8 @@ def build_workload(vp, cfg: int, sigma: float = 1.5, m: int = 4, fft: str = "fus
     p32 = vp.ParamsBiotSavart.build(np.float32, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl, backend_long=bl)
     p64 = vp.ParamsBiotSavart.build(np.float64, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl, backend_long=bl)
     batch = vp.FilamentBatch(fs)
+    if not host_points:
+        return fs, spl, p32, p64, DeviceWorkload(batch)
     t = time.perf_counter()
     pd = vp.add_point_charges(fs, p64, batch=batch)
     pd.host_build_s = time.perf_counter() - t
@@ -409,10 +420,11 @@ def time_learned(vp, pd, spl, steps: int, warmup: int, weights=None):
 
 def time_advance(vp, pd, spl, steps: int, warmup: int, dt: float, weights=None):
     """SSPRK33 steps with the filaments on the device (vpbs_advance, 3 velocity evaluations per
-    step), FP32 bulk; with a trained network every evaluation is the learned one."""
+    step), FP32 bulk; with a trained network (``weights`` "none": the plain FP32 path) every
+    evaluation is the learned one."""
     L = vp.learned
-    path = pathlib.Path(weights) if weights else L.DEFAULT_WEIGHTS
-    net = L.CorrectionNetwork.load(path) if path.exists() else None
+    path = pathlib.Path(weights) if weights and weights != "none" else L.DEFAULT_WEIGHTS
+    net = L.CorrectionNetwork.load(path) if weights != "none" and path.exists() else None
     if net is not None:
         st = learned_settings(vp, net, spl)
         lb = L.LearnedBiotSavart(st, net, T=np.float32, sample_fraction=0.01, tol=1e-4)
@@ -471,7 +483,7 @@ def main():
 
     vp = ge.load_package()
     t = time.perf_counter()
-    fs, spl, p32, p64, pd = build_workload(vp, args.config, args.sigma, args.m, args.fft)
+    fs, spl, p32, p64, pd = build_workload(vp, args.config, args.sigma, args.m, args.fft, host_points=not args.advance)
     log(f"workload: {len(fs)} filaments, Np = {pd.Np}, Nq = {pd.Nq}, Ns = {spl.Ns}, α = {spl.α:.4f}, "
         f"rcut = {spl.rcut:.5f} (host prep {time.perf_counter() - t:.1f} s)")
 
