@@ -418,10 +418,10 @@ def time_learned(vp, pd, spl, steps: int, warmup: int, weights=None):
                         "held_out": net.meta.get("held_out", {})}}
 
 
-def time_advance(vp, pd, spl, steps: int, warmup: int, dt: float, weights=None):
+def time_advance(vp, pd, spl, steps: int, warmup: int, dt: float, weights=None, T=np.float32):
     """SSPRK33 steps with the filaments on the device (vpbs_advance, 3 velocity evaluations per
-    step), FP32 bulk; with a trained network (``weights`` "none": the plain FP32 path) every
-    evaluation is the learned one."""
+    step): with a trained network every evaluation is the learned one (FP32 bulk); with
+    ``weights`` "none" the plain path in precision T."""
     L = vp.learned
     path = pathlib.Path(weights) if weights and weights != "none" else L.DEFAULT_WEIGHTS
     net = L.CorrectionNetwork.load(path) if weights != "none" and path.exists() else None
@@ -430,7 +430,7 @@ def time_advance(vp, pd, spl, steps: int, warmup: int, dt: float, weights=None):
         lb = L.LearnedBiotSavart(st, net, T=np.float32, sample_fraction=0.01, tol=1e-4)
         ctx, params = lb.ctx, lb.params
     else:
-        params = vp.ParamsBiotSavart.build(np.float32, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl)
+        params = vp.ParamsBiotSavart.build(T, Γ=1.0, a=1e-6, Δ=0.25, splitting=spl)
         ctx, lb = vp._lib.Context(params), None
     ctx.set_filaments(pd.batch, params)
     if warmup:
@@ -439,7 +439,8 @@ def time_advance(vp, pd, spl, steps: int, warmup: int, dt: float, weights=None):
     ctx.advance(dt, steps)
     wall = time.perf_counter() - t0
     out = {"steps": steps, "evaluations": 3 * steps, "dt": dt, "seconds": wall,
-           "ms_per_evaluation": wall * 1e3 / (3 * steps), "learned": net is not None}
+           "ms_per_evaluation": wall * 1e3 / (3 * steps), "learned": net is not None,
+           "precision": "f32" if net is not None or T == np.float32 else "f64"}
     if lb is not None:
         out["learned_stats"] = ctx.learned_stats()
         lb.close()
@@ -470,6 +471,7 @@ def main():
                     help="time STEPS SSPRK33 steps with the filaments on the device (vpbs_advance, 3 evaluations "
                          "per step; learned evaluations when a network is available) and exit")
     ap.add_argument("--dt", type=float, default=1e-5, help="time step of --advance")
+    ap.add_argument("--fp64-advance", action="store_true", help="--advance with --weights none: the plain FP64 path")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB copy kernel (counter calibration for rocprofv3 --pmc) and exit")
     args = ap.parse_args()
@@ -488,7 +490,8 @@ def main():
         f"rcut = {spl.rcut:.5f} (host prep {time.perf_counter() - t:.1f} s)")
 
     if args.advance:
-        r = time_advance(vp, pd, spl, args.advance, args.warmup, args.dt, args.weights)
+        r = time_advance(vp, pd, spl, args.advance, args.warmup, args.dt, args.weights,
+                         np.float64 if args.fp64_advance else np.float32)
         r.update({"workload": f"config {args.config}: {len(fs)} filaments, {pd.Np} nodes, Fourier grid {spl.Ns}",
                   "node_velocities_per_s": pd.Np / (r["ms_per_evaluation"] * 1e-3)})
         print(json.dumps(r), flush=True)

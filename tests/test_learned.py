@@ -200,3 +200,30 @@ def test_committed_network_gain(vp):
     assert rep["err_corrected"] < 1e-4 and gain > 2
     assert e64 < 1e-4
     assert worst_l < 2 * worst_c
+
+
+def test_advance_with_network(vp, small):
+    """vpbs_advance with a network set up makes every evaluation a learned one (3 per SSPRK33 step,
+    counted by vpbs_learned_stats). With a network that predicts zero and a tolerance that
+    accepts it, the learned velocity is the cheap one, so the filaments end where the plain
+    cheap splitting moves them."""
+    fs, st = small
+    dt = 1e-3
+    batch = vp.FilamentBatch(fs)
+    lb = vp.learned.LearnedBiotSavart(st, _zero_net(vp), T=np.float64, sample_fraction=0.05, tol=1.0, seed=1)
+    lb.ctx.set_filaments(batch, lb.params)
+    lb.ctx.advance(dt, 2)
+    X_learned = lb.ctx.get_filaments()["X"]
+    stats = lb.ctx.learned_stats()
+    lb.close()
+    p = st.params(np.float64)
+    ctx = vp._lib.Context(p)
+    ctx.set_filaments(batch, p)
+    ctx.advance(dt, 2)
+    X_plain = ctx.get_filaments()["X"]
+    ctx.close()
+    moved = np.abs(X_plain - batch.X).max()
+    print(f"  learned stats {stats}; largest node displacement {moved:.3e}")
+    assert stats["calls"] == 6 and stats["fallbacks"] == 0
+    assert moved > 1e-5
+    assert np.abs(X_learned - X_plain).max() <= 1e-12 * max(1.0, np.abs(X_plain).max())
