@@ -11,8 +11,10 @@
 // (FP64 pair sums, the kept Fourier field v̂ with the difference filter) to make training
 // targets and to judge the network at run time:
 //
-//   strand_records    cell-sorted node positions (the node cell list of the pair kernel), unit
-//                     tangents and the offsets to the two neighbours along each filament
+//   strand_records    per node in cell-sorted order (the node cell list of the pair kernel):
+//                     position, unit tangent and the offsets to the two neighbours along its
+//                     filament (read as a candidate strand point), and the node's own set-up
+//                     (local frame, filament, cell) read by the wavefront that builds its row
 //   feature_kernel    one wavefront per node: local frame (t, n, b) from s′, s″; the K
 //                     nearest strands within the feature radius, one point per strand (the
 //                     node where the distance along that strand has a local minimum, moved
@@ -82,15 +84,28 @@ __device__ inline Frame node_frame(const double *sp, const double *spp, int64_t 
     return f;
 }
 
+// A node's own set-up for the feature row (one 128-byte record per slot, read with uniform
+// loads by the node's wavefront): local frame t, n, b and |κ| (node_frame), the filament's
+// first node and length, the node's place on it and its cell.
+struct __align__(16) NodeSet {
+    double f[10];  // t, n, b, |κ|
+    int64_t oi;    // first node of the filament
+    int32_t i, Ni, pi, pad;
+    int4 cell;     // (ix, iy, iz, key) of the node cell list
+};
+
 // Per node, in cell-sorted order (slot k of the node cell list, node i = perm[k]): its
 // position (FP64) and a record of float4 × 3: (t, i) with t the unit tangent s′/|s′| and the
 // node index as bits, then w⁻ = s_{i−1} − s_i and w⁺ = s_{i+1} − s_i (minimum image, along
 // i's filament). With r = s_i − x for a target x, |x − s_{i∓1}|² − |x − s_i|² = 2 r·w∓ + |w∓|²,
-// so the local-minimum test of the features needs no neighbour loads.
+// so the local-minimum test of the features needs no neighbour loads. Also the node's set-up
+// (NodeSet) and its frame in FP32 for the correction ([Np][9], original order).
 __global__ void strand_records(int64_t Np, const int32_t *__restrict__ perm, const double *__restrict__ nodes,
-                               const double *__restrict__ sp, const int32_t *__restrict__ fid,
-                               const int64_t *__restrict__ foff, const int32_t *__restrict__ fn, Box box,
-                               double *__restrict__ npos, float4 *__restrict__ rec) {
+                               const double *__restrict__ sp, const double *__restrict__ spp,
+                               const int32_t *__restrict__ fid, const int64_t *__restrict__ foff,
+                               const int32_t *__restrict__ fn, const int4 *__restrict__ tcell, Box box, double alpha,
+                               double *__restrict__ npos, float4 *__restrict__ rec, NodeSet *__restrict__ nset,
+                               float *__restrict__ frame) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= Np) return;
     const int32_t i = perm[k];
@@ -113,26 +128,39 @@ __global__ void strand_records(int64_t Np, const int32_t *__restrict__ perm, con
     rec[3 * k] = make_float4((float)(s1x / nrm), (float)(s1y / nrm), (float)(s1z / nrm), __int_as_float(i));
     rec[3 * k + 1] = make_float4(wmx, wmy, wmz, wpx);
     rec[3 * k + 2] = make_float4(wpy, wpz, 0.f, 0.f);
+    const Frame F = node_frame(sp, spp, Np, i, alpha);
+    NodeSet ns;
+    for (int d = 0; d < 3; d++) {
+        ns.f[d] = F.t[d];
+        ns.f[3 + d] = F.n[d];
+        ns.f[6 + d] = F.b[d];
+        frame[(int64_t)i * 9 + d] = (float)F.t[d];
+        frame[(int64_t)i * 9 + 3 + d] = (float)F.n[d];
+        frame[(int64_t)i * 9 + 6 + d] = (float)F.b[d];
+    }
+    ns.f[9] = F.kappa;
+    ns.oi = o;
+    ns.i = i;
+    ns.Ni = n;
+    ns.pi = p;
+    ns.pad = 0;
+    ns.cell = tcell[i];
+    nset[k] = ns;
 }
 
 struct FeatArgs {
     int64_t Np;
-    const int32_t *tperm;   // cell-sorted slot → node
-    const int4 *tcell;      // per node: (ix, iy, iz, key)
+    const NodeSet *nset;    // cell-sorted node set-up (strand_records)
     const int32_t *tstart;  // node cell starts
     const double *npos;     // cell-sorted node positions [3][Np]
     const float4 *rec;      // cell-sorted strand records [Np][3] (strand_records)
-    const double *nodes, *sp, *spp;
-    const int32_t *fid;
-    const int64_t *foff;
-    const int32_t *fn;
+    const double *nodes;
     Box box;
     int3 Nc;
     int Mf;
     double rf2;   // feature radius²
     double alpha; // α_c (lengths in units of 1/α_c)
     float *feat;  // [Np][LC_NIN]
-    float *frame; // [Np][9]
 };
 
 // One wavefront per node (cell-sorted order, XCD-aware block order).
@@ -148,13 +176,17 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
     const int64_t per = ((int64_t)gridDim.x + 7) / 8;
     const int64_t k = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (k >= Np) return;
-    const int32_t i = a.tperm[k];
-    const double x[3] = {a.nodes[i], a.nodes[Np + i], a.nodes[2 * Np + i]};
-    const Frame F = node_frame(a.sp, a.spp, Np, i, a.alpha);
-    const int fi = a.fid[i];
-    const int64_t oi = a.foff[fi];
-    const int Ni = a.fn[fi];
-    const int pi = (int)(i - oi);
+    // the node's set-up: uniform loads, so the stretch and the cell rows below load at once
+    const NodeSet *__restrict__ ns = a.nset + k;
+    const int32_t i = ns->i;
+    const double x[3] = {a.npos[k], a.npos[Np + k], a.npos[2 * Np + k]};
+    const double Ft[3] = {ns->f[0], ns->f[1], ns->f[2]};
+    const double Fn[3] = {ns->f[3], ns->f[4], ns->f[5]};
+    const double Fb[3] = {ns->f[6], ns->f[7], ns->f[8]};
+    const double kappa = ns->f[9];
+    const int64_t oi = ns->oi;
+    const int Ni = ns->Ni;
+    const int pi = ns->pi;
 
     // the node's own stretch: consecutive neighbours along its filament within the radius
     int dplus, dminus;
@@ -174,7 +206,7 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
 
     // cell rows of the walk: lanes take rows lane and lane + 64 (≤ 2 runs each with the
     // periodic wrap along x); a wave scan gives each row's first candidate
-    const int4 tc = a.tcell[i];
+    const int4 tc = ns->cell;
     const int M = a.Mf, W2 = 2 * M + 1, nrows = W2 * W2;
     int len[2] = {0, 0};
     int4 run[2];
@@ -278,12 +310,12 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
 #pragma clang fp contract(off)
                 const float rt = (rx * ra.x + ry * ra.y) + rz * ra.z;
                 const float qx = rx - rt * ra.x, qy = ry - rt * ra.y, qz = rz - rt * ra.z;
-                fv[0] = (float)(a.alpha * ((qx * F.t[0] + qy * F.t[1]) + qz * F.t[2]));
-                fv[1] = (float)(a.alpha * ((qx * F.n[0] + qy * F.n[1]) + qz * F.n[2]));
-                fv[2] = (float)(a.alpha * ((qx * F.b[0] + qy * F.b[1]) + qz * F.b[2]));
-                fv[3] = (float)((ra.x * F.t[0] + ra.y * F.t[1]) + ra.z * F.t[2]);
-                fv[4] = (float)((ra.x * F.n[0] + ra.y * F.n[1]) + ra.z * F.n[2]);
-                fv[5] = (float)((ra.x * F.b[0] + ra.y * F.b[1]) + ra.z * F.b[2]);
+                fv[0] = (float)(a.alpha * ((qx * Ft[0] + qy * Ft[1]) + qz * Ft[2]));
+                fv[1] = (float)(a.alpha * ((qx * Fn[0] + qy * Fn[1]) + qz * Fn[2]));
+                fv[2] = (float)(a.alpha * ((qx * Fb[0] + qy * Fb[1]) + qz * Fb[2]));
+                fv[3] = (float)((ra.x * Ft[0] + ra.y * Ft[1]) + ra.z * Ft[2]);
+                fv[4] = (float)((ra.x * Fn[0] + ra.y * Fn[1]) + ra.z * Fn[2]);
+                fv[5] = (float)((ra.x * Fb[0] + ra.y * Fb[1]) + ra.z * Fb[2]);
                 key = (qx * qx + qy * qy) + qz * qz;
                 acc = true;
             }
@@ -317,21 +349,12 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
             for (int f = 0; f < 6; f++) row[6 + 6 * rank + f] = q_f[f][e];
     }
     if (lane == 0) {
-        row[1] = (float)(F.kappa / a.alpha);  // curvature vector in the frame: (0, |κ|/α, 0)
+        row[1] = (float)(kappa / a.alpha);    // curvature vector in the frame: (0, |κ|/α, 0)
         row[3] = 1.f;                         // tangent in the frame: (1, 0, 0)
     }
     __syncthreads();
     float4 *dst = (float4 *)(a.feat + (int64_t)i * LC_NIN);
     if (lane < LC_NIN / 4) dst[lane] = make_float4(row[4 * lane], row[4 * lane + 1], row[4 * lane + 2], row[4 * lane + 3]);
-    if (lane < 3) {
-        float *fr = a.frame + (int64_t)i * 9;
-        const double t = lane == 0 ? F.t[0] : (lane == 1 ? F.t[1] : F.t[2]);
-        const double n = lane == 0 ? F.n[0] : (lane == 1 ? F.n[1] : F.n[2]);
-        const double b = lane == 0 ? F.b[0] : (lane == 1 ? F.b[1] : F.b[2]);
-        fr[lane] = (float)t;
-        fr[3 + lane] = (float)n;
-        fr[6 + lane] = (float)b;
-    }
 }
 
 struct CheckArgs {
@@ -553,6 +576,7 @@ int lc_buffers(vpbs_ctx *c) {
     if (Np > c->lc_cap) {
         VPBS_TRY(regrow(c, &c->lc_npos, 3 * Np));
         VPBS_TRY(regrow(c, &c->lc_rec, 12 * Np));
+        VPBS_TRY(regrow(c, (NodeSet **)&c->lc_nset, Np));
         VPBS_TRY(regrow(c, &c->lc_feat, LC_NIN * Np));
         VPBS_TRY(regrow(c, &c->lc_frame, 9 * Np));
         VPBS_TRY(regrow(c, &c->lc_y, 3 * Np));
@@ -580,21 +604,17 @@ int sample_buffers(vpbs_ctx *c, int64_t ns) {
 int run_features(vpbs_ctx *c, hipStream_t s) {
     const int64_t Np = c->Np;
     VPBS_TRY(lc_buffers(c));
-    strand_records<<<nblk(Np, 256), 256, 0, s>>>(Np, c->tgt_perm, c->nodes, c->sp, c->fil_fid, c->fil_off, c->fil_n,
-                                                 c->box, c->lc_npos, (float4 *)c->lc_rec);
+    strand_records<<<nblk(Np, 256), 256, 0, s>>>(Np, c->tgt_perm, c->nodes, c->sp, c->spp, c->fil_fid, c->fil_off,
+                                                 c->fil_n, (const int4 *)c->tgt_cell, c->box, c->lc_alpha_c,
+                                                 c->lc_npos, (float4 *)c->lc_rec, (NodeSet *)c->lc_nset,
+                                                 c->lc_frame);
     FeatArgs a;
     a.Np = Np;
-    a.tperm = c->tgt_perm;
-    a.tcell = (const int4 *)c->tgt_cell;
+    a.nset = (const NodeSet *)c->lc_nset;
     a.tstart = c->tgt_start;
     a.npos = c->lc_npos;
     a.rec = (const float4 *)c->lc_rec;
     a.nodes = c->nodes;
-    a.sp = c->sp;
-    a.spp = c->spp;
-    a.fid = c->fil_fid;
-    a.foff = c->fil_off;
-    a.fn = c->fil_n;
     a.box = c->box;
     a.Nc = make_int3(c->Nc[0], c->Nc[1], c->Nc[2]);
     int Mf = 0;
@@ -603,7 +623,6 @@ int run_features(vpbs_ctx *c, hipStream_t s) {
     a.rf2 = c->lc_rfeat * c->lc_rfeat;
     a.alpha = c->lc_alpha_c;
     a.feat = c->lc_feat;
-    a.frame = c->lc_frame;
     feature_kernel<<<(unsigned)(8 * ((Np + 7) / 8)), 64, 0, s>>>(a);
     VPBS_HIP(c, hipGetLastError());
     return VPBS_OK;
@@ -706,7 +725,7 @@ int learned_after_merge(vpbs_ctx *c) {
 }
 
 void learned_release(vpbs_ctx *c) {
-    void *bufs[] = {c->lc_npos, c->lc_rec, c->lc_feat, c->lc_frame, c->lc_y, c->lc_dvp, c->lc_dsr, c->lc_dlr, c->lc_sidx,
+    void *bufs[] = {c->lc_npos, c->lc_rec, c->lc_nset, c->lc_feat, c->lc_frame, c->lc_y, c->lc_dvp, c->lc_dsr, c->lc_dlr, c->lc_sidx,
                     c->lc_spos, c->lc_stats};
     for (void *b : bufs)
         if (b) (void)hipFree(b);

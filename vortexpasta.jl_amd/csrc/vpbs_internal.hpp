@@ -189,6 +189,7 @@ struct vpbs_ctx {
     int64_t lc_cap = 0, lc_cap_s = 0;
     double *lc_npos = nullptr;        // cell-sorted node positions [3][Np]
     float *lc_rec = nullptr;          // cell-sorted strand records [Np][12] (tangent, index, neighbour offsets)
+    void *lc_nset = nullptr;          // cell-sorted node set-up records [Np] × 128 B (frame, filament, cell)
     float *lc_feat = nullptr;         // features [Np][128]
     float *lc_frame = nullptr;        // local frames (t, n, b) [Np][9]
     float *lc_y = nullptr;            // network outputs [Np][3]
