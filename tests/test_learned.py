@@ -68,10 +68,15 @@ def test_difference_vs_oracle(vp, O, small):
     assert 1e-5 < rel < 1e-1
 
 
-def test_features_vs_restatement(vp, small, LO):
-    """60 rings (≈2.9k nodes): ~1 strand per node on average within the feature radius, up to 6."""
-    _, st = small
-    fs = vp.tangles.random_rings(60, 0.5, 1.0, TWO_PI / 64, seed=11)
+@pytest.mark.parametrize("nrings,rmin,spacing,factor", [(60, 0.5, TWO_PI / 64, 2.0), (150, 0.3, TWO_PI / 96, 1.5)],
+                         ids=["60rings-2rcut", "150rings-1.5rcut"])
+def test_features_vs_restatement(vp, LO, nrings, rmin, spacing, factor):
+    """60 rings (≈2.9k nodes, feature radius 2 rcut: 9 × 9 cell rows) and 150 denser rings
+    (≈9k nodes, 1.5 rcut: 7 × 7 rows): ~1 strand per node on average within the radius, up
+    to 6. The restatement finds neighbours by brute force, so the kernel's cell walk and its
+    cut of the rows to the sphere are checked as well."""
+    st = vp.learned.LearnedSettings.create(TWO_PI, 64, feature_factor=factor)
+    fs = vp.tangles.random_rings(nrings, rmin, 1.0, spacing, seed=11)
     p_c = st.params(np.float64)
     batch = vp.FilamentBatch(fs)
     L = vp._lib

@@ -158,6 +158,9 @@ struct FeatArgs {
     Box box;
     int3 Nc;
     int Mf;
+    int w2inv;    // ⌈65536 / (2·Mf + 1)⌉: row r of the walk is in z-layer (r · w2inv) >> 16
+    double cinv[3];  // Nc / L: cells per unit length
+    double csz[3];   // L / Nc: cell sizes
     double rf2;   // feature radius²
     double alpha; // α_c (lengths in units of 1/α_c)
     float *feat;  // [Np][LC_NIN]
@@ -192,7 +195,13 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
     int dplus, dminus;
     {
         const int step = lane < 32 ? lane + 1 : -(lane - 31);
-        const int p = ((pi + step) % Ni + Ni) % Ni;
+        int p = pi + step;
+        if (Ni > 32) {  // one wrap at most
+            p += (p < 0) ? Ni : 0;
+            p -= (p >= Ni) ? Ni : 0;
+        } else {
+            p = (p % Ni + Ni) % Ni;
+        }
         const int64_t j = oi + p;
         const double r0 = fold_sep(a.nodes[j] - x[0], a.box.L[0], a.box.Lh[0]);
         const double r1 = fold_sep(a.nodes[Np + j] - x[1], a.box.L[1], a.box.Lh[1]);
@@ -205,33 +214,56 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
     }
 
     // cell rows of the walk: lanes take rows lane and lane + 64 (≤ 2 runs each with the
-    // periodic wrap along x); a wave scan gives each row's first candidate
+    // periodic wrap along x); a wave scan gives each row's first candidate. Each row keeps
+    // only the cells that can hold a point within the radius: rows whose (y, z) gap to the
+    // node exceeds it are empty, the others are cut to the x-cells within
+    // √(r_f² − gap²) of the node (widened by 1e-9 cells: a conservative cut; the distance
+    // test below decides).
     const int4 tc = ns->cell;
     const int M = a.Mf, W2 = 2 * M + 1, nrows = W2 * W2;
+    // the node's place in its cell (in cells; within a few ulps of the cell index's own
+    // (x / L)·Nc, far inside the 1e-9 widening)
+    const double u[3] = {x[0] * a.cinv[0] - tc.x, x[1] * a.cinv[1] - tc.y, x[2] * a.cinv[2] - tc.z};
     int len[2] = {0, 0};
     int4 run[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         const int r = lane + 64 * h;
         if (r < nrows) {
-            const int qz = r / W2;
-            int zz = tc.z + qz - M, yy = tc.y + (r - qz * W2) - M;
+            const int qz = (r * a.w2inv) >> 16, dz = qz - M, dy = (r - qz * W2) - M;
+            int zz = tc.z + dz, yy = tc.y + dy;
             zz += (zz < 0) ? a.Nc.z : 0;
             zz -= (zz >= a.Nc.z) ? a.Nc.z : 0;
             yy += (yy < 0) ? a.Nc.y : 0;
             yy -= (yy >= a.Nc.y) ? a.Nc.y : 0;
             const int64_t rb = ((int64_t)zz * a.Nc.y + yy) * a.Nc.x;
-            const int lo = tc.x - M, hi = tc.x + M;
-            int ca0, cb0, ca1 = 0, cb1 = -1;
-            if (lo < 0) {
+            const double gy = fmax((dy > 0 ? dy - u[1] : (dy < 0 ? u[1] - (dy + 1) : 0.0)) - 1e-9, 0.0) * a.csz[1];
+            const double gz = fmax((dz > 0 ? dz - u[2] : (dz < 0 ? u[2] - (dz + 1) : 0.0)) - 1e-9, 0.0) * a.csz[2];
+            const double rem = a.rf2 - (gy * gy + gz * gz);
+            int lo = 1, hi = 0;  // empty
+            if (rem >= 0.0) {
+                const double ext = sqrt(rem) * a.cinv[0] + 1e-9;
+                lo = tc.x + max((int)floor(u[0] - ext), -M);
+                hi = tc.x + min((int)floor(u[0] + ext), M);
+            }
+            int ca0 = 0, cb0 = -1, ca1 = 0, cb1 = -1;
+            if (lo > hi) {
+            } else if (hi < 0) {
+                ca0 = lo + a.Nc.x, cb0 = hi + a.Nc.x;
+            } else if (lo < 0) {
                 ca0 = lo + a.Nc.x, cb0 = a.Nc.x - 1, cb1 = hi;
+            } else if (lo >= a.Nc.x) {
+                ca0 = lo - a.Nc.x, cb0 = hi - a.Nc.x;
             } else if (hi >= a.Nc.x) {
                 ca0 = lo, cb0 = a.Nc.x - 1, cb1 = hi - a.Nc.x;
             } else {
                 ca0 = lo, cb0 = hi;
             }
-            const int s0 = a.tstart[rb + ca0], n0 = a.tstart[rb + cb0 + 1] - s0;
-            int s1 = 0, n1 = 0;
+            int s0 = 0, n0 = 0, s1 = 0, n1 = 0;
+            if (cb0 >= ca0) {
+                s0 = a.tstart[rb + ca0];
+                n0 = a.tstart[rb + cb0 + 1] - s0;
+            }
             if (cb1 >= ca1) {
                 s1 = a.tstart[rb + ca1];
                 n1 = a.tstart[rb + cb1 + 1] - s1;
@@ -253,25 +285,32 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
     }
     const int ncand = nrows > 64 ? __builtin_amdgcn_readlane(incl1, nrows - 65)
                                  : __builtin_amdgcn_readlane(incl0, nrows - 1);
-    for (int e = lane; e < LC_NIN; e += 64) row[e] = 0.f;
+    int *const s_mark = reinterpret_cast<int *>(row);  // row marks of the batch (row[] is zeroed after the walk)
     __syncthreads();
 
-    // Candidate b + lane: its row by a wave-uniform walk over the row boundaries (each passed
-    // once per node), then its slot, record and position, loaded one batch ahead.
-    int rw = 0;   // uniform: last row starting at or before the fetched batch's end
-    int crow = 0;
+    // Candidate b + lane: its row from markers (each non-empty row that meets the batch marks
+    // its first candidate in it; a max-scan carries the marks forward), then its slot, record
+    // and position, loaded one batch ahead.
     auto fetch = [&](int b, int &fs) {
-        const int cnd = b + lane;
-        crow = rw;
-        while (rw + 1 < nrows) {
-            const int next = rw < 64 ? __builtin_amdgcn_readlane(incl0, rw) : __builtin_amdgcn_readlane(incl1, rw - 64);
-            if (next > b + 63) break;
-            rw++;
-            crow += cnd >= next;
+        s_mark[lane] = -1;
+        wave_lds_order();
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int r = lane + 64 * h;
+            if (r < nrows && len[h] > 0) {
+                const int st = run[h].x, en = st + len[h];
+                if (st < b + 64 && en > b) s_mark[st > b ? st - b : 0] = r;
+            }
         }
-        const int4 rr = r_run[crow];
-        const int o = cnd - rr.x;
-        fs = cnd < ncand ? (o < rr.z ? rr.y : rr.w) + o : 0;
+        wave_lds_order();
+        const int crow = wave_incl_max(s_mark[lane]);
+        const int cnd = b + lane;
+        fs = 0;
+        if (cnd < ncand) {
+            const int4 rr = r_run[crow];
+            const int o = cnd - rr.x;
+            fs = (o < rr.z ? rr.y : rr.w) + o;
+        }
     };
     int s = 0;
     if (ncand > 0) fetch(0, s);
@@ -295,7 +334,7 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
             const double d2 = dist2_nocontract(r0, r1, r2v);
             bool ok = cnd < ncand && j != i && d2 <= a.rf2;
             if (ok && (unsigned)(j - oi) < (unsigned)Ni) {  // i's own filament
-                const int fo = ((j - i) % Ni + Ni) % Ni;  // steps forward from i to j
+                const int fo = (j - i) + ((j < i) ? Ni : 0);  // steps forward from i to j
                 if (fo <= dplus || Ni - fo <= dminus) ok = false;
             }
             const float rx = (float)r0, ry = (float)r1, rz = (float)r2v;
@@ -307,15 +346,13 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
                 ok = gm >= 0.f && gp > 0.f;
             }
             if (ok) {
+                // foot of the perpendicular on the strand's tangent line; projected onto the
+                // frame only for the K strands kept
 #pragma clang fp contract(off)
                 const float rt = (rx * ra.x + ry * ra.y) + rz * ra.z;
                 const float qx = rx - rt * ra.x, qy = ry - rt * ra.y, qz = rz - rt * ra.z;
-                fv[0] = (float)(a.alpha * ((qx * Ft[0] + qy * Ft[1]) + qz * Ft[2]));
-                fv[1] = (float)(a.alpha * ((qx * Fn[0] + qy * Fn[1]) + qz * Fn[2]));
-                fv[2] = (float)(a.alpha * ((qx * Fb[0] + qy * Fb[1]) + qz * Fb[2]));
-                fv[3] = (float)((ra.x * Ft[0] + ra.y * Ft[1]) + ra.z * Ft[2]);
-                fv[4] = (float)((ra.x * Fn[0] + ra.y * Fn[1]) + ra.z * Fn[2]);
-                fv[5] = (float)((ra.x * Fb[0] + ra.y * Fb[1]) + ra.z * Fb[2]);
+                fv[0] = qx, fv[1] = qy, fv[2] = qz;
+                fv[3] = ra.x, fv[4] = ra.y, fv[5] = ra.z;
                 key = (qx * qx + qy * qy) + qz * qz;
                 acc = true;
             }
@@ -334,6 +371,7 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
         }
         nq += __popcll(mask);
     }
+    for (int e = lane; e < LC_NIN; e += 64) row[e] = 0.f;
     __syncthreads();
     nq = nq < LC_QMAX ? nq : LC_QMAX;
     // the K nearest strands (by perpendicular distance, ties by node index) in order
@@ -345,8 +383,18 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
             const float ko = q_key[o];
             rank += (ko < ke) || (ko == ke && q_j[o] < je);
         }
-        if (rank < LC_K)
-            for (int f = 0; f < 6; f++) row[6 + 6 * rank + f] = q_f[f][e];
+        if (rank < LC_K) {
+#pragma clang fp contract(off)
+            const float qx = q_f[0][e], qy = q_f[1][e], qz = q_f[2][e];
+            const float tx = q_f[3][e], ty = q_f[4][e], tz = q_f[5][e];
+            float *rr = row + 6 + 6 * rank;
+            rr[0] = (float)(a.alpha * ((qx * Ft[0] + qy * Ft[1]) + qz * Ft[2]));
+            rr[1] = (float)(a.alpha * ((qx * Fn[0] + qy * Fn[1]) + qz * Fn[2]));
+            rr[2] = (float)(a.alpha * ((qx * Fb[0] + qy * Fb[1]) + qz * Fb[2]));
+            rr[3] = (float)((tx * Ft[0] + ty * Ft[1]) + tz * Ft[2]);
+            rr[4] = (float)((tx * Fn[0] + ty * Fn[1]) + tz * Fn[2]);
+            rr[5] = (float)((tx * Fb[0] + ty * Fb[1]) + tz * Fb[2]);
+        }
     }
     if (lane == 0) {
         row[1] = (float)(kappa / a.alpha);    // curvature vector in the frame: (0, |κ|/α, 0)
@@ -620,6 +668,11 @@ int run_features(vpbs_ctx *c, hipStream_t s) {
     int Mf = 0;
     for (int d = 0; d < 3; d++) Mf = std::max(Mf, (int)std::ceil(c->lc_rfeat * c->Nc[d] / c->p.Ls[d] - 1e-12));
     a.Mf = Mf;
+    a.w2inv = (65536 + 2 * Mf) / (2 * Mf + 1);
+    for (int d = 0; d < 3; d++) {
+        a.cinv[d] = c->Nc[d] / c->p.Ls[d];
+        a.csz[d] = c->p.Ls[d] / c->Nc[d];
+    }
     a.rf2 = c->lc_rfeat * c->lc_rfeat;
     a.alpha = c->lc_alpha_c;
     a.feat = c->lc_feat;
