@@ -189,9 +189,11 @@ def test_fp32_bulk_path(vp, O, cfg2):
 
 
 def test_pair_candidate_count(vp, cfg2):
-    """vpbs_timings.pair_candidates after a VPBS_NEIGHBOUR_STATS call equals the number of sources in
-    the (2M+1)³ neighbour cells of every node (periodic 3D histogram of the sources, cell index
-    trunc(x/L·N) as in the cell-list build), and is at least the accepted pair count; 0 otherwise."""
+    """vpbs_timings.pair_candidates after a VPBS_NEIGHBOUR_STATS call lies between the accepted
+    pair count and the number of sources in the (2M+1)³ neighbour cells of every node (periodic
+    3D histogram of the sources, cell index trunc(x/L·N) as in the cell-list build): the walk
+    visits only the cells of that cube that the rcut sphere around the node reaches (about
+    2/3 of them for M = 2); 0 without the flag."""
     params = _params(vp, Ns=128)
     pd = vp.add_point_charges(cfg2, params)
     ctx = vp._lib.Context(params)
@@ -216,8 +218,8 @@ def test_pair_candidate_count(vp, cfg2):
                 S += np.roll(H, (dz, dy, dx), axis=(0, 1, 2))
     nx, ny, nz = cell(np.asarray(pd.nodes))
     want = int(S[nz, ny, nx].sum())
-    print(f"  candidates {got} (expected {want}), accepted {int(counts.sum())}", flush=True)
-    assert got == want and got >= counts.sum() > 0
+    print(f"  candidates {got} (cube {want}, {got / want:.2f}), accepted {int(counts.sum())}", flush=True)
+    assert want > got >= counts.sum() > 0 and got < 0.8 * want
 
 
 @pytest.mark.parametrize("M,Ns,cells", [(1, 26, 3), (1, 34, 4), (2, 22, 5), (2, 26, 6), (3, 24, 8), (4, 20, 9),

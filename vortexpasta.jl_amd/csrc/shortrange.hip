@@ -126,6 +126,8 @@ struct PairArgs {
     Box box;
     int3 Nc;
     int M;
+    double cinv[3];  // Nc / L: cells per unit length
+    double csz[3];   // L / Nc: cell sizes
     double rcut2;
     CT alpha;
     double pref;
@@ -196,6 +198,7 @@ __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
     __shared__ int4 r_run[PAIR_MAXROWS];
     __shared__ int q_s[128];
     __shared__ CT q_r[4][128];  // rx, ry, rz, r² of the queued pairs
+    __shared__ int s_mark[64];  // rows meeting the fetched batch, at their first candidate in it
 
     const int lane = threadIdx.x;
     const int64_t Np = a.Np, Nq = a.Nq;
@@ -220,7 +223,12 @@ __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
     // for r < 128, W2 ≤ 9
     const float rcp_w2 = __builtin_amdgcn_rcpf((float)W2);
 
-    // 1) lanes take rows lane and lane + 64
+    // 1) lanes take rows lane and lane + 64. Each row keeps only the cells that can hold a
+    //    source within rcut: rows whose (y, z) gap to the node exceeds rcut are empty, the
+    //    others are cut to the x-cells within √(rcut² − gap²) of the node (widened by 1e-9
+    //    cells: a conservative cut that drops only sources the r² test rejects, so the
+    //    accepted pairs and their order are unchanged)
+    const double u[3] = {x * a.cinv[0] - cx, y * a.cinv[1] - cy, z * a.cinv[2] - cz};  // place in the cell
     int len[2] = {0, 0};
     int4 run[2];
 #pragma unroll
@@ -228,23 +236,40 @@ __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
         const int r = lane + 64 * h;
         if (r < nrows) {
             const int qz = (int)(((float)r + 0.5f) * rcp_w2);
-            int zz = cz + qz - M, yy = cy + (r - qz * W2) - M;
+            const int dz = qz - M, dy = (r - qz * W2) - M;
+            int zz = cz + dz, yy = cy + dy;
             zz += (zz < 0) ? a.Nc.z : 0;
             zz -= (zz >= a.Nc.z) ? a.Nc.z : 0;
             yy += (yy < 0) ? a.Nc.y : 0;
             yy -= (yy >= a.Nc.y) ? a.Nc.y : 0;
             const int64_t row = ((int64_t)zz * a.Nc.y + yy) * a.Nc.x;
-            const int lo = cx - M, hi = cx + M;
-            int ca0, cb0, ca1 = 0, cb1 = -1;
-            if (lo < 0) {
+            const double gy = fmax((dy > 0 ? dy - u[1] : (dy < 0 ? u[1] - (dy + 1) : 0.0)) - 1e-9, 0.0) * a.csz[1];
+            const double gz = fmax((dz > 0 ? dz - u[2] : (dz < 0 ? u[2] - (dz + 1) : 0.0)) - 1e-9, 0.0) * a.csz[2];
+            const double rem = a.rcut2 - (gy * gy + gz * gz);
+            int lo = 1, hi = 0;  // empty
+            if (rem >= 0.0) {
+                const double ext = sqrt(rem) * a.cinv[0] + 1e-9;
+                lo = cx + max((int)floor(u[0] - ext), -M);
+                hi = cx + min((int)floor(u[0] + ext), M);
+            }
+            int ca0 = 0, cb0 = -1, ca1 = 0, cb1 = -1;
+            if (lo > hi) {
+            } else if (hi < 0) {
+                ca0 = lo + a.Nc.x, cb0 = hi + a.Nc.x;
+            } else if (lo < 0) {
                 ca0 = lo + a.Nc.x, cb0 = a.Nc.x - 1, cb1 = hi;
+            } else if (lo >= a.Nc.x) {
+                ca0 = lo - a.Nc.x, cb0 = hi - a.Nc.x;
             } else if (hi >= a.Nc.x) {
                 ca0 = lo, cb0 = a.Nc.x - 1, cb1 = hi - a.Nc.x;
             } else {
                 ca0 = lo, cb0 = hi;
             }
-            const int s0 = a.cstart[row + ca0], n0 = a.cstart[row + cb0 + 1] - s0;
-            int s1 = 0, n1 = 0;
+            int s0 = 0, n0 = 0, s1 = 0, n1 = 0;
+            if (cb0 >= ca0) {
+                s0 = a.cstart[row + ca0];
+                n0 = a.cstart[row + cb0 + 1] - s0;
+            }
             if (cb1 >= ca1) {
                 s1 = a.cstart[row + ca1];
                 n1 = a.cstart[row + cb1 + 1] - s1;
@@ -295,24 +320,31 @@ __global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
     };
 
     int nqd = 0;   // queued pairs (uniform)
-    int rw = 0;    // uniform: last row starting at or before the fetched batch's end
-    int row = 0;   // per lane: last row starting at or before its candidate
-    // Locates candidate b + lane (passing the row boundaries that fall in batch b) and issues
-    // the loads of its source, index and position together. Branch-free, so that the loads of
-    // one batch can stay in flight while the previous batch is tested: lanes past the last
-    // candidate read source 0 (there is at least one source when there are candidates).
+    // Locates candidate b + lane and issues the loads of its source, index and position
+    // together, so that the loads of one batch stay in flight while the previous batch is
+    // tested. Its row: every non-empty row that meets the batch marks its first candidate in
+    // it, and a max-scan carries the marks forward. Lanes past the last candidate read source
+    // 0 (there is at least one source when there are candidates).
     auto fetch = [&](int b, int &fs, int32_t &fj, double &fx, double &fy, double &fz) {
-        const int cnd = b + lane;
-        row = rw;  // every boundary passed so far lies at or before b
-        while (rw + 1 < nrows) {
-            const int next = rw < 64 ? __builtin_amdgcn_readlane(incl0, rw) : __builtin_amdgcn_readlane(incl1, rw - 64);
-            if (next > b + 63) break;
-            rw++;
-            row += cnd >= next;  // boundaries are non-decreasing: row counts those at or before cnd
+        s_mark[lane] = -1;
+        wave_lds_order();
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int r = lane + 64 * h;
+            if (r < nrows && len[h] > 0) {
+                const int st = run[h].x, en = st + len[h];
+                if (st < b + 64 && en > b) s_mark[st > b ? st - b : 0] = r;
+            }
         }
-        const int4 rr = r_run[row];  // row < nrows
-        const int o = cnd - rr.x;
-        fs = cnd < ncand ? (o < rr.z ? rr.y : rr.w) + o : 0;
+        wave_lds_order();
+        const int crow = wave_incl_max(s_mark[lane]);
+        const int cnd = b + lane;
+        fs = 0;
+        if (cnd < ncand) {
+            const int4 rr = r_run[crow];
+            const int o = cnd - rr.x;
+            fs = (o < rr.z ? rr.y : rr.w) + o;
+        }
         fj = a.sidx[fs];
         fx = a.spos[fs];
         fy = a.spos[Nq + fs];
@@ -813,6 +845,10 @@ int run_pairs(vpbs_ctx *c, bool vel, bool psi, bool stats, bool include_local) {
     a.box = c->box;
     a.Nc = Nc;
     a.M = c->p.nsubdiv;
+    for (int d = 0; d < 3; d++) {
+        a.cinv[d] = c->Nc[d] / c->p.Ls[d];
+        a.csz[d] = c->p.Ls[d] / c->Nc[d];
+    }
     a.rcut2 = c->p.rcut * c->p.rcut;  // params.jl:107
     a.alpha = (CT)c->p.alpha;
     a.pref = c->p.Gamma / (4 * M_PI);
