@@ -15,7 +15,7 @@
 //                     position, unit tangent and the offsets to the two neighbours along its
 //                     filament (read as a candidate strand point), and the node's own set-up
 //                     (local frame, filament, cell) read by the wavefront that builds its row
-//   feature_kernel    one wavefront per node: local frame (t, n, b) from s′, s″; the K
+//   feature_kernel    one wavefront per node: local frame (t, n, b) from its set-up record; the K
 //                     nearest strands within the feature radius, one point per strand (the
 //                     node where the distance along that strand has a local minimum, moved
 //                     to the foot of the perpendicular on its tangent line), with their
@@ -418,6 +418,8 @@ struct CheckArgs {
     int nquad;
     Box box;
     int3 Nc, Mw;
+    double cinv[3];  // Nc / L: cells per unit length
+    double csz[3];   // L / Nc: cell sizes
     double rc2, rf2, alpha_c, alpha_f, pref;
     double *out;  // [3][n]
 };
@@ -447,24 +449,44 @@ __global__ __launch_bounds__(CHK_T) void check_sr_kernel(CheckArgs a) {
     const double x[3] = {a.nodes[i], a.nodes[Np + i], a.nodes[2 * Np + i]};
     const int4 tc = a.tcell[i];
     const int WY = 2 * a.Mw.y + 1, WZ = 2 * a.Mw.z + 1, nrows = WY * WZ;
+    // rows cut to the cells that the rcut_f sphere reaches (conservative by 1e-9 cells; the
+    // r² test decides), as in the pair kernel
+    const double u[3] = {x[0] * a.cinv[0] - tc.x, x[1] * a.cinv[1] - tc.y, x[2] * a.cinv[2] - tc.z};
     for (int r = tid; r < nrows; r += CHK_T) {
-        int zz = tc.z + r / WY - a.Mw.z, yy = tc.y + r % WY - a.Mw.y;
+        const int dz = r / WY - a.Mw.z, dy = r % WY - a.Mw.y;
+        int zz = tc.z + dz, yy = tc.y + dy;
         zz += (zz < 0) ? a.Nc.z : 0;
         zz -= (zz >= a.Nc.z) ? a.Nc.z : 0;
         yy += (yy < 0) ? a.Nc.y : 0;
         yy -= (yy >= a.Nc.y) ? a.Nc.y : 0;
         const int64_t rb = ((int64_t)zz * a.Nc.y + yy) * a.Nc.x;
-        const int lo = tc.x - a.Mw.x, hi = tc.x + a.Mw.x;
-        int ca0, cb0, ca1 = 0, cb1 = -1;
-        if (lo < 0) {
+        const double gy = fmax((dy > 0 ? dy - u[1] : (dy < 0 ? u[1] - (dy + 1) : 0.0)) - 1e-9, 0.0) * a.csz[1];
+        const double gz = fmax((dz > 0 ? dz - u[2] : (dz < 0 ? u[2] - (dz + 1) : 0.0)) - 1e-9, 0.0) * a.csz[2];
+        const double rem = a.rf2 - (gy * gy + gz * gz);
+        int lo = 1, hi = 0;  // empty
+        if (rem >= 0.0) {
+            const double ext = sqrt(rem) * a.cinv[0] + 1e-9;
+            lo = tc.x + max((int)floor(u[0] - ext), -a.Mw.x);
+            hi = tc.x + min((int)floor(u[0] + ext), a.Mw.x);
+        }
+        int ca0 = 0, cb0 = -1, ca1 = 0, cb1 = -1;
+        if (lo > hi) {
+        } else if (hi < 0) {
+            ca0 = lo + a.Nc.x, cb0 = hi + a.Nc.x;
+        } else if (lo < 0) {
             ca0 = lo + a.Nc.x, cb0 = a.Nc.x - 1, cb1 = hi;
+        } else if (lo >= a.Nc.x) {
+            ca0 = lo - a.Nc.x, cb0 = hi - a.Nc.x;
         } else if (hi >= a.Nc.x) {
             ca0 = lo, cb0 = a.Nc.x - 1, cb1 = hi - a.Nc.x;
         } else {
             ca0 = lo, cb0 = hi;
         }
-        const int s0 = a.cstart[rb + ca0], n0 = a.cstart[rb + cb0 + 1] - s0;
-        int s1 = 0, n1 = 0;
+        int s0 = 0, n0 = 0, s1 = 0, n1 = 0;
+        if (cb0 >= ca0) {
+            s0 = a.cstart[rb + ca0];
+            n0 = a.cstart[rb + cb0 + 1] - s0;
+        }
         if (cb1 >= ca1) {
             s1 = a.cstart[rb + ca1];
             n1 = a.cstart[rb + cb1 + 1] - s1;
@@ -475,9 +497,17 @@ __global__ __launch_bounds__(CHK_T) void check_sr_kernel(CheckArgs a) {
         r_pre[r + 1] = n0 + n1;
     }
     __syncthreads();
-    if (tid == 0) {
-        r_pre[0] = 0;
-        for (int r = 0; r < nrows; r++) r_pre[r + 1] += r_pre[r];
+    // prefix over the rows: each wavefront scans 64 rows at a time, carrying the total
+    if (tid < 64) {
+        int carry = 0;
+        for (int r0 = 0; r0 < nrows; r0 += 64) {
+            const int r = r0 + tid;
+            const int v = r < nrows ? r_pre[r + 1] : 0;
+            const int incl = wave_incl_scan(v) + carry;
+            if (r < nrows) r_pre[r + 1] = incl;
+            carry = __builtin_amdgcn_readlane(incl, 63);
+        }
+        if (tid == 0) r_pre[0] = 0;
     }
     __syncthreads();
     const int ncand = r_pre[nrows];
@@ -703,6 +733,10 @@ int run_check_sr(vpbs_ctx *c, int64_t n, const int32_t *list, double *out, hipSt
     int mw[3];
     for (int d = 0; d < 3; d++) mw[d] = (int)std::ceil(c->lc_rcut_f * c->Nc[d] / c->p.Ls[d] - 1e-12);
     a.Mw = make_int3(mw[0], mw[1], mw[2]);
+    for (int d = 0; d < 3; d++) {
+        a.cinv[d] = c->Nc[d] / c->p.Ls[d];
+        a.csz[d] = c->p.Ls[d] / c->Nc[d];
+    }
     a.rc2 = c->p.rcut * c->p.rcut;
     a.rf2 = c->lc_rcut_f * c->lc_rcut_f;
     a.alpha_c = c->p.alpha;

@@ -176,8 +176,8 @@ __device__ inline void rinv_r(float r2, float &rinv, float &r) {
 
 // One wavefront per target node (cell-sorted order; XCD-aware block order):
 //  1) lanes build the source runs of the (2M+1)² neighbour cell rows (≤ 2 contiguous
-//     runs per row with the periodic wrap along x) and a wave scan gives each row's
-//     candidate offset;
+//     runs per row with the periodic wrap along x), each cut to the cells that the rcut
+//     sphere around the node can reach, and a wave scan gives each row's candidate offset;
 //  2) the candidates are tested 64 at a time, consecutive lanes reading consecutive
 //     sources: FP64 separation, minimum-image fold and r² ≤ rcut² exactly as in the
 //     reference (shortrange.jl:255-256), local segments excluded when required; the
@@ -186,14 +186,14 @@ __device__ inline void rinv_r(float r2, float &rinv, float &r) {
 //     (erfc/Gaussian weights, cross product; integrands.jl:2-18, gaussian.jl:273-279)
 //     into its own compensated accumulator, so the expensive part runs on full waves;
 //  4) the 64 partial sums are combined by a fixed butterfly in FP64.
-// A candidate's row is found by walking the row boundaries in order with a wave-uniform
-// row pointer (each boundary is passed once per node). The accepted pair set is the
-// thread-per-target one; the summation order is fixed (deterministic) but differs from a
-// sequential sum.
+// A candidate's row: every row meeting a batch marks its first candidate in it and a
+// max-scan over the lanes carries the marks forward. The cut drops only sources beyond
+// rcut, so the accepted pair set is the thread-per-target one, in the same order; the
+// summation order is fixed (deterministic) but differs from a sequential sum.
 constexpr int PAIR_MAXROWS = 81;  // (2M+1)² for M ≤ 4
 
 template <typename CT, bool VEL, bool PSI, bool STATS>
-__global__ __launch_bounds__(64) void pair_wave_kernel(PairArgs<CT> a) {
+__global__ __launch_bounds__(64, 4) void pair_wave_kernel(PairArgs<CT> a) {
     // per row: (candidates before it, run 0 start, run 0 length, run 1 start - run 0 length)
     __shared__ int4 r_run[PAIR_MAXROWS];
     __shared__ int q_s[128];
