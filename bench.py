@@ -237,7 +237,8 @@ def pair_kernel_report(vp, params, pd, stage_ms: float, reps: int = 5):
     """Short-range pair kernel against the VALU peak.
 
     One untimed evaluation with VPBS_NEIGHBOUR_STATS gives the candidate sources
-    tested (the (2M+1)³ neighbour cells of every node) and the accepted pairs;
+    tested (the cells of every node's (2M+1)³ neighbourhood that its rcut sphere
+    reaches) and the accepted pairs;
     flops = 11 per candidate (FP64 separation, fold, r²) + 40 per accepted pair
     (sqrt, reciprocal, erfc, exp, weights, cross product, accumulation). The
     kernel is timed alone (short range only, HIP events) and beside the long range

@@ -105,7 +105,8 @@ typedef struct vpbs_timings {
                             streamfunction, also its inverse transforms) */
     double merge;        /* SR + LR sum */
     int64_t pair_candidates; /* candidate sources the pair kernel tested (sum over nodes of the sources in
-                                the (2M+1)^3 neighbour cells); set by calls with VPBS_NEIGHBOUR_STATS, else 0 */
+                                the cells of the (2M+1)^3 neighbourhood that the rcut sphere around the
+                                node reaches); set by calls with VPBS_NEIGHBOUR_STATS, else 0 */
     int64_t grid[3];     /* oversampled NUFFT grid */
     int64_t cells[3];    /* cell-list cells per direction */
     double point_data;   /* kernels of the last vpbs_set_filaments (0 after vpbs_set_points*) */
