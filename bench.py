@@ -500,7 +500,8 @@ def main():
     if args.only:
         if args.only == "learned":
             r = time_learned(vp, pd, spl, args.steps, args.warmup, args.weights)
-            log(f"learned path: {r['ms_per_step']:.3f} ms/step" if r else "learned path: no weights")
+            log(f"learned path: {r['ms_per_step']:.3f} ms/step; {json.dumps(r['sampled_check'])}" if r
+                else "learned path: no weights")
         else:
             ms, st, _, _, _ = time_path(vp, p32 if args.only == "fp32" else p64, pd, args.steps, args.warmup)
             log(f"{args.only} path: {ms:.3f} ms/step; stages {json.dumps({k: round(v, 3) for k, v in st.items()})}")

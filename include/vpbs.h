@@ -344,7 +344,9 @@ int vpbs_mlp_train(vpbs_mlp *net, int64_t n, const float *X, const float *T, int
  * training helpers below. Requires filaments from vpbs_set_filaments.
  */
 typedef struct vpbs_learned_report {
-    double err_corrected;   /* sampled relative L2 error of the corrected velocity vs the converged splitting */
+    double err_corrected;   /* relative L2 error of the corrected velocity vs the converged splitting over
+                               the whole field, estimated from the sample: its squared errors times
+                               Np / nsample over the sum of |v|^2 over every node */
     double err_uncorrected; /* the same for the cheap velocity alone */
     int64_t nsample;        /* nodes in the sample (0: no check) */
     int32_t fallback;       /* 1: correction rejected, the converged splitting was evaluated on every node */

@@ -144,6 +144,27 @@ def test_errors_and_limits(vp, small):
     ctx.close()
 
 
+def test_sampled_error_estimates_whole_field(vp):
+    """The 1 % sampled error estimates the whole field's relative L2 error: 400 rings of radii
+    U[0.3, 1.0] (≈130k nodes; a few fast nodes hold much of Σ|v|²), a network predicting zero, six
+    successive samples against the error over every node; each estimate within a factor 1.6,
+    their mean within 25 %."""
+    st = vp.learned.LearnedSettings.create(TWO_PI, 64)
+    fs = vp.tangles.random_rings(400, 0.3, 1.0, TWO_PI / 512, seed=7)
+    whole = vp.learned.LearnedBiotSavart(st, _zero_net(vp), T=np.float64, sample_fraction=1.0, tol=1.0)
+    _, rep = whole.velocity_on_nodes(fs, fetch=False)
+    whole.close()
+    ref = rep["err_uncorrected"]
+    lb = vp.learned.LearnedBiotSavart(st, _zero_net(vp), T=np.float64, sample_fraction=0.01, tol=1.0, seed=5)
+    batch = vp.FilamentBatch(fs)
+    lb.ctx.set_filaments(batch, lb.params)
+    est = [lb.ctx.compute_learned(fetch=False)[1]["err_uncorrected"] for _ in range(6)]
+    lb.close()
+    print(f"  {batch.Np} nodes: whole field {ref:.3e}; 1 % samples {', '.join(f'{e:.3e}' for e in est)}")
+    assert all(ref / 1.6 < e < 1.6 * ref for e in est)
+    assert abs(np.mean(est) / ref - 1) < 0.25
+
+
 def test_training_pipeline_gain(vp):
     """Pairs from 8 tangles of 64 rings (≈135k nodes) on a 128³ grid, with a cheap splitting whose
     error is dominated by the short-range truncation (β = 3 for α, cut-off 2/α: ~1.6e-3), Adam

@@ -24,7 +24,9 @@
 //                     rcut_f (cells of the cheap cell list), FP64; the node's own two
 //                     segments contribute −LR_f + LR_c (their long-range parts)
 //   correction_apply  Δv_pred = (Γα_c/4π)·(y₀ t + y₁ n + y₂ b) from the network output y
-//   check_reduce      sampled errors of the corrected and the cheap velocity
+//   check_reduce      sampled errors of the corrected and the cheap velocity: the sample's
+//                     squared errors scaled to the whole field (×Np/ns) over Σ|v|² of every
+//                     node (vnorm_partial), an estimate of the whole field's relative L2 error
 //
 // The correction is judged on a stratified random sample of nodes every call: when the
 // relative L2 error of the corrected velocity against the converged one exceeds the
@@ -590,33 +592,58 @@ __global__ void correction_apply(int64_t Np, const float *__restrict__ y, const 
 }
 
 // Sums over the sample of |Δv_pred − Δv|², |Δv|² and |v_cheap + Δv|² (fixed-order tree).
+// Per-block partial sums of |v|² over every node (grid-stride, CHK_NV blocks), added in block
+// order by check_reduce.
+constexpr int CHK_NV = 1024;
+__global__ __launch_bounds__(256) void vnorm_partial(int64_t Np, const double *__restrict__ v,
+                                                     double *__restrict__ part) {
+    __shared__ double red[256];
+    double a = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < Np; i += (int64_t)gridDim.x * 256)
+        a += v[i] * v[i] + v[Np + i] * v[Np + i] + v[2 * Np + i] * v[2 * Np + i];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// out = (Σ_s |Δv_pred − Δv|², Σ_s |Δv|², (ns/Np)·Σ_all |v|²) over the sample s: the ratios of the
+// first two to the third estimate the squared relative L2 errors of the corrected and the cheap
+// velocity over the whole field. (Normalising by the sample's own Σ|v|² would not: a few fast
+// nodes hold much of the field's Σ|v|², and a 1 % sample that misses them overstates the
+// relative error two- to four-fold.)
 __global__ __launch_bounds__(1024) void check_reduce(int64_t Np, int64_t ns, const int32_t *__restrict__ sidx,
                                                     const double *__restrict__ dsr, const double *__restrict__ dlr,
-                                                    const double *__restrict__ dvp, const double *__restrict__ v,
-                                                    double *__restrict__ out) {
-    __shared__ double red[3][1024];
-    double a0 = 0, a1 = 0, a2 = 0;
+                                                    const double *__restrict__ dvp, const double *__restrict__ vpart,
+                                                    int nvp, double *__restrict__ out) {
+    __shared__ double red[2][1024];
+    double a0 = 0, a1 = 0;
     for (int64_t k = threadIdx.x; k < ns; k += 1024) {
         const int64_t i = sidx[k];
         for (int d = 0; d < 3; d++) {
             const double dt = dsr[d * ns + k] + dlr[d * ns + k];
             const double dp = dvp ? dvp[d * Np + i] : 0.0;
-            const double vc = v[d * Np + i] - dp;
             a0 += (dp - dt) * (dp - dt);
             a1 += dt * dt;
-            a2 += (vc + dt) * (vc + dt);
         }
     }
     red[0][threadIdx.x] = a0;
     red[1][threadIdx.x] = a1;
-    red[2][threadIdx.x] = a2;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
-            for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+            for (int q = 0; q < 2; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x < 3) out[threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 2) out[threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x == 2) {
+        double v2 = 0;
+        for (int b = 0; b < nvp; b++) v2 += vpart[b];
+        out[2] = v2 * ((double)ns / (double)Np);
+    }
 }
 
 // v ← v − Δv_pred + Δv (exact difference on every node).
@@ -663,7 +690,7 @@ int lc_buffers(vpbs_ctx *c) {
         VPBS_TRY(regrow(c, &c->lc_dlr, 3 * Np));
         c->lc_cap = Np;
     }
-    if (!c->lc_stats) VPBS_TRY(regrow(c, &c->lc_stats, 4));
+    if (!c->lc_stats) VPBS_TRY(regrow(c, &c->lc_stats, 4 + CHK_NV));
     if (!c->lc_ev) VPBS_HIP(c, hipEventCreateWithFlags(&c->lc_ev, hipEventDisableTiming));
     if (!c->lc_ev_cells) VPBS_HIP(c, hipEventCreateWithFlags(&c->lc_ev_cells, hipEventDisableTiming));
     return VPBS_OK;
@@ -945,7 +972,10 @@ int vpbs_compute_learned(vpbs_ctx *c, double *v_out, vpbs_learned_report *rep) {
     double sums[3] = {0, 0, 0};
     if (ns > 0) {
         hipStream_t s = c->s_sr;
-        check_reduce<<<1, 1024, 0, s>>>(Np, ns, c->lc_sidx, c->lc_dsr, c->lc_dlr, c->lc_dvp, c->v_out, c->lc_stats);
+        const int nvp = (int)std::min<int64_t>(CHK_NV, (int64_t)nblk(Np, 256));
+        vnorm_partial<<<nvp, 256, 0, s>>>(Np, c->v_out, c->lc_stats + 4);
+        check_reduce<<<1, 1024, 0, s>>>(Np, ns, c->lc_sidx, c->lc_dsr, c->lc_dlr, c->lc_dvp, c->lc_stats + 4, nvp,
+                                        c->lc_stats);
         VPBS_HIP(c, hipGetLastError());
         VPBS_HIP(c, hipMemcpyAsync(sums, c->lc_stats, sizeof sums, hipMemcpyDeviceToHost, s));
         VPBS_HIP(c, hipStreamSynchronize(s));

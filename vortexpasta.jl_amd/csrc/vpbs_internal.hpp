@@ -198,7 +198,7 @@ struct vpbs_ctx {
     double *lc_dlr = nullptr;         // converged − cheap long-range difference [3][n]
     int32_t *lc_sidx = nullptr;       // sample nodes
     double *lc_spos = nullptr;        // their positions [3][ns]
-    double *lc_stats = nullptr;       // check sums (3 doubles)
+    double *lc_stats = nullptr;       // check sums (3 doubles, 1 spare), then per-block Σ|v|² partials
     hipEvent_t lc_ev = nullptr;
     hipEvent_t lc_ev_cells = nullptr;  // short-range cell lists built (learned evaluation)
     int64_t lc_ns = 0;                // sample size of the current learned evaluation
