@@ -373,6 +373,7 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
         }
         nq += __popcll(mask);
     }
+    wave_lds_order();  // the row marks (ints in row's memory) are read before row is zeroed
     for (int e = lane; e < LC_NIN; e += 64) row[e] = 0.f;
     __syncthreads();
     nq = nq < LC_QMAX ? nq : LC_QMAX;
