@@ -593,9 +593,10 @@ __global__ void correction_apply(int64_t Np, const float *__restrict__ y, const 
 }
 
 // Sums over the sample of |Δv_pred − Δv|², |Δv|² and |v_cheap + Δv|² (fixed-order tree).
-// Per-block partial sums of |v|² over every node (grid-stride, CHK_NV blocks), added in block
-// order by check_reduce.
+// Per-block partial sums of |v|² over every node (grid-stride, at most CHK_NV blocks), added by
+// check_reduce's fixed reduction tree (one partial per thread).
 constexpr int CHK_NV = 1024;
+static_assert(CHK_NV <= 1024, "one partial per thread of check_reduce");
 __global__ __launch_bounds__(256) void vnorm_partial(int64_t Np, const double *__restrict__ v,
                                                      double *__restrict__ part) {
     __shared__ double red[256];
@@ -620,7 +621,7 @@ __global__ __launch_bounds__(1024) void check_reduce(int64_t Np, int64_t ns, con
                                                     const double *__restrict__ dsr, const double *__restrict__ dlr,
                                                     const double *__restrict__ dvp, const double *__restrict__ vpart,
                                                     int nvp, double *__restrict__ out) {
-    __shared__ double red[2][1024];
+    __shared__ double red[3][1024];
     double a0 = 0, a1 = 0;
     for (int64_t k = threadIdx.x; k < ns; k += 1024) {
         const int64_t i = sidx[k];
@@ -633,18 +634,15 @@ __global__ __launch_bounds__(1024) void check_reduce(int64_t Np, int64_t ns, con
     }
     red[0][threadIdx.x] = a0;
     red[1][threadIdx.x] = a1;
+    red[2][threadIdx.x] = (int)threadIdx.x < nvp ? vpart[threadIdx.x] : 0.0;  // nvp ≤ 1024
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
-            for (int q = 0; q < 2; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+            for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x < 2) out[threadIdx.x] = red[threadIdx.x][0];
-    if (threadIdx.x == 2) {
-        double v2 = 0;
-        for (int b = 0; b < nvp; b++) v2 += vpart[b];
-        out[2] = v2 * ((double)ns / (double)Np);
-    }
+    if (threadIdx.x == 2) out[2] = red[2][0] * ((double)ns / (double)Np);
 }
 
 // v ← v − Δv_pred + Δv (exact difference on every node).
