@@ -223,6 +223,13 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
     // test below decides).
     const int4 tc = ns->cell;
     const int M = a.Mf, W2 = 2 * M + 1, nrows = W2 * W2;
+    // The minimum-image fold changes no separation to a candidate when the node's neighbour
+    // cells do not wrap and every direction has at least 2M + 2 cells: then |x − s| < (M + 1)
+    // cells ≤ L/2 (up to rounding at exactly L/2, beyond the radius either way). Such nodes
+    // (most of them) skip the fold; the own-stretch test above keeps it (neighbours along a
+    // filament may sit across the periodic boundary).
+    const bool wraps = tc.x < M || tc.y < M || tc.z < M || tc.x + M >= a.Nc.x || tc.y + M >= a.Nc.y ||
+                       tc.z + M >= a.Nc.z || min(min(a.Nc.x, a.Nc.y), a.Nc.z) < 2 * M + 2;
     // the node's place in its cell (in cells; within a few ulps of the cell index's own
     // (x / L)·Nc, far inside the 1e-9 widening)
     const double u[3] = {x[0] * a.cinv[0] - tc.x, x[1] * a.cinv[1] - tc.y, x[2] * a.cinv[2] - tc.z};
@@ -330,9 +337,12 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
         float key = 0.f, fv[6] = {0, 0, 0, 0, 0, 0};
         const int32_t j = __float_as_int(ra.w);
         {
-            const double r0 = fold_sep(px0 - x[0], a.box.L[0], a.box.Lh[0]);
-            const double r1 = fold_sep(px1 - x[1], a.box.L[1], a.box.Lh[1]);
-            const double r2v = fold_sep(px2 - x[2], a.box.L[2], a.box.Lh[2]);
+            double r0 = px0 - x[0], r1 = px1 - x[1], r2v = px2 - x[2];
+            if (wraps) {  // wave-uniform
+                r0 = fold_sep(r0, a.box.L[0], a.box.Lh[0]);
+                r1 = fold_sep(r1, a.box.L[1], a.box.Lh[1]);
+                r2v = fold_sep(r2v, a.box.L[2], a.box.Lh[2]);
+            }
             const double d2 = dist2_nocontract(r0, r1, r2v);
             bool ok = cnd < ncand && j != i && d2 <= a.rf2;
             if (ok && (unsigned)(j - oi) < (unsigned)Ni) {  // i's own filament
