@@ -68,15 +68,19 @@ def test_difference_vs_oracle(vp, O, small):
     assert 1e-5 < rel < 1e-1
 
 
-@pytest.mark.parametrize("nrings,rmin,spacing,factor", [(60, 0.5, TWO_PI / 64, 2.0), (150, 0.3, TWO_PI / 96, 1.5)],
-                         ids=["60rings-2rcut", "150rings-1.5rcut"])
-def test_features_vs_restatement(vp, LO, nrings, rmin, spacing, factor):
+@pytest.mark.parametrize("nrings,rmin,rmax,spacing,factor,box",
+                         [(60, 0.5, 1.0, TWO_PI / 64, 2.0, TWO_PI), (150, 0.3, 1.0, TWO_PI / 96, 1.5, TWO_PI),
+                          (300, 0.05, 0.12, TWO_PI / 256, 2.0, 0.6)],
+                         ids=["60rings-2rcut", "150rings-1.5rcut", "300small-rings-packed"])
+def test_features_vs_restatement(vp, LO, nrings, rmin, rmax, spacing, factor, box):
     """60 rings (≈2.9k nodes, feature radius 2 rcut: 9 × 9 cell rows) and 150 denser rings
     (≈9k nodes, 1.5 rcut: 7 × 7 rows): ~1 strand per node on average within the radius, up
-    to 6. The restatement finds neighbours by brute force, so the kernel's cell walk and its
-    cut of the rows to the sphere are checked as well."""
+    to 6; and 300 small rings with centres in a 0.6-wide cube: over a hundred strands within
+    the radius of most nodes, so the kernel's queue of strand points is cut back to the 16
+    nearest several times per node. The restatement finds neighbours by brute force, so the
+    kernel's cell walk and its cut of the rows to the sphere are checked as well."""
     st = vp.learned.LearnedSettings.create(TWO_PI, 64, feature_factor=factor)
-    fs = vp.tangles.random_rings(nrings, rmin, 1.0, spacing, seed=11)
+    fs = vp.tangles.random_rings(nrings, rmin, rmax, spacing, L=box, seed=11)
     p_c = st.params(np.float64)
     batch = vp.FilamentBatch(fs)
     L = vp._lib
@@ -92,8 +96,22 @@ def test_features_vs_restatement(vp, LO, nrings, rmin, spacing, factor):
     Xw, Fw = LO.features(pts["nodes"], pts["sp"], pts["spp"], batch.offs, batch.Ns, st.Ls, st.cheap.α,
                          st.feature_radius)
     nslots = (np.abs(Xw[:, 6::6][:, :16]) + np.abs(Xw[:, 9::6][:, :16]) > 0).sum(axis=1)
-    print(f"  strands per node: mean {nslots.mean():.2f}, max {nslots.max()}")
+    # other filaments with a node within the radius (a lower bound on the strand points queued)
+    P = np.asarray(pts["nodes"])
+    fid = np.repeat(np.arange(len(batch.Ns)), batch.Ns)
+    near = np.zeros(batch.Np, dtype=np.int64)
+    for c0 in range(0, batch.Np, 2048):
+        d = P[:, c0:c0 + 2048, None] - P[:, None, :]
+        d -= np.round(d / TWO_PI) * TWO_PI
+        inside = (d * d).sum(axis=0) <= st.feature_radius ** 2
+        for r in range(inside.shape[0]):
+            f = np.unique(fid[inside[r]])
+            near[c0 + r] = len(f) - (fid[c0 + r] in f)
+    print(f"  strands per node: mean {nslots.mean():.2f}, max {nslots.max()}; other filaments within the "
+          f"radius: mean {near.mean():.1f}, max {near.max()}")
     assert nslots.mean() > 0.5 and nslots.max() >= 4
+    if box < 1:
+        assert np.median(near) > 64  # the queue is cut back
     assert np.abs(Fr - Fw).max() < 1e-6
     assert np.abs(X - Xw).max() < 2e-5, np.abs(X - Xw).max()
 

@@ -44,7 +44,7 @@ namespace {
 
 constexpr int LC_K = 16;        // strands per node
 constexpr int LC_NIN = 128;     // feature row (6 + 6·K used)
-constexpr int LC_QMAX = 128;    // strand candidates kept per node
+constexpr int LC_QMAX = 128;    // queued strand points per node (reduced to the K nearest when it could overflow)
 constexpr int LC_MAXROWS = 81;  // (2·4 + 1)² cell rows of the feature walk
 constexpr double TWO_OVER_SQRT_PI = 1.1283791670955126;
 
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
         const unsigned long long mask = __ballot(acc);
         if (acc) {
             const int pos = nq + __popcll(mask & ((1ull << lane) - 1ull));
-            if (pos < LC_QMAX) {
+            {  // pos < LC_QMAX: the queue holds at most LC_QMAX − 64 entries before a batch
                 q_key[pos] = key;
                 q_j[pos] = j;
 #pragma unroll
@@ -382,6 +382,43 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
             }
         }
         nq += __popcll(mask);
+        if (nq > LC_QMAX - 64) {
+            // the next batch could overflow the queue: keep its K nearest entries (by
+            // perpendicular distance, ties by node index), in that order. An entry with K
+            // others nearer can never be among the K nearest of the node, so the kept set
+            // is exact whatever the density.
+            wave_lds_order();
+            int rk[2] = {LC_K, LC_K};
+            float ck[2] = {0.f, 0.f}, cf[2][6];
+            int cj[2] = {0, 0};
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int e = lane + 64 * h;
+                if (e < nq) {
+                    ck[h] = q_key[e];
+                    cj[h] = q_j[e];
+#pragma unroll
+                    for (int f = 0; f < 6; f++) cf[h][f] = q_f[f][e];
+                    int r = 0;
+                    for (int o = 0; o < nq; o++) {
+                        const float ko = q_key[o];
+                        r += (ko < ck[h]) || (ko == ck[h] && q_j[o] < cj[h]);
+                    }
+                    rk[h] = r;
+                }
+            }
+            wave_lds_order();
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+                if (rk[h] < LC_K) {
+                    q_key[rk[h]] = ck[h];
+                    q_j[rk[h]] = cj[h];
+#pragma unroll
+                    for (int f = 0; f < 6; f++) q_f[f][rk[h]] = cf[h][f];
+                }
+            nq = LC_K;
+            wave_lds_order();
+        }
     }
     wave_lds_order();  // the row marks (ints in row's memory) are read before row is zeroed
     for (int e = lane; e < LC_NIN; e += 64) row[e] = 0.f;
