@@ -64,10 +64,10 @@ def build_workload(vp, cfg: int, sigma: float = 1.5, m: int = 4, fft: str = "fus
     return fs, spl, p32, p64, pd
 
 
-def time_path(vp, params, pd, steps: int, warmup: int):
+def time_path(vp, params, pd, steps: int, warmup: int, extra_flags: int = 0):
     ctx = vp._lib.Context(params)
     ctx.set_points(pd)  # H2D, outside the timed region
-    flags = vp._lib.VELOCITY
+    flags = vp._lib.VELOCITY | extra_flags
     for _ in range(warmup):
         ctx.compute(flags, fetch=False)
     acc = {}
@@ -543,6 +543,13 @@ def main():
              "fourier": f"spec_z_fourier<float, {grid[2]}>" if fused or args.fft == "separate" else "fourier_kernel<float>",
              "interp": f"x_inverse_interp<float, {2 * args.m}, {grid[0]}>" if slab
              else f"interp_kernel_wave<float, {2 * args.m}>"}[dom]
+    # the same stages with each range alone (the other stream idle): what the kernels reach by
+    # themselves, against the figures above where the two streams share the GPU
+    ms_lr, st_lr, _, _, _ = time_path(vp, p32, pd, args.steps, args.warmup, vp._lib.NO_SHORTRANGE)
+    ms_sr, st_sr, _, _, _ = time_path(vp, p32, pd, args.steps, args.warmup, vp._lib.NO_LONGRANGE)
+    alone = {"long_range_ms": round(ms_lr, 4), "short_range_ms": round(ms_sr, 4),
+             "stages_ms": {k: round(v, 4) for k, v in {**st_sr, **st_lr}.items() if v > 0}}
+    log(f"alone: {json.dumps(alone)}")
     pair = pair_kernel_report(vp, p32, pd, st32.get("pair", 0.0))
     pair["kernel"] = "pair_wave_kernel<float, true, false, false>"
     log(f"pair kernel: {json.dumps(pair)}")
@@ -584,8 +591,12 @@ def main():
             if traffic is not None else None,
             "algorithmic_bytes": nbytes,
             "avg_ms": round(st32[dom], 4),
+            "alone": {"avg_ms": round(st_lr[dom], 4),
+                      "achieved": round(nbytes / (st_lr[dom] * 1e-3) / 1e9, 1),
+                      "frac": round(nbytes / (st_lr[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         },
         "stages_ms": {k: round(v, 4) for k, v in st32.items()},
+        "ranges_alone": alone,
         "stage_algorithmic_GBps": per_stage,
         "pair_kernel": pair,
         "point_data": pdt,
