@@ -127,7 +127,8 @@ struct PairArgs {
     int3 Nc;
     int M;
     double cinv[3];  // Nc / L: cells per unit length
-    double csz[3];   // L / Nc: cell sizes
+    float cinvf[3], cszf[3];  // the same in FP32, and L / Nc (the rows' cut to the rcut sphere)
+    float rcut2f;
     double rcut2;
     CT alpha;
     double pref;
@@ -225,10 +226,12 @@ __global__ __launch_bounds__(64, 4) void pair_wave_kernel(PairArgs<CT> a) {
 
     // 1) lanes take rows lane and lane + 64. Each row keeps only the cells that can hold a
     //    source within rcut: rows whose (y, z) gap to the node exceeds rcut are empty, the
-    //    others are cut to the x-cells within √(rcut² − gap²) of the node (widened by 1e-9
+    //    others are cut to the x-cells within √(rcut² − gap²) of the node (widened by 1e-4
     //    cells: a conservative cut that drops only sources the r² test rejects, so the
     //    accepted pairs and their order are unchanged)
-    const double u[3] = {x * a.cinv[0] - cx, y * a.cinv[1] - cy, z * a.cinv[2] - cz};  // place in the cell
+    // place in the cell (FP64, then FP32: the cut below is widened by 1e-4 cells, far above
+    // the FP32 rounding of these O(M)-cell quantities)
+    const float u[3] = {(float)(x * a.cinv[0] - cx), (float)(y * a.cinv[1] - cy), (float)(z * a.cinv[2] - cz)};
     int len[2] = {0, 0};
     int4 run[2];
 #pragma unroll
@@ -243,14 +246,14 @@ __global__ __launch_bounds__(64, 4) void pair_wave_kernel(PairArgs<CT> a) {
             yy += (yy < 0) ? a.Nc.y : 0;
             yy -= (yy >= a.Nc.y) ? a.Nc.y : 0;
             const int64_t row = ((int64_t)zz * a.Nc.y + yy) * a.Nc.x;
-            const double gy = fmax((dy > 0 ? dy - u[1] : (dy < 0 ? u[1] - (dy + 1) : 0.0)) - 1e-9, 0.0) * a.csz[1];
-            const double gz = fmax((dz > 0 ? dz - u[2] : (dz < 0 ? u[2] - (dz + 1) : 0.0)) - 1e-9, 0.0) * a.csz[2];
-            const double rem = a.rcut2 - (gy * gy + gz * gz);
+            const float gy = fmaxf((dy > 0 ? dy - u[1] : (dy < 0 ? u[1] - (dy + 1) : 0.f)) - 1e-4f, 0.f) * a.cszf[1];
+            const float gz = fmaxf((dz > 0 ? dz - u[2] : (dz < 0 ? u[2] - (dz + 1) : 0.f)) - 1e-4f, 0.f) * a.cszf[2];
+            const float rem = a.rcut2f - (gy * gy + gz * gz);
             int lo = 1, hi = 0;  // empty
-            if (rem >= 0.0) {
-                const double ext = sqrt(rem) * a.cinv[0] + 1e-9;
-                lo = cx + max((int)floor(u[0] - ext), -M);
-                hi = cx + min((int)floor(u[0] + ext), M);
+            if (rem >= 0.f) {
+                const float ext = sqrtf(rem) * a.cinvf[0] + 1e-4f;
+                lo = cx + max((int)floorf(u[0] - ext), -M);
+                hi = cx + min((int)floorf(u[0] + ext), M);
             }
             int ca0 = 0, cb0 = -1, ca1 = 0, cb1 = -1;
             if (lo > hi) {
@@ -416,26 +419,22 @@ __global__ __launch_bounds__(64, 4) void pair_wave_kernel(PairArgs<CT> a) {
     }
     if (lane < nqd) eval(lane);
 
-    // 4) fixed-order reduction of the lane partial sums
+    // 4) fixed-order reduction of the lane partial sums (DPP steps in FP64; total in lane 63)
     if (VEL) {
-        double v3[3] = {av.get(0), av.get(1), av.get(2)};
+        double v3[3];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-            for (int d = 0; d < 3; d++) v3[d] += __shfl_xor(v3[d], off, 64);
-        if (lane == 0) {
+        for (int d = 0; d < 3; d++) v3[d] = wave_sum_f64_last(av.get(d));
+        if (lane == 63) {
             a.v[i] += a.pref * v3[0];
             a.v[Np + i] += a.pref * v3[1];
             a.v[2 * Np + i] += a.pref * v3[2];
         }
     }
     if (PSI) {
-        double p3[3] = {ap.get(0), ap.get(1), ap.get(2)};
+        double p3[3];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-            for (int d = 0; d < 3; d++) p3[d] += __shfl_xor(p3[d], off, 64);
-        if (lane == 0) {
+        for (int d = 0; d < 3; d++) p3[d] = wave_sum_f64_last(ap.get(d));
+        if (lane == 63) {
             a.psi[i] += a.pref * p3[0];
             a.psi[Np + i] += a.pref * p3[1];
             a.psi[2 * Np + i] += a.pref * p3[2];
@@ -847,9 +846,11 @@ int run_pairs(vpbs_ctx *c, bool vel, bool psi, bool stats, bool include_local) {
     a.M = c->p.nsubdiv;
     for (int d = 0; d < 3; d++) {
         a.cinv[d] = c->Nc[d] / c->p.Ls[d];
-        a.csz[d] = c->p.Ls[d] / c->Nc[d];
+        a.cinvf[d] = (float)a.cinv[d];
+        a.cszf[d] = (float)(c->p.Ls[d] / c->Nc[d]);
     }
     a.rcut2 = c->p.rcut * c->p.rcut;  // params.jl:107
+    a.rcut2f = (float)a.rcut2;
     a.alpha = (CT)c->p.alpha;
     a.pref = c->p.Gamma / (4 * M_PI);
     a.include_local = include_local ? 1 : 0;
