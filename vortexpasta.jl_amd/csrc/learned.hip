@@ -162,7 +162,8 @@ struct FeatArgs {
     int Mf;
     int w2inv;    // ⌈65536 / (2·Mf + 1)⌉: row r of the walk is in z-layer (r · w2inv) >> 16
     double cinv[3];  // Nc / L: cells per unit length
-    double csz[3];   // L / Nc: cell sizes
+    float cinvf[3], cszf[3];  // the same in FP32, and L / Nc (the rows' cut to the feature sphere)
+    float rf2f;
     double rf2;   // feature radius²
     double alpha; // α_c (lengths in units of 1/α_c)
     float *feat;  // [Np][LC_NIN]
@@ -219,8 +220,8 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
     // periodic wrap along x); a wave scan gives each row's first candidate. Each row keeps
     // only the cells that can hold a point within the radius: rows whose (y, z) gap to the
     // node exceeds it are empty, the others are cut to the x-cells within
-    // √(r_f² − gap²) of the node (widened by 1e-9 cells: a conservative cut; the distance
-    // test below decides).
+    // √(r_f² − gap²) of the node (in FP32, widened by 1e-4 cells: a conservative cut; the
+    // distance test below decides).
     const int4 tc = ns->cell;
     const int M = a.Mf, W2 = 2 * M + 1, nrows = W2 * W2;
     // The minimum-image fold changes no separation to a candidate when the node's neighbour
@@ -231,8 +232,9 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
     const bool wraps = tc.x < M || tc.y < M || tc.z < M || tc.x + M >= a.Nc.x || tc.y + M >= a.Nc.y ||
                        tc.z + M >= a.Nc.z || min(min(a.Nc.x, a.Nc.y), a.Nc.z) < 2 * M + 2;
     // the node's place in its cell (in cells; within a few ulps of the cell index's own
-    // (x / L)·Nc, far inside the 1e-9 widening)
-    const double u[3] = {x[0] * a.cinv[0] - tc.x, x[1] * a.cinv[1] - tc.y, x[2] * a.cinv[2] - tc.z};
+    // (x / L)·Nc, and within FP32 rounding after the cast: far inside the 1e-4 widening)
+    const float u[3] = {(float)(x[0] * a.cinv[0] - tc.x), (float)(x[1] * a.cinv[1] - tc.y),
+                        (float)(x[2] * a.cinv[2] - tc.z)};
     int len[2] = {0, 0};
     int4 run[2];
 #pragma unroll
@@ -246,14 +248,14 @@ __global__ __launch_bounds__(64) void feature_kernel(FeatArgs a) {
             yy += (yy < 0) ? a.Nc.y : 0;
             yy -= (yy >= a.Nc.y) ? a.Nc.y : 0;
             const int64_t rb = ((int64_t)zz * a.Nc.y + yy) * a.Nc.x;
-            const double gy = fmax((dy > 0 ? dy - u[1] : (dy < 0 ? u[1] - (dy + 1) : 0.0)) - 1e-9, 0.0) * a.csz[1];
-            const double gz = fmax((dz > 0 ? dz - u[2] : (dz < 0 ? u[2] - (dz + 1) : 0.0)) - 1e-9, 0.0) * a.csz[2];
-            const double rem = a.rf2 - (gy * gy + gz * gz);
+            const float gy = fmaxf((dy > 0 ? dy - u[1] : (dy < 0 ? u[1] - (dy + 1) : 0.f)) - 1e-4f, 0.f) * a.cszf[1];
+            const float gz = fmaxf((dz > 0 ? dz - u[2] : (dz < 0 ? u[2] - (dz + 1) : 0.f)) - 1e-4f, 0.f) * a.cszf[2];
+            const float rem = a.rf2f - (gy * gy + gz * gz);
             int lo = 1, hi = 0;  // empty
-            if (rem >= 0.0) {
-                const double ext = sqrt(rem) * a.cinv[0] + 1e-9;
-                lo = tc.x + max((int)floor(u[0] - ext), -M);
-                hi = tc.x + min((int)floor(u[0] + ext), M);
+            if (rem >= 0.f) {
+                const float ext = sqrtf(rem) * a.cinvf[0] + 1e-4f;
+                lo = tc.x + max((int)floorf(u[0] - ext), -M);
+                hi = tc.x + min((int)floorf(u[0] + ext), M);
             }
             int ca0 = 0, cb0 = -1, ca1 = 0, cb1 = -1;
             if (lo > hi) {
@@ -774,9 +776,11 @@ int run_features(vpbs_ctx *c, hipStream_t s) {
     a.w2inv = (65536 + 2 * Mf) / (2 * Mf + 1);
     for (int d = 0; d < 3; d++) {
         a.cinv[d] = c->Nc[d] / c->p.Ls[d];
-        a.csz[d] = c->p.Ls[d] / c->Nc[d];
+        a.cinvf[d] = (float)a.cinv[d];
+        a.cszf[d] = (float)(c->p.Ls[d] / c->Nc[d]);
     }
     a.rf2 = c->lc_rfeat * c->lc_rfeat;
+    a.rf2f = (float)a.rf2;
     a.alpha = c->lc_alpha_c;
     a.feat = c->lc_feat;
     feature_kernel<<<(unsigned)(8 * ((Np + 7) / 8)), 64, 0, s>>>(a);
