@@ -1062,6 +1062,31 @@ constexpr int slab_rec(int W) {
     return slab_rec_len(W, (int)sizeof(CT));
 }
 
+// Lane-to-lane move by a DPP control (every lane has a source), FP32 and FP64.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over each group of 8 consecutive lanes, in every lane of the group: partners at xor 1
+// and xor 2 (quad permutations), then the mirror lane of the half-row, which after those
+// steps holds the same value as the xor-4 partner: the additions of an xor butterfly over
+// offsets 1, 2, 4, so the same bits, without going through LDS.
+template <typename CT>
+__device__ __forceinline__ CT sum8_lanes(CT v) {
+    v += dpp_move<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_move<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_move<0x141>(v);  // row_half_mirror
+    return v;
+}
+
 // i in (-n, 2n) -> i mod n
 __device__ __forceinline__ int wrap1(int i, int n) {
     i += (i < 0) ? n : 0;
@@ -1220,12 +1245,9 @@ __global__ __launch_bounds__(SLAB_NT) void x_inverse_interp(SlabArgs a, const C2
                 v0 *= wyz, v1 *= wyz, v2 *= wyz;
             }
         }
-#pragma unroll
-        for (int o = 1; o < SLAB_ROWS; o <<= 1) {
-            v0 += __shfl_xor(v0, o, 64);
-            v1 += __shfl_xor(v1, o, 64);
-            v2 += __shfl_xor(v2, o, 64);
-        }
+        v0 = sum8_lanes(v0);
+        v1 = sum8_lanes(v1);
+        v2 = sum8_lanes(v2);
         if (cnd < ncand && r == 0) {
             CT *dst = part + (int64_t)q * 3 * Np + sidx;
             dst[0] = v0;
