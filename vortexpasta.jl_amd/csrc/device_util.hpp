@@ -57,6 +57,19 @@ __device__ inline double wave_sum_f64_last(double v) {
     return v;
 }
 
+// The same fixed-order sum for FP32 values (one DPP move per step).
+__device__ inline float wave_sum_f32_last(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xf, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xf, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xf, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xa, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xc, 0xf, false));
+    return v;
+}
+__device__ inline double wave_sum_last(double v) { return wave_sum_f64_last(v); }
+__device__ inline float wave_sum_last(float v) { return wave_sum_f32_last(v); }
+
 // Orders this wavefront's LDS accesses for the compiler (no wait: the LDS executes one
 // wavefront's instructions in issue order). For one-wavefront blocks exchanging values
 // between lanes through LDS.

@@ -34,6 +34,7 @@
 // Everything is templated on the grid type (double for the FP64 path, float
 // for the FP32-bulk path).
 #include "vpbs_internal.hpp"
+#include "device_util.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -370,15 +371,10 @@ __global__ __launch_bounds__(64) void spread_rows_kernel(LRGeom g, int64_t Nq, c
             nb_start[lane] = s0;
             len = dup ? 0 : tstart[tl + 1] - s0;
         }
-        int incl = len;
-#pragma unroll
-        for (int off = 1; off < 32; off <<= 1) {
-            const int y = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += y;
-        }
+        const int incl = wave_incl_scan(len);  // DPP steps, no LDS round trips
         if (lane < 27) nb_pre[lane + 1] = incl;
         if (lane == 0) nb_pre[0] = 0;
-        const int ncand = __shfl(incl, 26, 64);
+        const int ncand = __builtin_amdgcn_readlane(incl, 26);
         __syncthreads();
 
         for (int base = 0; base < ncand; base += 64) {
@@ -829,7 +825,7 @@ __global__ void spectrum_reduce(const double *__restrict__ partial, int nblocks,
 // LDS (exact copies); lane (a, b) = (lane % W, lane / W) then owns stencil column
 // x = i0x + a and rows y = i0y + b + R·pass (R = ⌊64/W⌋ rows per pass) and walks the
 // W z-planes, so a load instruction touches R rows of W consecutive values; the
-// partial sums are combined by a butterfly reduction (pairwise summation).
+// partial sums are combined by DPP steps in a fixed order (total in lane 63).
 template <typename CT, int W>
 __global__ __launch_bounds__(64) void interp_kernel_wave(LRGeom g, int64_t Np, const double *__restrict__ nodes,
                                                          CT beta, const CT *__restrict__ grid,
@@ -887,13 +883,10 @@ __global__ __launch_bounds__(64) void interp_kernel_wave(LRGeom g, int64_t Np, c
             }
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        acc0 += __shfl_xor(acc0, o, 64);
-        acc1 += __shfl_xor(acc1, o, 64);
-        acc2 += __shfl_xor(acc2, o, 64);
-    }
-    if (lane == 0) {
+    acc0 = wave_sum_last(acc0);
+    acc1 = wave_sum_last(acc1);
+    acc2 = wave_sum_last(acc2);
+    if (lane == 63) {
         out[i] = (double)acc0;
         out[Np + i] = (double)acc1;
         out[2 * Np + i] = (double)acc2;

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "vpbs_internal.hpp"
+#include "device_util.hpp"
 
 namespace vpbs {
 namespace {
@@ -1130,13 +1131,8 @@ __global__ __launch_bounds__(SLAB_NT) void x_inverse_interp(SlabArgs a, const C2
         s0 = a.start[key];
         len = a.start[key + 1] - s0;
     }
-    int incl = len;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int v = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += v;
-    }
-    const int ncand = __shfl(incl, NRG - 1, 64);  // the same in every wavefront
+    const int incl = wave_incl_scan(len);
+    const int ncand = __builtin_amdgcn_readlane(incl, NRG - 1);  // the same in every wavefront
     if (ncand == 0) return;
     // candidate cnd of range q is node off[q] + cnd (cnd ≥ pre[q]); wavefront-uniform
     const int excl = incl - len;
