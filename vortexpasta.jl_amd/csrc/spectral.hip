@@ -568,6 +568,9 @@ __global__ __launch_bounds__(NTX) void spec_x_forward(SpecGeom g, const C2<CT> *
         A = C2<CT>{(zk.re + zm.re) * (CT)0.5, (zk.im - zm.im) * (CT)0.5};
         Bv = C2<CT>{(zk.im + zm.im) * (CT)0.5, (zm.re - zk.re) * (CT)0.5};
     };
+    // consecutive lanes take consecutive k of one row: each store instruction writes one
+    // contiguous run of x1 (column-fastest lanes halve the LDS bank conflicts of the reads
+    // below but split the stores into four runs, and this kernel is bound by HBM)
     if (use_pairs<CT>() && !(nkx & 1)) {
         const int h = nkx >> 1;  // pairs (k, k+1) per row
         for (int e = threadIdx.x; e < (h << lgB); e += NTX) {
@@ -1156,10 +1159,11 @@ __global__ __launch_bounds__(SLAB_NT) void x_inverse_interp(SlabArgs a, const C2
         Sm[k * SLAB_BW + b] = C2<CT>{A.re - Bv.im, A.im + Bv.re};
         if (k > 0) Sm[(LS - k) * SLAB_BW + b] = C2<CT>{A.re + Bv.im, Bv.re - A.im};
     };
+    // consecutive lanes take consecutive columns b (LDS writes along a slab row)
     if (use_pairs<CT>() && !(nkx & 1)) {
         const int h = nkx >> 1;  // pairs (k, k+1) per row: 16-byte loads
         for (int e = threadIdx.x; e < h * SLAB_BW; e += SLAB_NT) {
-            const int b = e / h, k = (e - b * h) * 2;
+            const int kp = e / SLAB_BW, b = e - kp * SLAB_BW, k = kp * 2;
             const C2<CT> *src = x1 + row_of(b) * nkx + k;
             const C2x2<CT> A = load_pair(src), Bv = load_pair(src + nkx);
             put(k, b, A.a, Bv.a);
@@ -1167,7 +1171,7 @@ __global__ __launch_bounds__(SLAB_NT) void x_inverse_interp(SlabArgs a, const C2
         }
     } else {
         for (int e = threadIdx.x; e < nkx * SLAB_BW; e += SLAB_NT) {
-            const int b = e / nkx, k = e - b * nkx;
+            const int k = e / SLAB_BW, b = e - k * SLAB_BW;
             const C2<CT> *src = x1 + row_of(b) * nkx + k;
             put(k, b, src[0], src[nkx]);
         }
