@@ -1139,12 +1139,10 @@ __global__ __launch_bounds__(SLAB_NT) void x_inverse_interp(SlabArgs a, const C2
     if (ncand == 0) return;
     // candidate cnd of range q is node off[q] + cnd (cnd ≥ pre[q]); wavefront-uniform
     const int excl = incl - len;
-    int pre[NRG], off[NRG];
+    const int offv = s0 - excl;  // lane q: node of candidate 0 of range q
+    int pre[NRG];
 #pragma unroll
-    for (int q = 0; q < NRG; q++) {
-        pre[q] = __builtin_amdgcn_readlane(excl, q);
-        off[q] = __builtin_amdgcn_readlane(s0 - excl, q);
-    }
+    for (int q = 0; q < NRG; q++) pre[q] = __builtin_amdgcn_readlane(excl, q);
 
     // columns: Z_k = A_k + i B_k (k < nkx; DC real) and conj(A_k) + i conj(B_k) at L - k
     C2<CT> *Sm = lds<CT>(0);
@@ -1184,6 +1182,11 @@ __global__ __launch_bounds__(SLAB_NT) void x_inverse_interp(SlabArgs a, const C2
     }
     __syncthreads();
     fft_static<CT, 1, SLAB_NT, LS, SLAB_BW>(0, LS * SLAB_BW);
+    // x rows 0..W-2 again after row LS-1 (over the twiddles, no longer needed), so that a
+    // node's W x-cells are consecutive rows also across the periodic end
+    static_assert((W - 1) * SLAB_BW <= LS, "the x halo fits in the twiddle area");
+    for (int e = threadIdx.x; e < (W - 1) * SLAB_BW; e += SLAB_NT) Sm[LS * SLAB_BW + e] = Sm[e];
+    __syncthreads();
 
     // stencil sums: lane (j, r) = node j of the group, slab row r
     const CT *Sf = reinterpret_cast<const CT *>(spec_smem);
@@ -1197,49 +1200,44 @@ __global__ __launch_bounds__(SLAB_NT) void x_inverse_interp(SlabArgs a, const C2
     CT *part = static_cast<CT *>(a.part);
     for (int g = wave; g * 8 < ncand; g += SLAB_NT / 64) {
         const int cnd = g * 8 + j;
-        int q = 0, sidx = 0;
+        // range of the candidate: the number of later ranges starting at or before it (the
+        // starts never decrease); its offset from lane q, with every lane active
+        int q = 0;
+#pragma unroll
+        for (int t = 1; t < NRG; t++) q += cnd >= pre[t] ? 1 : 0;
+        const int o = __shfl(offv, q, 64);
+        int sidx = 0;
         CT v0 = (CT)0, v1 = (CT)0, v2 = (CT)0;
         if (cnd < ncand) {
-            int o = off[0];
-#pragma unroll
-            for (int t = 1; t < NRG; t++) {
-                const bool ge = cnd >= pre[t];
-                o = ge ? off[t] : o;
-                q = ge ? t : q;
-            }
             sidx = cnd + o;
             const int ix = si0[sidx], iy = si0[Np + sidx], iz = si0[2 * Np + sidx];
-            CT w[REC];
+            // the record's x weights (whole vectors), and its y and z weights of this row
+            constexpr int VL = 16 / sizeof(CT), NXV = (W + VL - 1) / VL;
+            static_assert(NXV * VL <= REC, "x-weight vectors inside the record");
+            CT wx[NXV * VL];
             const CT *rp = rec + (int64_t)sidx * REC;
             if constexpr (sizeof(CT) == 4) {
 #pragma unroll
-                for (int k = 0; k < REC / 4; k++) {
+                for (int k = 0; k < NXV; k++) {
                     const float4 v = reinterpret_cast<const float4 *>(rp)[k];
-                    w[4 * k] = v.x, w[4 * k + 1] = v.y, w[4 * k + 2] = v.z, w[4 * k + 3] = v.w;
+                    wx[4 * k] = v.x, wx[4 * k + 1] = v.y, wx[4 * k + 2] = v.z, wx[4 * k + 3] = v.w;
                 }
             } else {
 #pragma unroll
-                for (int k = 0; k < REC / 2; k++) {
+                for (int k = 0; k < NXV; k++) {
                     const double2 v = reinterpret_cast<const double2 *>(rp)[k];
-                    w[2 * k] = v.x, w[2 * k + 1] = v.y;
+                    wx[2 * k] = v.x, wx[2 * k + 1] = v.y;
                 }
             }
             const int ky = wrap1(y - iy, G1), kz = wrap1(z - iz, G2);
             if (ky < W && kz < W) {
-                CT wy = w[W], wz = w[2 * W];
-#pragma unroll
-                for (int k = 1; k < W; k++) {
-                    wy = ky == k ? w[W + k] : wy;
-                    wz = kz == k ? w[2 * W + k] : wz;
-                }
-                int x = ix;
+                const CT wy = rp[W + ky], wz = rp[2 * W + kz];
+                const CT *p = row + ix * (2 * SLAB_BW);  // rows ix .. ix+W-1, the halo past LS-1
 #pragma unroll
                 for (int k = 0; k < W; k++) {
-                    const CT *p = row + x * (2 * SLAB_BW);
-                    v0 += w[k] * p[0];
-                    v1 += w[k] * p[CST];
-                    v2 += w[k] * p[2 * CST];
-                    x = x + 1 == LS ? 0 : x + 1;
+                    v0 += wx[k] * p[k * 2 * SLAB_BW];
+                    v1 += wx[k] * p[k * 2 * SLAB_BW + CST];
+                    v2 += wx[k] * p[k * 2 * SLAB_BW + 2 * CST];
                 }
                 const CT wyz = wy * wz;
                 v0 *= wyz, v1 *= wyz, v2 *= wyz;
