@@ -1559,9 +1559,10 @@ bool spectral_slab_supported(const vpbs_ctx *c) {
     // whole 8×8 blocks of slabs, and no home slab counted twice
     if (c->G[1] % (8 * SLAB_HY) || c->G[2] % (8 * SLAB_HZ)) return false;
     if (c->G[1] / SLAB_HY < slab_nyh(W) || c->G[2] / SLAB_HZ < slab_nzh(W)) return false;
-    // two blocks per CU: one loads x1 while the other transforms and interpolates (with one
-    // block per CU, FP64 on 768 cells, the separate kernels are faster)
-    return 2 * (f32 ? slab_lds<float>(c->G[0]) : slab_lds<double>(c->G[0])) <= c->spec_lds_max;
+    // FP32 on 768 cells: two blocks per CU (one loads x1 while the other transforms); FP64 on
+    // 768 cells: one, still faster than writing the real grid and gathering from it (7.1 against
+    // 8.1 ms at 1M nodes)
+    return (f32 ? slab_lds<float>(c->G[0]) : slab_lds<double>(c->G[0])) <= c->spec_lds_max;
 }
 
 int spectral_slab_slots(const vpbs_ctx *c) { return slab_nyh(2 * c->p.nufft_m) * slab_nzh(2 * c->p.nufft_m); }
