@@ -1368,7 +1368,10 @@ const void *slab_kernel_w(int L) {
         case 192: return (const void *)x_inverse_interp<CT, W, 192>;
         case 384: return (const void *)x_inverse_interp<CT, W, 384>;
         case 768: return (const void *)x_inverse_interp<CT, W, 768>;
-        default: return nullptr;  // 1536 and more: one block per CU at most (see spectral_slab_supported)
+        case 1536:  // FP32 only (one 156 KB block per CU; FP64 would need 312 KB)
+            if constexpr (sizeof(CT) == 4) return (const void *)x_inverse_interp<CT, W, 1536>;
+            return nullptr;
+        default: return nullptr;
     }
 }
 
@@ -1560,8 +1563,8 @@ bool spectral_slab_supported(const vpbs_ctx *c) {
     if (c->G[1] % (8 * SLAB_HY) || c->G[2] % (8 * SLAB_HZ)) return false;
     if (c->G[1] / SLAB_HY < slab_nyh(W) || c->G[2] / SLAB_HZ < slab_nzh(W)) return false;
     // FP32 on 768 cells: two blocks per CU (one loads x1 while the other transforms); FP64 on
-    // 768 cells: one, still faster than writing the real grid and gathering from it (7.1 against
-    // 8.1 ms at 1M nodes)
+    // 768 cells and FP32 on 1536: one, still faster than writing the real grid and gathering
+    // from it (FP64, 768 cells: 7.1 against 8.1 ms at 1M nodes)
     return (f32 ? slab_lds<float>(c->G[0]) : slab_lds<double>(c->G[0])) <= c->spec_lds_max;
 }
 
