@@ -13,7 +13,7 @@ and the final sum. ``value`` is node-velocities/s of the FP32-bulk path; the
 plain FP64 path is timed too, and the FP32 result is compared with the FP64
 one (the FP64 path matches the CPU oracle to ≤1e-12 in the parity tests).
 
-Prints ONE JSON line. ``--gpus`` above 1 is not supported (one GPU per box).
+Prints ONE JSON line. ``--gpus`` above 1 is not supported (the library runs on one GPU).
 """
 
 from __future__ import annotations
@@ -32,7 +32,7 @@ import __graft_entry__ as ge
 
 METRIC = "Biot–Savart node-velocities/s at 1M nodes; rel L2 error vs FP64 oracle"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-VALU_PEAK_TFS = 78.6  # MI355X vector FP64, and FP32 without packed instructions (disabled in this build)
+VALU_PEAK_TFS = 78.6  # MI355X vector FP64, and FP32 without packed instructions (compiled out of all device code)
 
 
 def log(*a):
@@ -480,7 +480,7 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU per box "
+        print(f"bench.py: --gpus {args.gpus} is not supported: the library runs on one GPU "
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
         sys.exit(2)
 

@@ -286,7 +286,7 @@ int vpbs_get_sizes(vpbs_ctx *ctx, int64_t *grid3, int64_t *cells3);
  * predicts, per node, the difference between the velocity of a cheap splitting
  * (small β: short cut-off, coarse grid) and that of a converged one (β = 4 on
  * the same grid), from local geometric features. The plain path stays the
- * judge: every learned evaluation checks the corrected velocity on a sample of
+ * reference: every learned evaluation checks the corrected velocity on a sample of
  * nodes against the converged setting and falls back to it when the error is
  * above tolerance.
  *

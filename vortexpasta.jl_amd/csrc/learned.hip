@@ -9,7 +9,7 @@
 //
 // is what the network predicts from local geometry. Both parts are computed exactly here
 // (FP64 pair sums, the kept Fourier field v̂ with the difference filter) to make training
-// targets and to judge the network at run time:
+// targets and to check the network at run time:
 //
 //   strand_records    per node in cell-sorted order (the node cell list of the pair kernel):
 //                     position, unit tangent and the offsets to the two neighbours along its
@@ -28,7 +28,7 @@
 //                     squared errors scaled to the whole field (×Np/ns) over Σ|v|² of every
 //                     node (vnorm_partial), an estimate of the whole field's relative L2 error
 //
-// The correction is judged on a stratified random sample of nodes every call: when the
+// The correction is checked on a stratified random sample of nodes every call: when the
 // relative L2 error of the corrected velocity against the converged one exceeds the
 // tolerance, the correction is rejected and Δv is evaluated exactly on every node instead.
 #include "vpbs_internal.hpp"

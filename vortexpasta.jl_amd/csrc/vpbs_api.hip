@@ -1,4 +1,4 @@
-// C ABI of libvpbs: context life cycle, point upload and the compute driver.
+// C ABI of libvpbs: context life cycle, point upload and the compute entry points.
 //
 // vpbs_compute is the device part of _compute_on_nodes!
 // (src/BiotSavart/BiotSavart.jl:515-612): the short-range task (LIA, pairs,
